@@ -1,0 +1,131 @@
+/*
+ * lapgpu.h — C-ABI of the MI355X (gfx950) solve phase of approxchol_lap.
+ *
+ * The reference's solve is the closure returned by approxchol_lapGreedy
+ * (src/approxChol.jl:1624):
+ *     f(b; tol, maxits, maxtime, verbose, pcgIts) =
+ *         pcg(la, b .- mean(b), F; tol, maxits, maxtime, verbose, pcgIts,
+ *             stag_test = params.stag_test)
+ * with F(b) = LDLsolver(ldli, b) (src/approxChol.jl:1347-1408, 1614) and pcg
+ * from src/pcg.jl:256-445.  A host (Julia via ccall, Python via ctypes, C++)
+ * builds `la = lap(a)` and the LDLinv factor, uploads both once, then calls
+ * lapgpu_solve per right-hand side.  Everything inside lapgpu_solve runs on
+ * the GPU: the Laplacian SpMV, the LDLinv forward/backward sweeps and PCG's
+ * reductions, updates and stopping rules.
+ *
+ * Conventions
+ *  - Return codes: 0 = success; < 0 = failure, with a message available from
+ *    lapgpu_last_error(ctx) (or lapgpu_last_error(NULL) when ctx creation
+ *    failed).  The Julia shim turns a nonzero code into `error(...)`, the
+ *    reference's error style (src/approxChol.jl:1533).
+ *  - Index arrays are int64 with an explicit base (1 for Julia's native
+ *    arrays, 0 for C/Python); they are converted on upload.
+ *  - All host buffers belong to the caller; uploads deep-copy.
+ *  - One host thread per context at a time.
+ */
+#ifndef LAPGPU_H
+#define LAPGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lapgpu_ctx lapgpu_ctx;
+
+/* Stop reasons, one per exit of pcg (src/pcg.jl:269-430). */
+enum lapgpu_stop {
+  LAPGPU_CONVERGED = 0,   /* nr < tol                          pcg.jl:355  */
+  LAPGPU_MAXITS = 1,      /* itcnt == maxits                   pcg.jl:288  */
+  LAPGPU_MAXTIME = 2,     /* time() - t1 > maxtime             pcg.jl:425  */
+  LAPGPU_STAG_TEST = 3,   /* stag_count > stag_test            pcg.jl:370  */
+  LAPGPU_STAG_AL_P = 4,   /* al*sum|p| < eps*sum|x|            pcg.jl:308  */
+  LAPGPU_STAG_AL_Q = 5,   /* al*sum|q| < eps*sum|r|            pcg.jl:330  */
+  LAPGPU_STAG_BETA = 6,   /* beta*sum|p| < eps*sum|z|          pcg.jl:406  */
+  LAPGPU_INF_RHO = 7,     /* isinf(rho)                        pcg.jl:385  */
+  LAPGPU_INF_BETA = 8,    /* isinf(beta)                       pcg.jl:397  */
+  LAPGPU_ZERO_RHS = 9     /* nb == 0: returns zeros            pcg.jl:269  */
+};
+
+/* lapgpu_solve flags */
+#define LAPGPU_VERBOSE 1      /* print the reference's verbose lines        */
+#define LAPGPU_NO_CENTER 2    /* skip the closure's b .- mean(b) (plain pcg) */
+
+/* Precision modes (lapgpu_set_precision). */
+#define LAPGPU_FP64 0         /* everything in FP64 (the reference's path)   */
+#define LAPGPU_FP32_INNER 1   /* SpMV + LDLinv apply in FP32; dots, alpha,
+                                 beta and a true-residual replacement every
+                                 `replace_every` iterations in FP64          */
+
+/* Create a context on HIP device `device`: one stream of default priority
+ * and an empty graph slot.  No reference counterpart (the reference has no
+ * device); this is the handle the Julia shim keeps in its closure. */
+int lapgpu_create(lapgpu_ctx** out, int device);
+
+/* Upload la = lap(a) (src/graphOps.jl:25), the operand of `mat*p`
+ * (src/pcg.jl:291).  CSC of an n-by-n symmetric matrix; since it is
+ * symmetric the device treats it as CSR.  Replaces SparseArrays' CSC SpMV. */
+int lapgpu_upload_csc(lapgpu_ctx* ctx, int64_t n, const int64_t* colptr,
+                      const int64_t* rowval, const double* nzval,
+                      int index_base);
+
+/* Upload an LDLinv (src/approxCholTypes.jl:73-79): col (n-1), colptr (n),
+ * rowval/fval (nnzF), d (n, by vertex).  Computes on the host the level
+ * schedule of the column DAG and the per-entry sweep weights, then copies
+ * the level-ordered factor to the device.  Replaces the data that
+ * `F(b) = LDLsolver(ldli, b)` (src/approxChol.jl:1614) closes over. */
+int lapgpu_upload_ldli(lapgpu_ctx* ctx, int64_t n, int64_t nnzF,
+                       const int64_t* col, const int64_t* colptr,
+                       const int64_t* rowval, const double* fval,
+                       const double* d, int index_base);
+
+/* Solve la x = b with host buffers b (n) and x_out (n): the closure
+ * f(b; tol, maxits, maxtime, verbose, pcgIts) of src/approxChol.jl:1624.
+ * Returns pcg's bestx (src/pcg.jl:444).  its_out receives itcnt (what
+ * pcgIts[1] receives, src/pcg.jl:439-441), relres_out the last recursive
+ * relative residual norm(r)/nb, stop_reason_out a lapgpu_stop value.  Any
+ * of the three out pointers may be NULL.  maxits < 0 means no limit and
+ * maxits == 0 runs no iteration (returns zeros, as the reference does);
+ * maxtime <= 0 or infinite means no limit. */
+int lapgpu_solve(lapgpu_ctx* ctx, const double* b, double* x_out, int64_t n,
+                 double tol, int64_t maxits, double maxtime, int stag_test,
+                 int flags, int64_t* its_out, double* relres_out,
+                 int* stop_reason_out);
+
+/* As lapgpu_solve, with b and x_out in device memory (resident inputs). */
+int lapgpu_solve_device(lapgpu_ctx* ctx, const double* d_b, double* d_x,
+                        int64_t n, double tol, int64_t maxits, double maxtime,
+                        int stag_test, int flags, int64_t* its_out,
+                        double* relres_out, int* stop_reason_out);
+
+/* Apply the preconditioner once: z = LDLsolver(ldli, r)
+ * (src/approxChol.jl:1347-1366), host buffers. */
+int lapgpu_apply_ldli(lapgpu_ctx* ctx, const double* r, double* z, int64_t n);
+
+/* y = la * x once (src/pcg.jl:291), host buffers. */
+int lapgpu_spmv(lapgpu_ctx* ctx, const double* x, double* y, int64_t n);
+
+/* Select the precision mode for later solves (LAPGPU_FP64 by default). */
+int lapgpu_set_precision(lapgpu_ctx* ctx, int mode, int replace_every);
+
+/* Schedule facts of the uploaded factor: number of levels, number of levels
+ * run one launch each, the first vertex position of the single-workgroup
+ * tail, nnzF. Any pointer may be NULL. */
+int lapgpu_factor_info(lapgpu_ctx* ctx, int64_t* nlevels, int64_t* nwide,
+                       int64_t* tail_start, int64_t* nnzF);
+
+/* Average device time per call of each kernel class over the last solve
+ * (ms), measured with HIP events when LAPGPU_PROFILE is set in flags. */
+#define LAPGPU_PROFILE 4
+int lapgpu_last_timings(lapgpu_ctx* ctx, double* spmv_ms, double* ldl_ms,
+                        double* vec_ms, double* total_ms, int64_t* its);
+
+const char* lapgpu_last_error(lapgpu_ctx* ctx);
+void lapgpu_destroy(lapgpu_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LAPGPU_H */

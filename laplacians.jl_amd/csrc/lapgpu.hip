@@ -1,0 +1,619 @@
+// lapgpu.hip — C-ABI host side of the gfx950 solve phase of approxchol_lap.
+//
+// One context = one HIP stream (default priority) + one uploaded graph
+// (la and its LDLinv in level order) + the PCG workspace.  lapgpu_solve runs
+// the whole of pcg (src/pcg.jl:256-445) on the device: the host only
+// enqueues kernels and polls a stop flag one iteration behind, so the GPU
+// never waits for the host.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lapgpu.h"
+#include "lapgpu_kernels.hpp"
+#include "lapgpu_plan.hpp"
+
+using namespace lapgpu;
+
+namespace {
+
+std::string g_create_err;
+
+template <typename T>
+struct DevArr {
+  T* p = nullptr;
+  size_t n = 0;
+};
+
+}  // namespace
+
+struct lapgpu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double wallclock_khz = 100000.0;
+  std::string err;
+
+  HostCsr L;
+  bool has_L = false, has_F = false;
+  Plan plan;
+  int64_t n = 0;
+  int64_t tail_width = 256;
+  int G_spmv = 8, G_ldl = 8;
+  int precision = LAPGPU_FP64, replace_every = 20;
+
+  // device graph
+  int *perm = nullptr, *Lptr = nullptr, *Lidx = nullptr;
+  double* Lval = nullptr;
+  int *Fptr = nullptr, *Fidx = nullptr, *Bptr = nullptr, *Bidx = nullptr,
+      *lvl = nullptr;
+  double *Fw = nullptr, *Bw = nullptr, *s = nullptr;
+
+  // workspace
+  double *b = nullptr, *r = nullptr, *x = nullptr, *bestx = nullptr,
+         *p = nullptr, *q = nullptr, *y = nullptr, *z = nullptr,
+         *io = nullptr, *part = nullptr;
+  DevState* st = nullptr;
+  DevState* hst = nullptr;        // pinned host mirror
+  int32_t* hflags = nullptr;      // pinned ring of stop flags
+  static constexpr int kRing = 4;
+  hipEvent_t ring_ev[kRing] = {};
+
+  // profiling
+  hipEvent_t ev[8] = {};
+  double t_spmv = 0, t_ldl = 0, t_vec = 0, t_total = 0;
+  int64_t t_its = 0;
+};
+
+namespace {
+
+int set_err(lapgpu_ctx* c, int code, const std::string& msg) {
+  if (c)
+    c->err = msg;
+  else
+    g_create_err = msg;
+  return code;
+}
+
+#define HIPC(ctx, call)                                                      \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return set_err(ctx, -10, std::string(#call) + ": " +                   \
+                                   hipGetErrorString(e_));                   \
+  } while (0)
+
+template <typename T>
+int dev_upload(lapgpu_ctx* c, T** dst, const std::vector<T>& src) {
+  if (*dst) {
+    HIPC(c, hipFree(*dst));
+    *dst = nullptr;
+  }
+  size_t bytes = sizeof(T) * (src.empty() ? 1 : src.size());
+  HIPC(c, hipMalloc((void**)dst, bytes));
+  if (!src.empty())
+    HIPC(c, hipMemcpy(*dst, src.data(), sizeof(T) * src.size(),
+                      hipMemcpyHostToDevice));
+  return 0;
+}
+
+template <typename T>
+int dev_alloc(lapgpu_ctx* c, T** dst, size_t count) {
+  if (*dst) {
+    HIPC(c, hipFree(*dst));
+    *dst = nullptr;
+  }
+  HIPC(c, hipMalloc((void**)dst, sizeof(T) * (count ? count : 1)));
+  return 0;
+}
+
+int pick_group(double avg) {
+  // lanes per row: the power of two nearest to the mean row length
+  int g = 2;
+  while (g < 64 && (double)g * 1.5 < avg) g *= 2;
+  if (g < 4) g = 4;
+  return g;
+}
+
+int grid_for(int64_t items, int per_block) {
+  int64_t b = (items + per_block - 1) / per_block;
+  return (int)(b < 1 ? 1 : b);
+}
+
+// ---- templated launch helpers ----------------------------------------------
+
+template <int G>
+void launch_spmv(lapgpu_ctx* c, const double* p, double* q) {
+  hipLaunchKernelGGL(k_spmv_dot<G>, dim3(kRedBlocks), dim3(kBlock), 0,
+                     c->stream, (int)c->n, c->Lptr, c->Lidx, c->Lval, p, q,
+                     c->part, c->st);
+}
+
+void spmv(lapgpu_ctx* c, const double* p, double* q) {
+  switch (c->G_spmv) {
+    case 4: launch_spmv<4>(c, p, q); break;
+    case 8: launch_spmv<8>(c, p, q); break;
+    case 16: launch_spmv<16>(c, p, q); break;
+    case 32: launch_spmv<32>(c, p, q); break;
+    default: launch_spmv<64>(c, p, q); break;
+  }
+}
+
+template <int G>
+void launch_sweeps(lapgpu_ctx* c, const double* rin, double* y) {
+  const Plan& P = c->plan;
+  const int64_t tl = P.tail_level;
+  for (int64_t l = 0; l < tl; ++l) {
+    const int lo = P.lvl[l], hi = P.lvl[l + 1];
+    hipLaunchKernelGGL(k_fwd_level<G>, dim3(grid_for((int64_t)(hi - lo) * G, kBlock)),
+                       dim3(kBlock), 0, c->stream, lo, hi, c->Fptr, c->Fidx,
+                       c->Fw, rin, y, c->st);
+  }
+  if (tl < P.nlev) {
+    hipLaunchKernelGGL(k_fwd_tail<G>, dim3(1), dim3(kTailThreads), 0,
+                       c->stream, (int)tl, (int)P.nlev, c->lvl, c->Fptr,
+                       c->Fidx, c->Fw, rin, y, c->st);
+    hipLaunchKernelGGL(k_bwd_tail<G>, dim3(1), dim3(kTailThreads), 0,
+                       c->stream, (int)tl, (int)P.nlev, c->lvl, c->Bptr,
+                       c->Bidx, c->Bw, c->s, y, c->st);
+  }
+  for (int64_t l = tl - 1; l >= 0; --l) {
+    const int lo = P.lvl[l], hi = P.lvl[l + 1];
+    hipLaunchKernelGGL(k_bwd_level<G>, dim3(grid_for((int64_t)(hi - lo) * G, kBlock)),
+                       dim3(kBlock), 0, c->stream, lo, hi, c->Bptr, c->Bidx,
+                       c->Bw, c->s, y, c->st);
+  }
+}
+
+// y = forward, diagonal and backward sweeps of r (no mean removal)
+void sweeps(lapgpu_ctx* c, const double* rin, double* y) {
+  switch (c->G_ldl) {
+    case 4: launch_sweeps<4>(c, rin, y); break;
+    case 8: launch_sweeps<8>(c, rin, y); break;
+    case 16: launch_sweeps<16>(c, rin, y); break;
+    case 32: launch_sweeps<32>(c, rin, y); break;
+    default: launch_sweeps<64>(c, rin, y); break;
+  }
+}
+
+// z = LDLsolver(r) = sweeps then mean removal; partial z.r etc.
+void precond(lapgpu_ctx* c, const double* rin) {
+  sweeps(c, rin, c->y);
+  const int n = (int)c->n;
+  hipLaunchKernelGGL(k_vsum, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
+                     c->y, c->part, c->st);
+  hipLaunchKernelGGL(k_scalar_mu, dim3(1), dim3(kWave), 0, c->stream, n,
+                     c->part, c->st);
+  hipLaunchKernelGGL(k_zdot, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
+                     c->y, rin, c->p, c->z, c->part, c->st);
+}
+
+int free_graph(lapgpu_ctx* c) {
+  int* ip[] = {c->perm, c->Lptr, c->Lidx, c->Fptr, c->Fidx, c->Bptr, c->Bidx, c->lvl};
+  for (int* a : ip)
+    if (a) (void)hipFree(a);
+  double* dp[] = {c->Lval, c->Fw, c->Bw, c->s, c->b, c->r, c->x, c->bestx,
+                  c->p, c->q, c->y, c->z, c->io};
+  for (double* a : dp)
+    if (a) (void)hipFree(a);
+  c->perm = c->Lptr = c->Lidx = c->Fptr = c->Fidx = c->Bptr = c->Bidx = c->lvl = nullptr;
+  c->Lval = c->Fw = c->Bw = c->s = c->b = c->r = c->x = c->bestx = c->p =
+      c->q = c->y = c->z = c->io = nullptr;
+  return 0;
+}
+
+int upload_plan(lapgpu_ctx* c) {
+  const Plan& P = c->plan;
+  const int64_t n = P.n;
+  int rc;
+  if ((rc = dev_upload(c, &c->perm, P.perm))) return rc;
+  if ((rc = dev_upload(c, &c->Lptr, P.Lptr))) return rc;
+  if ((rc = dev_upload(c, &c->Lidx, P.Lidx))) return rc;
+  if ((rc = dev_upload(c, &c->Lval, P.Lval))) return rc;
+  if ((rc = dev_upload(c, &c->Fptr, P.Fptr))) return rc;
+  if ((rc = dev_upload(c, &c->Fidx, P.Fidx))) return rc;
+  if ((rc = dev_upload(c, &c->Fw, P.Fw))) return rc;
+  if ((rc = dev_upload(c, &c->Bptr, P.Bptr))) return rc;
+  if ((rc = dev_upload(c, &c->Bidx, P.Bidx))) return rc;
+  if ((rc = dev_upload(c, &c->Bw, P.Bw))) return rc;
+  if ((rc = dev_upload(c, &c->s, P.s))) return rc;
+  if ((rc = dev_upload(c, &c->lvl, P.lvl))) return rc;
+  double** ws[] = {&c->b, &c->r, &c->x, &c->bestx, &c->p, &c->q, &c->y, &c->z, &c->io};
+  for (double** w : ws)
+    if ((rc = dev_alloc(c, w, (size_t)n))) return rc;
+  c->n = n;
+  const double avgL = (double)P.Lidx.size() / (double)n;
+  const double avgF = (double)P.nnzF / (double)n;
+  c->G_spmv = pick_group(avgL);
+  c->G_ldl = pick_group(avgF);
+  if (const char* e = getenv("LAPGPU_G_SPMV")) c->G_spmv = atoi(e);
+  if (const char* e = getenv("LAPGPU_G_LDL")) c->G_ldl = atoi(e);
+  return 0;
+}
+
+// Identity plan: la only, no factor (lapgpu_spmv without an LDLinv).
+void identity_plan(lapgpu_ctx* c) {
+  Plan& P = c->plan;
+  const int64_t n = c->L.n;
+  P = Plan();
+  P.n = n;
+  P.nnzF = 0;
+  P.perm.resize(n);
+  P.iperm.resize(n);
+  for (int64_t i = 0; i < n; ++i) P.perm[i] = P.iperm[i] = (int32_t)i;
+  P.Lptr = c->L.ptr;
+  P.Lidx = c->L.idx;
+  P.Lval = c->L.val;
+  P.Fptr.assign(n + 1, 0);
+  P.Bptr.assign(n + 1, 0);
+  P.s.assign(n, 1.0);
+  P.nlev = 1;
+  P.lvl = {0, (int32_t)n};
+  P.tail_level = 1;
+}
+
+void init_state(lapgpu_ctx* c, double tol, int64_t maxits, double maxtime,
+                int stag_test, int flags) {
+  DevState& h = *c->hst;
+  std::memset(&h, 0, sizeof(DevState));
+  h.tol = tol;
+  h.maxits = maxits;
+  h.stag_test = stag_test;
+  h.stag_fac = 1.0 - 1.0 / (double)stag_test;  // -Inf for stag_test == 0
+  h.center = (flags & LAPGPU_NO_CENTER) ? 0 : 1;
+  h.has_maxtime = (maxtime > 0 && std::isfinite(maxtime)) ? 1 : 0;
+  h.maxticks = h.has_maxtime ? (uint64_t)(maxtime * c->wallclock_khz * 1e3) : 0;
+  h.bestnr = 1.0;
+  h.nr = 1.0;
+}
+
+const char* stop_message(int reason) {
+  switch (reason) {
+    case 0: return "PCG stopped due to small nr (norm(r)/nb)";
+    case 2: return "PCG New stopped at maxtime.";
+    case 3: return "PCG Stopped by stagnation test";
+    case 4: return "PCG: Stopped due to stagnation: al*sum(abs.(p)) < eps(Tval)*sum(abs.(x))";
+    case 5: return "PCG: Stopped due to stagnation: al*sum(abs.(q)) < eps(Tval)*sum(abs.(r))";
+    case 6: return "PCG: Stopped due to stagnation: beta*sum(abs.(p)) < eps(Tval)*sum(abs.(z))";
+    case 7: return "PCG Stopped due to large rho";
+    case 8: return "PCG Stopped due to large beta";
+    default: return "";
+  }
+}
+
+// one PCG iteration (src/pcg.jl:288-431)
+void enqueue_iteration(lapgpu_ctx* c, bool prof) {
+  const int n = (int)c->n;
+  if (prof) (void)hipEventRecord(c->ev[0], c->stream);
+  spmv(c, c->p, c->q);
+  if (prof) (void)hipEventRecord(c->ev[1], c->stream);
+  hipLaunchKernelGGL(k_scalar_al, dim3(1), dim3(kWave), 0, c->stream, c->part, c->st);
+  hipLaunchKernelGGL(k_update, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
+                     c->x, c->bestx, c->p, c->q, c->r, c->part, c->st);
+  hipLaunchKernelGGL(k_scalar_nr, dim3(1), dim3(kWave), 0, c->stream, c->part, c->st);
+  if (prof) (void)hipEventRecord(c->ev[2], c->stream);
+  sweeps(c, c->r, c->y);
+  if (prof) (void)hipEventRecord(c->ev[3], c->stream);
+  hipLaunchKernelGGL(k_vsum, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
+                     c->y, c->part, c->st);
+  hipLaunchKernelGGL(k_scalar_mu, dim3(1), dim3(kWave), 0, c->stream, n, c->part, c->st);
+  hipLaunchKernelGGL(k_zdot, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
+                     c->y, c->r, c->p, c->z, c->part, c->st);
+  hipLaunchKernelGGL(k_scalar_rho, dim3(1), dim3(kWave), 0, c->stream, c->part, c->st);
+  hipLaunchKernelGGL(k_pupdate, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
+                     c->z, c->p, c->st);
+  if (prof) (void)hipEventRecord(c->ev[4], c->stream);
+}
+
+int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
+                 int64_t maxits, double maxtime, int stag_test, int flags,
+                 int64_t* its_out, double* relres_out, int* stop_out) {
+  if (!c->has_F) return set_err(c, -2, "solve: upload the Laplacian and LDLinv first");
+  const auto t_start = std::chrono::steady_clock::now();
+  const bool prof = (flags & LAPGPU_PROFILE) != 0;
+  const bool verbose = (flags & LAPGPU_VERBOSE) != 0;
+  const int n = (int)c->n;
+  init_state(c, tol, maxits, maxtime, stag_test, flags);
+  HIPC(c, hipMemcpyAsync(c->st, c->hst, sizeof(DevState), hipMemcpyHostToDevice, c->stream));
+
+  // ---- setup: r = b - mean(b), z = pre(r), p = z, rho = r.z ---------------
+  hipLaunchKernelGGL(k_gather_perm, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
+                     c->perm, d_b, c->b, c->part);
+  hipLaunchKernelGGL(k_scalar_mean, dim3(1), dim3(kWave), 0, c->stream, n, c->part, c->st);
+  hipLaunchKernelGGL(k_center, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n, c->b,
+                     c->r, c->x, c->bestx, c->part, c->st);
+  hipLaunchKernelGGL(k_scalar_nb, dim3(1), dim3(kWave), 0, c->stream, c->part, c->st);
+  precond(c, c->r);
+  hipLaunchKernelGGL(k_scalar_rho0, dim3(1), dim3(kWave), 0, c->stream, c->part, c->st);
+  hipLaunchKernelGGL(k_copy, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n, c->z,
+                     c->p, c->st);
+  HIPC(c, hipGetLastError());
+
+  // the setup may already have stopped (b == 0, maxits == 0)
+  HIPC(c, hipMemcpyAsync(c->hst, c->st, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  if (verbose) {
+    printf("<b, LDLi b> is %.17g\n", c->hst->nb == 0.0 ? 0.0 : c->hst->rho0);
+    fflush(stdout);
+  }
+
+  // ---- iterations: poll the stop flag one iteration behind ---------------
+  double acc_spmv = 0, acc_ldl = 0, acc_vec = 0;
+  int64_t enq = 0, polled = 0;
+  bool done = c->hst->stopped != 0;
+  while (!done) {
+    enqueue_iteration(c, prof);
+    const int slot = (int)(enq % lapgpu_ctx::kRing);
+    HIPC(c, hipMemcpyAsync(&c->hflags[slot], &c->st->stopped, sizeof(int32_t),
+                           hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipEventRecord(c->ring_ev[slot], c->stream));
+    ++enq;
+    if (prof) {
+      HIPC(c, hipEventSynchronize(c->ev[4]));
+      float a = 0, b2 = 0, d = 0, e = 0;
+      (void)hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
+      (void)hipEventElapsedTime(&b2, c->ev[2], c->ev[3]);
+      (void)hipEventElapsedTime(&d, c->ev[1], c->ev[2]);
+      (void)hipEventElapsedTime(&e, c->ev[3], c->ev[4]);
+      acc_spmv += a;
+      acc_ldl += b2;
+      acc_vec += d + e;
+    }
+    // keep one iteration in flight: wait for the flags of all older ones
+    while (!done && polled < enq - 1) {
+      const int ps = (int)(polled % lapgpu_ctx::kRing);
+      HIPC(c, hipEventSynchronize(c->ring_ev[ps]));
+      ++polled;
+      if (c->hflags[ps]) done = true;
+    }
+  }
+  hipLaunchKernelGGL(k_final_copy, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
+                     c->x, c->bestx, c->st);
+  hipLaunchKernelGGL(k_scatter_perm, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
+                     c->perm, c->bestx, d_x);
+  HIPC(c, hipMemcpyAsync(c->hst, c->st, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, hipGetLastError());
+  const DevState& h = *c->hst;
+  const double secs =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  if (its_out) *its_out = h.itcnt;
+  if (relres_out) *relres_out = h.nr;
+  if (stop_out) *stop_out = h.stop_reason;
+  if (prof && h.itcnt > 0) {
+    c->t_spmv = acc_spmv / (double)enq;
+    c->t_ldl = acc_ldl / (double)enq;
+    c->t_vec = acc_vec / (double)enq;
+  }
+  c->t_total = secs * 1e3;
+  c->t_its = h.itcnt;
+  if (h.stop_reason == 3) {
+    // printed unconditionally by the reference (src/pcg.jl:378)
+    printf("PCG Stopped by stagnation test %d\n", stag_test);
+  } else if (verbose && h.stop_reason != 1 && h.stop_reason != 9) {
+    if (h.stop_reason == 0)
+      printf("%s = %.17g\nnb = %.17g\n", stop_message(0), h.nr, h.nb);
+    else
+      printf("%s\n", stop_message(h.stop_reason));
+  }
+  if (verbose && h.stop_reason != 9) {
+    printf("PCG stopped after: %.3f seconds and %lld iterations with relative error %.17g.\n",
+           secs, (long long)h.itcnt, h.nr);
+    printf("rho = %.17g\n", h.rho);
+  }
+  fflush(stdout);
+  c->err.clear();
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lapgpu_create(lapgpu_ctx** out, int device) {
+  if (!out) return set_err(nullptr, -1, "lapgpu_create: null output");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    return set_err(nullptr, -10, "lapgpu_create: no HIP device visible");
+  if (device < 0 || device >= count)
+    return set_err(nullptr, -1, "lapgpu_create: device index out of range");
+  lapgpu_ctx* c = new lapgpu_ctx();
+  c->device = device;
+  auto fail = [&](const char* what, hipError_t e) {
+    g_create_err = std::string(what) + ": " + hipGetErrorString(e);
+    delete c;
+    return -10;
+  };
+  hipError_t e;
+  if ((e = hipSetDevice(device)) != hipSuccess) return fail("hipSetDevice", e);
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+    return fail("hipStreamCreate", e);
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+    c->wallclock_khz = (double)khz;
+  if ((e = hipMalloc((void**)&c->part, sizeof(double) * kNumSlots * kRedBlocks)) != hipSuccess)
+    return fail("hipMalloc", e);
+  if ((e = hipMalloc((void**)&c->st, sizeof(DevState))) != hipSuccess) return fail("hipMalloc", e);
+  if ((e = hipHostMalloc((void**)&c->hst, sizeof(DevState), hipHostMallocDefault)) != hipSuccess)
+    return fail("hipHostMalloc", e);
+  if ((e = hipHostMalloc((void**)&c->hflags, sizeof(int32_t) * lapgpu_ctx::kRing,
+                         hipHostMallocDefault)) != hipSuccess)
+    return fail("hipHostMalloc", e);
+  for (auto& ev : c->ring_ev)
+    if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess)
+      return fail("hipEventCreate", e);
+  for (auto& ev : c->ev)
+    if ((e = hipEventCreate(&ev)) != hipSuccess) return fail("hipEventCreate", e);
+  if (const char* tw = getenv("LAPGPU_TAIL_WIDTH")) c->tail_width = atoll(tw);
+  *out = c;
+  return 0;
+}
+
+void lapgpu_destroy(lapgpu_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_graph(c);
+  if (c->part) (void)hipFree(c->part);
+  if (c->st) (void)hipFree(c->st);
+  if (c->hst) (void)hipHostFree(c->hst);
+  if (c->hflags) (void)hipHostFree(c->hflags);
+  for (auto& ev : c->ring_ev)
+    if (ev) (void)hipEventDestroy(ev);
+  for (auto& ev : c->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* lapgpu_last_error(lapgpu_ctx* c) {
+  return c ? c->err.c_str() : g_create_err.c_str();
+}
+
+int lapgpu_upload_csc(lapgpu_ctx* c, int64_t n, const int64_t* colptr,
+                      const int64_t* rowval, const double* nzval,
+                      int index_base) {
+  if (!c) return -1;
+  HIPC(c, hipSetDevice(c->device));
+  std::string err;
+  HostCsr L;
+  if (!csc_to_csr(n, colptr, rowval, nzval, index_base, L, err)) return set_err(c, -1, err);
+  c->L = std::move(L);
+  c->has_L = true;
+  c->has_F = false;
+  identity_plan(c);
+  int rc = upload_plan(c);
+  if (rc) return rc;
+  c->err.clear();
+  return 0;
+}
+
+int lapgpu_upload_ldli(lapgpu_ctx* c, int64_t n, int64_t nnzF,
+                       const int64_t* col, const int64_t* colptr,
+                       const int64_t* rowval, const double* fval,
+                       const double* d, int index_base) {
+  if (!c) return -1;
+  if (!c->has_L) return set_err(c, -2, "upload_ldli: upload the Laplacian first");
+  if (n != c->L.n) return set_err(c, -1, "upload_ldli: n differs from the Laplacian's");
+  if (index_base != 0 && index_base != 1)
+    return set_err(c, -1, "upload_ldli: index_base must be 0 or 1");
+  HIPC(c, hipSetDevice(c->device));
+  std::string err;
+  Plan P;
+  if (!build_plan(c->L, nnzF, col, colptr, rowval, fval, d, index_base, c->tail_width, P, err))
+    return set_err(c, -1, err);
+  c->plan = std::move(P);
+  int rc = upload_plan(c);
+  if (rc) return rc;
+  c->has_F = true;
+  c->err.clear();
+  return 0;
+}
+
+int lapgpu_solve_device(lapgpu_ctx* c, const double* d_b, double* d_x,
+                        int64_t n, double tol, int64_t maxits, double maxtime,
+                        int stag_test, int flags, int64_t* its_out,
+                        double* relres_out, int* stop_out) {
+  if (!c) return -1;
+  if (n != c->n) return set_err(c, -1, "solve: length of b differs from n");
+  if (!d_b || !d_x) return set_err(c, -1, "solve: null vector");
+  HIPC(c, hipSetDevice(c->device));
+  return solve_device(c, d_b, d_x, tol, maxits, maxtime, stag_test, flags, its_out,
+                      relres_out, stop_out);
+}
+
+int lapgpu_solve(lapgpu_ctx* c, const double* b, double* x_out, int64_t n,
+                 double tol, int64_t maxits, double maxtime, int stag_test,
+                 int flags, int64_t* its_out, double* relres_out,
+                 int* stop_out) {
+  if (!c) return -1;
+  if (n != c->n) return set_err(c, -1, "solve: length of b differs from n");
+  if (!b || !x_out) return set_err(c, -1, "solve: null vector");
+  if (!c->has_F) return set_err(c, -2, "solve: upload the Laplacian and LDLinv first");
+  HIPC(c, hipSetDevice(c->device));
+  HIPC(c, hipMemcpyAsync(c->io, b, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  int rc = solve_device(c, c->io, c->io, tol, maxits, maxtime, stag_test, flags, its_out,
+                        relres_out, stop_out);
+  if (rc) return rc;
+  HIPC(c, hipMemcpyAsync(x_out, c->io, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int lapgpu_apply_ldli(lapgpu_ctx* c, const double* rin, double* zout, int64_t n) {
+  if (!c) return -1;
+  if (!c->has_F) return set_err(c, -2, "apply_ldli: upload the Laplacian and LDLinv first");
+  if (n != c->n || !rin || !zout) return set_err(c, -1, "apply_ldli: bad arguments");
+  HIPC(c, hipSetDevice(c->device));
+  init_state(c, 0.0, 1, 0.0, 0, LAPGPU_NO_CENTER);
+  HIPC(c, hipMemcpyAsync(c->st, c->hst, sizeof(DevState), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemcpyAsync(c->io, rin, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_gather_perm, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, (int)n,
+                     c->perm, c->io, c->r, c->part);
+  precond(c, c->r);
+  hipLaunchKernelGGL(k_scatter_perm, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, (int)n,
+                     c->perm, c->z, c->io);
+  HIPC(c, hipGetLastError());
+  HIPC(c, hipMemcpyAsync(zout, c->io, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  c->err.clear();
+  return 0;
+}
+
+int lapgpu_spmv(lapgpu_ctx* c, const double* xin, double* yout, int64_t n) {
+  if (!c) return -1;
+  if (!c->has_L) return set_err(c, -2, "spmv: upload the Laplacian first");
+  if (n != c->n || !xin || !yout) return set_err(c, -1, "spmv: bad arguments");
+  HIPC(c, hipSetDevice(c->device));
+  init_state(c, 0.0, 1, 0.0, 0, LAPGPU_NO_CENTER);
+  HIPC(c, hipMemcpyAsync(c->st, c->hst, sizeof(DevState), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemcpyAsync(c->io, xin, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_gather_perm, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, (int)n,
+                     c->perm, c->io, c->p, c->part);
+  spmv(c, c->p, c->q);
+  hipLaunchKernelGGL(k_scatter_perm, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, (int)n,
+                     c->perm, c->q, c->io);
+  HIPC(c, hipGetLastError());
+  HIPC(c, hipMemcpyAsync(yout, c->io, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  c->err.clear();
+  return 0;
+}
+
+int lapgpu_set_precision(lapgpu_ctx* c, int mode, int replace_every) {
+  if (!c) return -1;
+  if (mode != LAPGPU_FP64) return set_err(c, -1, "set_precision: only LAPGPU_FP64 is built");
+  c->precision = mode;
+  c->replace_every = replace_every > 0 ? replace_every : 20;
+  return 0;
+}
+
+int lapgpu_factor_info(lapgpu_ctx* c, int64_t* nlevels, int64_t* nwide,
+                       int64_t* tail_start, int64_t* nnzF) {
+  if (!c) return -1;
+  if (!c->has_F) return set_err(c, -2, "factor_info: no LDLinv uploaded");
+  if (nlevels) *nlevels = c->plan.nlev;
+  if (nwide) *nwide = c->plan.tail_level;
+  if (tail_start) *tail_start = c->plan.lvl[c->plan.tail_level];
+  if (nnzF) *nnzF = c->plan.nnzF;
+  return 0;
+}
+
+int lapgpu_last_timings(lapgpu_ctx* c, double* spmv_ms, double* ldl_ms,
+                        double* vec_ms, double* total_ms, int64_t* its) {
+  if (!c) return -1;
+  if (spmv_ms) *spmv_ms = c->t_spmv;
+  if (ldl_ms) *ldl_ms = c->t_ldl;
+  if (vec_ms) *vec_ms = c->t_vec;
+  if (total_ms) *total_ms = c->t_total;
+  if (its) *its = c->t_its;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,245 @@
+// lapgpu_plan.cpp — host preprocessing for the device solve.
+//
+// The LDLinv sweeps of the reference (forward!/backward!,
+// src/approxChol.jl:1369-1408) walk the n-1 columns strictly in elimination
+// order.  Written in closed form (own derivation) with, per column c and its
+// entries k = 1..K (K the sentinel, fval = 1):
+//     w_k = f_k * prod_{l<k} (1 - f_l)   for k < K,     w_K = P_c,
+//     P_c = prod_{l<K} (1 - f_l),
+// forward! computes y0[v] = b[v] + sum_{columns c that hold row v} w * y0[c]
+// and leaves y[c] = P_c y0[c]; the diagonal step divides by d; backward!
+// computes yF[c] = P_c * (y[c] + yF[j_K]) + sum_{k<K} w_k yF[j_k].  Hence
+//     yF[c] = s_c y0[c] + sum_k w_k yF[j_k],  s_c = P_c^2 / d_c (d_c != 0),
+// and the never-eliminated root keeps yF = y0.  Both sweeps become gathers
+// ("pull" form): no atomics, a fixed summation order, and each depends only
+// on vertices of lower (forward) or higher (backward) DAG level.  Vertices
+// are renumbered by forward level so every level is a contiguous range.
+#include "lapgpu_plan.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+namespace lapgpu {
+
+static constexpr int64_t kMaxI32 = 2147483647LL;
+
+bool csc_to_csr(int64_t n, const int64_t* colptr, const int64_t* rowval,
+                const double* nzval, int base, HostCsr& out,
+                std::string& err) {
+  if (n < 1 || n >= kMaxI32) {
+    err = "upload_csc: n out of range";
+    return false;
+  }
+  if (base != 0 && base != 1) {
+    err = "upload_csc: index_base must be 0 or 1";
+    return false;
+  }
+  if (!colptr || !rowval || !nzval) {
+    err = "upload_csc: null array";
+    return false;
+  }
+  const int64_t nnz = colptr[n] - base;
+  if (colptr[0] != base || nnz < 0 || nnz >= kMaxI32) {
+    err = "upload_csc: malformed colptr or nnz >= 2^31";
+    return false;
+  }
+  out.n = n;
+  out.ptr.assign(n + 1, 0);
+  for (int64_t j = 0; j < n; ++j) {
+    if (colptr[j + 1] < colptr[j]) {
+      err = "upload_csc: colptr not monotone";
+      return false;
+    }
+  }
+  // Symmetric: column j of the CSC is row j of the CSR.
+  out.idx.resize(nnz);
+  out.val.resize(nnz);
+  for (int64_t j = 0; j <= n; ++j) out.ptr[j] = (int32_t)(colptr[j] - base);
+  for (int64_t k = 0; k < nnz; ++k) {
+    const int64_t r = rowval[k] - base;
+    if (r < 0 || r >= n) {
+      err = "upload_csc: row index out of range";
+      return false;
+    }
+    out.idx[k] = (int32_t)r;
+    out.val[k] = nzval[k];
+  }
+  return true;
+}
+
+bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
+                const int64_t* colptr, const int64_t* rowval,
+                const double* fval, const double* d, int base,
+                int64_t tail_width, Plan& P, std::string& err) {
+  const int64_t n = L.n;
+  if (n < 1) {
+    err = "upload_ldli: upload the Laplacian first";
+    return false;
+  }
+  if (nnzF < 0 || nnzF >= kMaxI32) {
+    err = "upload_ldli: nnzF out of range";
+    return false;
+  }
+  if (n > 1 && (!col || !colptr || !rowval || !fval || !d)) {
+    err = "upload_ldli: null array";
+    return false;
+  }
+  // ---- validate the factor and find the elimination position -------------
+  std::vector<int32_t> elim(n, -1);
+  for (int64_t ii = 0; ii < n - 1; ++ii) {
+    const int64_t v = col[ii] - base;
+    if (v < 0 || v >= n || elim[v] != -1) {
+      err = "upload_ldli: col is not a list of distinct vertices";
+      return false;
+    }
+    elim[v] = (int32_t)ii;
+  }
+  int64_t root = -1;
+  for (int64_t v = 0; v < n; ++v)
+    if (elim[v] < 0) {
+      root = v;
+      elim[v] = (int32_t)(n - 1);
+    }
+  if (n > 1) {
+    if (colptr[0] != base || colptr[n - 1] - base != nnzF) {
+      err = "upload_ldli: colptr does not span nnzF entries";
+      return false;
+    }
+    for (int64_t ii = 0; ii < n - 1; ++ii) {
+      if (colptr[ii + 1] <= colptr[ii]) {
+        err = "upload_ldli: empty or non-monotone column";
+        return false;
+      }
+      for (int64_t k = colptr[ii] - base; k < colptr[ii + 1] - base; ++k) {
+        const int64_t j = rowval[k] - base;
+        if (j < 0 || j >= n || elim[j] <= ii) {
+          err = "upload_ldli: a row of a column is not eliminated later";
+          return false;
+        }
+      }
+    }
+  }
+  (void)root;
+  P.n = n;
+  P.nnzF = nnzF;
+
+  // ---- per-entry weights and per-column scale -----------------------------
+  std::vector<double> w(nnzF);
+  std::vector<double> Pc(n, 1.0);
+  for (int64_t ii = 0; ii < n - 1; ++ii) {
+    const int64_t k0 = colptr[ii] - base, k1 = colptr[ii + 1] - base - 1;
+    double prod = 1.0;
+    for (int64_t k = k0; k < k1; ++k) {
+      w[k] = fval[k] * prod;
+      prod *= (1.0 - fval[k]);
+    }
+    w[k1] = prod;
+    Pc[col[ii] - base] = prod;
+  }
+
+  // ---- forward levels (longest path from a source) ------------------------
+  std::vector<int32_t> lev(n, 0);
+  for (int64_t ii = 0; ii < n - 1; ++ii) {
+    const int64_t v = col[ii] - base;
+    const int32_t l = lev[v] + 1;
+    for (int64_t k = colptr[ii] - base; k < colptr[ii + 1] - base; ++k) {
+      const int64_t j = rowval[k] - base;
+      if (lev[j] < l) lev[j] = l;
+    }
+  }
+  int32_t maxlev = 0;
+  for (int64_t v = 0; v < n; ++v) maxlev = std::max(maxlev, lev[v]);
+  P.nlev = (int64_t)maxlev + 1;
+  P.lvl.assign(P.nlev + 1, 0);
+  for (int64_t v = 0; v < n; ++v) P.lvl[lev[v] + 1]++;
+  for (int64_t l = 0; l < P.nlev; ++l) P.lvl[l + 1] += P.lvl[l];
+
+  P.perm.assign(n, 0);
+  P.iperm.assign(n, 0);
+  {
+    std::vector<int32_t> cur(P.lvl.begin(), P.lvl.end() - 1);
+    for (int64_t v = 0; v < n; ++v) {
+      const int32_t pos = cur[lev[v]]++;
+      P.perm[pos] = (int32_t)v;
+      P.iperm[v] = pos;
+    }
+  }
+
+  // tail: the maximal suffix of levels narrower than tail_width
+  {
+    int64_t t = P.nlev;
+    while (t > 0 && (P.lvl[t] - P.lvl[t - 1]) < tail_width) --t;
+    P.tail_level = t;
+  }
+
+  // ---- permuted Laplacian --------------------------------------------------
+  const int64_t nnzL = (int64_t)L.idx.size();
+  P.Lptr.assign(n + 1, 0);
+  P.Lidx.resize(nnzL);
+  P.Lval.resize(nnzL);
+  for (int64_t pos = 0; pos < n; ++pos) {
+    const int32_t v = P.perm[pos];
+    P.Lptr[pos + 1] = P.Lptr[pos] + (L.ptr[v + 1] - L.ptr[v]);
+  }
+  {
+    std::vector<std::pair<int32_t, double>> row;
+    for (int64_t pos = 0; pos < n; ++pos) {
+      const int32_t v = P.perm[pos];
+      row.clear();
+      for (int32_t k = L.ptr[v]; k < L.ptr[v + 1]; ++k)
+        row.emplace_back(P.iperm[L.idx[k]], L.val[k]);
+      std::sort(row.begin(), row.end(),
+                [](const auto& a, const auto& b) { return a.first < b.first; });
+      int32_t o = P.Lptr[pos];
+      for (auto& e : row) {
+        P.Lidx[o] = e.first;
+        P.Lval[o] = e.second;
+        ++o;
+      }
+    }
+  }
+
+  // ---- backward rows: column of v, in column order ------------------------
+  P.Bptr.assign(n + 1, 0);
+  P.Bidx.resize(nnzF);
+  P.Bw.resize(nnzF);
+  P.s.assign(n, 1.0);
+  for (int64_t ii = 0; ii < n - 1; ++ii) {
+    const int64_t v = col[ii] - base;
+    P.Bptr[P.iperm[v] + 1] = (int32_t)(colptr[ii + 1] - colptr[ii]);
+  }
+  for (int64_t pos = 0; pos < n; ++pos) P.Bptr[pos + 1] += P.Bptr[pos];
+  for (int64_t ii = 0; ii < n - 1; ++ii) {
+    const int64_t v = col[ii] - base;
+    int32_t o = P.Bptr[P.iperm[v]];
+    for (int64_t k = colptr[ii] - base; k < colptr[ii + 1] - base; ++k) {
+      P.Bidx[o] = P.iperm[rowval[k] - base];
+      P.Bw[o] = w[k];
+      ++o;
+    }
+    const double dv = d[v];
+    P.s[P.iperm[v]] = dv != 0.0 ? Pc[v] * Pc[v] / dv : Pc[v] * Pc[v];
+  }
+
+  // ---- forward rows: in-edges of v, in elimination order of the source ----
+  P.Fptr.assign(n + 1, 0);
+  P.Fidx.resize(nnzF);
+  P.Fw.resize(nnzF);
+  for (int64_t k = 0; k < nnzF; ++k) P.Fptr[P.iperm[rowval[k] - base] + 1]++;
+  for (int64_t pos = 0; pos < n; ++pos) P.Fptr[pos + 1] += P.Fptr[pos];
+  {
+    std::vector<int32_t> cur(P.Fptr.begin(), P.Fptr.end() - 1);
+    for (int64_t ii = 0; ii < n - 1; ++ii) {
+      const int32_t src = P.iperm[col[ii] - base];
+      for (int64_t k = colptr[ii] - base; k < colptr[ii + 1] - base; ++k) {
+        const int32_t o = cur[P.iperm[rowval[k] - base]]++;
+        P.Fidx[o] = src;
+        P.Fw[o] = w[k];
+      }
+    }
+  }
+  return true;
+}
+
+}  // namespace lapgpu

@@ -1,0 +1,20 @@
+"""lapgpu — the MI355X (gfx950) solve phase of Laplacians.jl's approxchol_lap.
+
+Public surface (mirrors the reference's names):
+  approxchol_lap_gpu(a; tol, maxits, maxtime, verbose, pcgIts, params) -> f(b)
+  approxchol_sddm_gpu(M; ...) -> f(b)
+  ApproxCholParams, lap, build_ldli (host factor), Context (raw C-ABI handle)
+"""
+
+from .factor import LDLinv, build_ldli, load_ldli, save_ldli  # noqa: F401
+from .graphs import lap  # noqa: F401
+from ._native import Context, LapGPUError  # noqa: F401
+from .solver import (  # noqa: F401
+    ApproxCholParams,
+    approxchol_lap1_gpu,
+    approxchol_lap_gpu,
+    approxchol_sddm_gpu,
+    block_solver,
+    lap_wrap_components,
+    sddm_wrap_lap,
+)

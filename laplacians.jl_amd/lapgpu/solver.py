@@ -1,0 +1,222 @@
+"""The drop-in: `approxchol_lap_gpu(a; tol, maxits, maxtime, verbose, pcgIts,
+params)` returning the solver closure `f(b; ...)`, with the same names,
+argument meaning and error behaviour as the reference's `approxchol_lap`
+(src/approxChol.jl:1522-1545) and its wrappers (src/solverInterface.jl).
+
+The build of the factor stays on the host (here liblapfactor, the C++
+restatement of `approxChol(LLmatp(a))`, or a caller-supplied LDLinv from the
+reference's Julia); every solve `f(b)` runs entirely on the GPU through
+liblapgpu (include/lapgpu.h).  There is no CPU solve path.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+import warnings
+from dataclasses import dataclass
+from typing import Callable
+
+import numpy as np
+import scipy.sparse as sp
+from scipy.sparse.csgraph import connected_components
+
+from . import _native
+from .factor import LDLinv, build_ldli
+from .graphs import lap
+
+__all__ = [
+    "ApproxCholParams",
+    "approxchol_lap_gpu",
+    "approxchol_lap1_gpu",
+    "lap_wrap_components",
+    "block_solver",
+    "null_solver",
+    "approxchol_sddm_gpu",
+    "sddm_wrap_lap",
+]
+
+
+@dataclass
+class ApproxCholParams:
+    """ApproxCholParams (src/approxChol.jl:80-90); default (:deg, 5, 0, 0)."""
+
+    order: str = "deg"
+    stag_test: int = 5
+    split: int = 0
+    merge: int = 0
+
+
+def null_solver(b, tol=1e-6, maxits=1000, maxtime=math.inf, verbose=False, pcgIts=None):
+    """nullSolver: zeros for singleton components (src/solverInterface.jl)."""
+    return np.zeros(len(b))
+
+
+def _set_its(pcgIts, its):
+    if pcgIts is not None and len(pcgIts) > 0:
+        pcgIts[0] = its
+
+
+class GpuLapSolver:
+    """The solve closure of approxchol_lapGreedy (src/approxChol.jl:1624):
+    f(b; tol, maxits, maxtime, verbose, pcgIts) = pcg(la, b .- mean(b), F; ...)
+    with keyword defaults captured at construction."""
+
+    def __init__(self, ctx: _native.Context, ldli: LDLinv, tol, maxits, maxtime, verbose, pcgIts, stag_test):
+        self.ctx = ctx
+        self.ldli = ldli
+        self.defaults = dict(tol=tol, maxits=maxits, maxtime=maxtime, verbose=verbose, pcgIts=pcgIts)
+        self.stag_test = stag_test
+        self.last = None
+
+    def __call__(self, b, tol=None, maxits=None, maxtime=None, verbose=None, pcgIts=None):
+        d = self.defaults
+        tol = d["tol"] if tol is None else tol
+        maxits = d["maxits"] if maxits is None else maxits
+        maxtime = d["maxtime"] if maxtime is None else maxtime
+        verbose = d["verbose"] if verbose is None else verbose
+        pcgIts = d["pcgIts"] if pcgIts is None else pcgIts
+        b = np.asarray(b, dtype=np.float64)
+        if b.shape != (self.ctx.n,):
+            raise ValueError(f"b has shape {b.shape}, expected ({self.ctx.n},)")
+        mi = -1 if (maxits is None or (isinstance(maxits, float) and math.isinf(maxits))) else int(maxits)
+        flags = _native.VERBOSE if verbose else 0
+        x, info = self.ctx.solve(b, tol=tol, maxits=mi, maxtime=float(maxtime), stag_test=self.stag_test, flags=flags)
+        self.last = info
+        _set_its(pcgIts, info["its"])
+        return x
+
+
+def approxchol_lap1_gpu(a, tol=1e-6, maxits=1000, maxtime=math.inf, verbose=False, pcgIts=None,
+                        params: ApproxCholParams | None = None, seed: int = 1, device: int = 0,
+                        ldli: LDLinv | None = None):
+    """approxchol_lap1 -> approxchol_lapGreedy (src/approxChol.jl:1581-1626,
+    1791-1851) for a connected graph: la = lap(a), the LDLinv on the host,
+    both uploaded once; returns the GPU solve closure."""
+    params = params or ApproxCholParams()
+    if params.order != "deg" or params.split >= 1:
+        raise NotImplementedError("only ApproxCholParams(:deg) factors are built on this host; "
+                                  "pass ldli= to use a factor built elsewhere")
+    t1 = time.time()
+    a = sp.csc_matrix(a)
+    la = lap(a)
+    if ldli is None:
+        ldli = build_ldli(a, seed=seed)
+    ctx = _native.Context(device)
+    ctx.upload_lap(la)
+    ctx.upload_ldli(ldli)
+    if verbose:
+        print(f"Using greedy degree ordering. Factorization time: {time.time() - t1}")
+        print(f"Factorization size: {2 * ldli.nnz}")
+        print(f"Ratio of operator edges to original edges: {2 * ldli.nnz / a.nnz}")
+        dg = la.diagonal()
+        print(f"ratio of max to min diagonal of laplacian : {dg.max() / dg.min()}")
+    return GpuLapSolver(ctx, ldli, tol, maxits, maxtime, verbose, pcgIts, params.stag_test)
+
+
+def block_solver(comps, solvers, tol=1e-6, maxits=1000, maxtime=math.inf, verbose=False, pcgIts=None):
+    """blockSolver (src/solverInterface.jl:152-188): apply the i-th solver on
+    the i-th component; pcgIts[1] receives the max over components."""
+    defaults = dict(tol=tol, maxits=maxits, maxtime=maxtime, verbose=verbose, pcgIts=pcgIts)
+
+    def f(b, tol=None, maxits=None, maxtime=None, verbose=None, pcgIts=None):
+        tol = defaults["tol"] if tol is None else tol
+        maxits = defaults["maxits"] if maxits is None else maxits
+        maxtime = defaults["maxtime"] if maxtime is None else maxtime
+        verbose = defaults["verbose"] if verbose is None else verbose
+        pcgIts = defaults["pcgIts"] if pcgIts is None else pcgIts
+        if pcgIts is not None and len(pcgIts) > 0:
+            pcgIts[0] = 0
+            tmp = [0]
+        else:
+            tmp = []
+        b = np.asarray(b, dtype=np.float64)
+        x = np.zeros(len(b))
+        for i, ind in enumerate(comps):
+            if verbose:
+                print(f"BlockSolver component index {i + 1}")
+            x[ind] = solvers[i](b[ind], tol=tol, maxits=maxits, maxtime=maxtime, verbose=verbose, pcgIts=tmp)
+            if pcgIts is not None and len(pcgIts) > 0:
+                pcgIts[0] = max(pcgIts[0], tmp[0])
+        return x
+
+    return f
+
+
+def lap_wrap_components(solver: Callable, a, tol=1e-6, maxits=1000, maxtime=math.inf, verbose=False,
+                        pcgIts=None, **params):
+    """lapWrapComponents (src/solverInterface.jl:212-275).  Deviation: the
+    reference solves components of fewer than 50 vertices with CHOLMOD on the
+    host; here every component with two or more vertices goes to the GPU
+    solver (same tolerance contract), so no solve runs on the CPU."""
+    t1 = time.time()
+    a = sp.csc_matrix(a)
+    if np.any(a.diagonal() != 0):
+        warnings.warn("The matrix should not have any nonzero diagonal entries.")
+        a = sp.csc_matrix(a - sp.diags(a.diagonal()))
+        a.eliminate_zeros()
+    ncomp, labels = connected_components(a, directed=False)
+    if ncomp == 1:
+        s = solver(a, tol=tol, maxits=maxits, maxtime=maxtime, verbose=verbose, pcgIts=pcgIts, **params)
+        if verbose:
+            print(f"Solver build time: {round(time.time() - t1, 3)} seconds.")
+        return s
+    order = np.argsort(labels, kind="stable")
+    bounds = np.searchsorted(labels[order], np.arange(ncomp + 1))
+    comps = [order[bounds[i]:bounds[i + 1]] for i in range(ncomp)]
+    singles = [c for c in comps if len(c) == 1]
+    others = [c for c in comps if len(c) > 1]
+    solvers = []
+    out_comps = []
+    if singles:
+        out_comps.append(np.concatenate(singles))
+        solvers.append(null_solver)
+    for ind in others:
+        asub = a[ind][:, ind]
+        solvers.append(solver(asub, tol=tol, maxits=maxits, maxtime=maxtime, verbose=verbose, pcgIts=pcgIts, **params))
+        out_comps.append(ind)
+    if verbose:
+        print(f"Solver build time: {round(time.time() - t1, 3)} seconds.")
+    return block_solver(out_comps, solvers, tol=tol, maxits=maxits, maxtime=maxtime, verbose=verbose, pcgIts=pcgIts)
+
+
+def approxchol_lap_gpu(a, tol=1e-6, maxits=1000, maxtime=math.inf, verbose=False, pcgIts=None,
+                       params: ApproxCholParams | None = None, **kw):
+    """approxchol_lap (src/approxChol.jl:1522-1545) with the solve on the GPU.
+    Raises ValueError for negative weights, as the reference's error()."""
+    a = sp.csc_matrix(a)
+    if a.nnz and a.data.min() < 0:
+        raise ValueError("Adjacency matrix can not have negative edge weights")
+    return lap_wrap_components(approxchol_lap1_gpu, a, tol=tol, maxits=maxits, maxtime=maxtime,
+                               verbose=verbose, pcgIts=pcgIts, params=params or ApproxCholParams(), **kw)
+
+
+def sddm_wrap_lap(lap_solver: Callable, sddm, tol=1e-6, maxits=1000, maxtime=math.inf, verbose=False,
+                  pcgIts=None, **params):
+    """sddmWrapLap (src/solverInterface.jl:324-352): solve an SDDM system by
+    adding a ground vertex and solving [b; -sum(b)] on the extended graph."""
+    M = sp.csc_matrix(sddm)
+    n = M.shape[0]
+    a = sp.csc_matrix(-(M - sp.diags(M.diagonal())))
+    a.eliminate_zeros()
+    excess = np.asarray(M.sum(axis=1)).ravel()
+    excess[np.abs(excess) < 1e-15 * np.abs(M.diagonal()).max()] = 0.0
+    if np.all(excess == 0):
+        return lap_solver(a, tol=tol, maxits=maxits, maxtime=maxtime, verbose=verbose, pcgIts=pcgIts, **params)
+    ex = sp.csc_matrix(excess.reshape(-1, 1))
+    a1 = sp.bmat([[a, ex], [ex.T, None]], format="csc")
+    F = lap_solver(a1, tol=tol, maxits=maxits, maxtime=maxtime, verbose=verbose, pcgIts=pcgIts, **params)
+
+    def f(b, tol=tol, maxits=maxits, maxtime=maxtime, verbose=verbose, pcgIts=pcgIts):
+        b = np.asarray(b, dtype=np.float64)
+        xaug = F(np.concatenate([b, [-b.sum()]]), tol=tol, maxits=maxits, maxtime=maxtime, verbose=verbose,
+                 pcgIts=pcgIts)
+        xaug = xaug - xaug[-1]
+        return xaug[:n]
+
+    return f
+
+
+def approxchol_sddm_gpu(sddm, **kw):
+    """approxchol_sddm = sddmWrapLap(approxchol_lap) (src/approxChol.jl:1862)."""
+    return sddm_wrap_lap(approxchol_lap_gpu, sddm, **kw)
