@@ -7,6 +7,7 @@
 // never waits for the host.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -43,7 +44,7 @@ struct lapgpu_ctx {
   bool has_L = false, has_F = false;
   Plan plan;
   int64_t n = 0;
-  int64_t tail_width = 256;
+  int64_t max_tail_steps = 12;
   int G_spmv = 8, G_ldl = 8;
   int precision = LAPGPU_FP64, replace_every = 20;
 
@@ -53,6 +54,14 @@ struct lapgpu_ctx {
   int *Fptr = nullptr, *Fidx = nullptr, *Bptr = nullptr, *Bidx = nullptr,
       *lvl = nullptr;
   double *Fw = nullptr, *Bw = nullptr, *s = nullptr;
+  // tail program
+  int *Eptr = nullptr, *Eidx = nullptr;
+  double *Ew = nullptr, *ext = nullptr;
+  TailStepD *fsteps = nullptr, *bsteps = nullptr;
+  TailSlotD *fslots = nullptr, *bslots = nullptr;
+  // one PCG iteration captured as a HIP graph (rebuilt after each upload)
+  hipGraphExec_t iter_exec = nullptr;
+  bool use_graph = true;
 
   // workspace
   double *b = nullptr, *r = nullptr, *x = nullptr, *bestx = nullptr,
@@ -154,13 +163,14 @@ void launch_sweeps(lapgpu_ctx* c, const double* rin, double* y) {
                        dim3(kBlock), 0, c->stream, lo, hi, c->Fptr, c->Fidx,
                        c->Fw, rin, y, c->st);
   }
-  if (tl < P.nlev) {
-    hipLaunchKernelGGL(k_fwd_tail<G>, dim3(1), dim3(kTailThreads), 0,
-                       c->stream, (int)tl, (int)P.nlev, c->lvl, c->Fptr,
-                       c->Fidx, c->Fw, rin, y, c->st);
-    hipLaunchKernelGGL(k_bwd_tail<G>, dim3(1), dim3(kTailThreads), 0,
-                       c->stream, (int)tl, (int)P.nlev, c->lvl, c->Bptr,
-                       c->Bidx, c->Bw, c->s, y, c->st);
+  const TailProgram& T = P.tail;
+  if (T.NT > 0) {
+    hipLaunchKernelGGL(k_tail_ext<G>, dim3(grid_for(T.NT * G, kBlock)), dim3(kBlock), 0,
+                       c->stream, (int)T.T0, (int)T.NT, c->Eptr, c->Eidx, c->Ew, rin,
+                       y, c->ext, c->st);
+    hipLaunchKernelGGL(k_tail, dim3(1), dim3(kTailThreads), 0, c->stream, (int)T.T0,
+                       (int)T.NT, c->ext, c->s, y, c->fsteps, (int)T.fsteps.size(),
+                       c->fslots, c->bsteps, (int)T.bsteps.size(), c->bslots, c->st);
   }
   for (int64_t l = tl - 1; l >= 0; --l) {
     const int lo = P.lvl[l], hi = P.lvl[l + 1];
@@ -187,13 +197,22 @@ void precond(lapgpu_ctx* c, const double* rin) {
   const int n = (int)c->n;
   hipLaunchKernelGGL(k_vsum, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
                      c->y, c->part, c->st);
-  hipLaunchKernelGGL(k_scalar_mu, dim3(1), dim3(kWave), 0, c->stream, n,
+  hipLaunchKernelGGL(k_scalar_mu, dim3(1), dim3(kScalarThreads), 0, c->stream, n,
                      c->part, c->st);
   hipLaunchKernelGGL(k_zdot, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
                      c->y, rin, c->p, c->z, c->part, c->st);
 }
 
 int free_graph(lapgpu_ctx* c) {
+  if (c->iter_exec) (void)hipGraphExecDestroy(c->iter_exec);
+  c->iter_exec = nullptr;
+  void* tp[] = {c->Eptr, c->Eidx, c->Ew, c->ext, c->fsteps, c->bsteps, c->fslots, c->bslots};
+  for (void* a : tp)
+    if (a) (void)hipFree(a);
+  c->Eptr = c->Eidx = nullptr;
+  c->Ew = c->ext = nullptr;
+  c->fsteps = c->bsteps = nullptr;
+  c->fslots = c->bslots = nullptr;
   int* ip[] = {c->perm, c->Lptr, c->Lidx, c->Fptr, c->Fidx, c->Bptr, c->Bidx, c->lvl};
   for (int* a : ip)
     if (a) (void)hipFree(a);
@@ -223,6 +242,28 @@ int upload_plan(lapgpu_ctx* c) {
   if ((rc = dev_upload(c, &c->Bw, P.Bw))) return rc;
   if ((rc = dev_upload(c, &c->s, P.s))) return rc;
   if ((rc = dev_upload(c, &c->lvl, P.lvl))) return rc;
+  if ((rc = dev_upload(c, &c->Eptr, P.tail.Eptr))) return rc;
+  if ((rc = dev_upload(c, &c->Eidx, P.tail.Eidx))) return rc;
+  if ((rc = dev_upload(c, &c->Ew, P.tail.Ew))) return rc;
+  if ((rc = dev_alloc(c, &c->ext, (size_t)std::max<int64_t>(P.tail.NT, 1)))) return rc;
+  {
+    static_assert(sizeof(TailStepD) == sizeof(TailStep), "step layout");
+    static_assert(sizeof(TailSlotD) == sizeof(TailSlot), "slot layout");
+    std::vector<TailStepD> fs(P.tail.fsteps.size()), bs(P.tail.bsteps.size());
+    std::memcpy(fs.data(), P.tail.fsteps.data(), sizeof(TailStep) * fs.size());
+    std::memcpy(bs.data(), P.tail.bsteps.data(), sizeof(TailStep) * bs.size());
+    std::vector<TailSlotD> fl(P.tail.fslots.size()), bl(P.tail.bslots.size());
+    std::memcpy(fl.data(), P.tail.fslots.data(), sizeof(TailSlot) * fl.size());
+    std::memcpy(bl.data(), P.tail.bslots.data(), sizeof(TailSlot) * bl.size());
+    if ((rc = dev_upload(c, &c->fsteps, fs))) return rc;
+    if ((rc = dev_upload(c, &c->bsteps, bs))) return rc;
+    if ((rc = dev_upload(c, &c->fslots, fl))) return rc;
+    if ((rc = dev_upload(c, &c->bslots, bl))) return rc;
+  }
+  if (c->iter_exec) {
+    HIPC(c, hipGraphExecDestroy(c->iter_exec));
+    c->iter_exec = nullptr;
+  }
   double** ws[] = {&c->b, &c->r, &c->x, &c->bestx, &c->p, &c->q, &c->y, &c->z, &c->io};
   for (double** w : ws)
     if ((rc = dev_alloc(c, w, (size_t)n))) return rc;
@@ -255,6 +296,7 @@ void identity_plan(lapgpu_ctx* c) {
   P.nlev = 1;
   P.lvl = {0, (int32_t)n};
   P.tail_level = 1;
+  P.tail = TailProgram();
 }
 
 void init_state(lapgpu_ctx* c, double tol, int64_t maxits, double maxtime,
@@ -292,19 +334,19 @@ void enqueue_iteration(lapgpu_ctx* c, bool prof) {
   if (prof) (void)hipEventRecord(c->ev[0], c->stream);
   spmv(c, c->p, c->q);
   if (prof) (void)hipEventRecord(c->ev[1], c->stream);
-  hipLaunchKernelGGL(k_scalar_al, dim3(1), dim3(kWave), 0, c->stream, c->part, c->st);
+  hipLaunchKernelGGL(k_scalar_al, dim3(1), dim3(kScalarThreads), 0, c->stream, c->part, c->st);
   hipLaunchKernelGGL(k_update, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
                      c->x, c->bestx, c->p, c->q, c->r, c->part, c->st);
-  hipLaunchKernelGGL(k_scalar_nr, dim3(1), dim3(kWave), 0, c->stream, c->part, c->st);
+  hipLaunchKernelGGL(k_scalar_nr, dim3(1), dim3(kScalarThreads), 0, c->stream, c->part, c->st);
   if (prof) (void)hipEventRecord(c->ev[2], c->stream);
   sweeps(c, c->r, c->y);
   if (prof) (void)hipEventRecord(c->ev[3], c->stream);
   hipLaunchKernelGGL(k_vsum, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
                      c->y, c->part, c->st);
-  hipLaunchKernelGGL(k_scalar_mu, dim3(1), dim3(kWave), 0, c->stream, n, c->part, c->st);
+  hipLaunchKernelGGL(k_scalar_mu, dim3(1), dim3(kScalarThreads), 0, c->stream, n, c->part, c->st);
   hipLaunchKernelGGL(k_zdot, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
                      c->y, c->r, c->p, c->z, c->part, c->st);
-  hipLaunchKernelGGL(k_scalar_rho, dim3(1), dim3(kWave), 0, c->stream, c->part, c->st);
+  hipLaunchKernelGGL(k_scalar_rho, dim3(1), dim3(kScalarThreads), 0, c->stream, c->part, c->st);
   hipLaunchKernelGGL(k_pupdate, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
                      c->z, c->p, c->st);
   if (prof) (void)hipEventRecord(c->ev[4], c->stream);
@@ -324,12 +366,12 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
   // ---- setup: r = b - mean(b), z = pre(r), p = z, rho = r.z ---------------
   hipLaunchKernelGGL(k_gather_perm, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
                      c->perm, d_b, c->b, c->part);
-  hipLaunchKernelGGL(k_scalar_mean, dim3(1), dim3(kWave), 0, c->stream, n, c->part, c->st);
+  hipLaunchKernelGGL(k_scalar_mean, dim3(1), dim3(kScalarThreads), 0, c->stream, n, c->part, c->st);
   hipLaunchKernelGGL(k_center, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n, c->b,
                      c->r, c->x, c->bestx, c->part, c->st);
-  hipLaunchKernelGGL(k_scalar_nb, dim3(1), dim3(kWave), 0, c->stream, c->part, c->st);
+  hipLaunchKernelGGL(k_scalar_nb, dim3(1), dim3(kScalarThreads), 0, c->stream, c->part, c->st);
   precond(c, c->r);
-  hipLaunchKernelGGL(k_scalar_rho0, dim3(1), dim3(kWave), 0, c->stream, c->part, c->st);
+  hipLaunchKernelGGL(k_scalar_rho0, dim3(1), dim3(kScalarThreads), 0, c->stream, c->part, c->st);
   hipLaunchKernelGGL(k_copy, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n, c->z,
                      c->p, c->st);
   HIPC(c, hipGetLastError());
@@ -346,8 +388,20 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
   double acc_spmv = 0, acc_ldl = 0, acc_vec = 0;
   int64_t enq = 0, polled = 0;
   bool done = c->hst->stopped != 0;
+  if (!prof && c->use_graph && !c->iter_exec) {
+    hipGraph_t g = nullptr;
+    HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_iteration(c, false);
+    HIPC(c, hipStreamEndCapture(c->stream, &g));
+    hipError_t ie = hipGraphInstantiate(&c->iter_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPC(c, ie);
+  }
   while (!done) {
-    enqueue_iteration(c, prof);
+    if (!prof && c->iter_exec)
+      HIPC(c, hipGraphLaunch(c->iter_exec, c->stream));
+    else
+      enqueue_iteration(c, prof);
     const int slot = (int)(enq % lapgpu_ctx::kRing);
     HIPC(c, hipMemcpyAsync(&c->hflags[slot], &c->st->stopped, sizeof(int32_t),
                            hipMemcpyDeviceToHost, c->stream));
@@ -450,7 +504,8 @@ int lapgpu_create(lapgpu_ctx** out, int device) {
       return fail("hipEventCreate", e);
   for (auto& ev : c->ev)
     if ((e = hipEventCreate(&ev)) != hipSuccess) return fail("hipEventCreate", e);
-  if (const char* tw = getenv("LAPGPU_TAIL_WIDTH")) c->tail_width = atoll(tw);
+  if (const char* tw = getenv("LAPGPU_TAIL_STEPS")) c->max_tail_steps = atoll(tw);
+  if (const char* ug = getenv("LAPGPU_NO_GRAPH")) c->use_graph = atoi(ug) == 0;
   *out = c;
   return 0;
 }
@@ -506,7 +561,7 @@ int lapgpu_upload_ldli(lapgpu_ctx* c, int64_t n, int64_t nnzF,
   HIPC(c, hipSetDevice(c->device));
   std::string err;
   Plan P;
-  if (!build_plan(c->L, nnzF, col, colptr, rowval, fval, d, index_base, c->tail_width, P, err))
+  if (!build_plan(c->L, nnzF, col, colptr, rowval, fval, d, index_base, c->max_tail_steps, P, err))
     return set_err(c, -1, err);
   c->plan = std::move(P);
   int rc = upload_plan(c);
@@ -600,7 +655,7 @@ int lapgpu_factor_info(lapgpu_ctx* c, int64_t* nlevels, int64_t* nwide,
   if (!c->has_F) return set_err(c, -2, "factor_info: no LDLinv uploaded");
   if (nlevels) *nlevels = c->plan.nlev;
   if (nwide) *nwide = c->plan.tail_level;
-  if (tail_start) *tail_start = c->plan.lvl[c->plan.tail_level];
+  if (tail_start) *tail_start = c->plan.tail.T0;
   if (nnzF) *nnzF = c->plan.nnzF;
   return 0;
 }

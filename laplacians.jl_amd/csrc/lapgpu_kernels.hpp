@@ -76,12 +76,22 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
   }
 }
 
-// Sum of the kRedBlocks partials of one slot, by one wave (fixed order).
+// Sum of the kRedBlocks partials of one slot by a block of kScalarThreads
+// (= kRedBlocks) threads, one partial each, in a fixed tree order.  The
+// result is returned to every thread.
+constexpr int kScalarThreads = kRedBlocks;
 __device__ __forceinline__ double wave_total(const double* part, int slot) {
-  const int lane = threadIdx.x;
+  __shared__ double red[kScalarThreads / kWave];
+  double v = part[slot * kRedBlocks + threadIdx.x];
+  v = group_sum<kWave>(v);
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
   double s = 0.0;
-  for (int b = lane; b < kRedBlocks; b += kWave) s += part[slot * kRedBlocks + b];
-  return group_sum<kWave>(s);
+#pragma unroll
+  for (int w = 0; w < kScalarThreads / kWave; ++w) s += red[w];
+  return s;
 }
 
 // ---------------------------------------------------------------------------
@@ -153,7 +163,6 @@ k_fwd_level(int lo, int hi, const int* __restrict__ Fptr,
             const int* __restrict__ Fidx, const double* __restrict__ Fw,
             const double* __restrict__ r, double* y,
             const DevState* __restrict__ st) {
-  if (st->stopped) return;
   const int gid = blockIdx.x * kBlock + threadIdx.x;
   const int row = lo + gid / G;
   const int lane = gid % G;
@@ -162,7 +171,7 @@ k_fwd_level(int lo, int hi, const int* __restrict__ Fptr,
   double acc = 0.0;
   for (int k = k0 + lane; k < k1; k += G) acc += Fw[k] * y[Fidx[k]];
   acc = group_sum<G>(acc);
-  if (lane == 0) y[row] = r[row] + acc;
+  if (lane == 0 && !st->stopped) y[row] = r[row] + acc;
 }
 
 template <int G>
@@ -171,7 +180,6 @@ k_bwd_level(int lo, int hi, const int* __restrict__ Bptr,
             const int* __restrict__ Bidx, const double* __restrict__ Bw,
             const double* __restrict__ s, double* y,
             const DevState* __restrict__ st) {
-  if (st->stopped) return;
   const int gid = blockIdx.x * kBlock + threadIdx.x;
   const int row = lo + gid / G;
   const int lane = gid % G;
@@ -180,53 +188,140 @@ k_bwd_level(int lo, int hi, const int* __restrict__ Bptr,
   double acc = 0.0;
   for (int k = k0 + lane; k < k1; k += G) acc += Bw[k] * y[Bidx[k]];
   acc = group_sum<G>(acc);
-  if (lane == 0) y[row] = s[row] * y[row] + acc;
+  if (lane == 0 && !st->stopped) y[row] = s[row] * y[row] + acc;
 }
 
-// The narrow tail of the DAG: one workgroup walks levels [l0, l1) with a
-// workgroup barrier between levels (forward) or (l1, l0] (backward).
-template <int G>
-__global__ void __launch_bounds__(kTailThreads)
-k_fwd_tail(int l0, int l1, const int* __restrict__ lvl,
-           const int* __restrict__ Fptr, const int* __restrict__ Fidx,
-           const double* __restrict__ Fw, const double* __restrict__ r,
-           double* y, const DevState* __restrict__ st) {
-  if (st->stopped) return;
-  const int grp = threadIdx.x / G, lane = threadIdx.x % G;
-  constexpr int kGroups = kTailThreads / G;
-  for (int l = l0; l < l1; ++l) {
-    const int lo = lvl[l], hi = lvl[l + 1];
-    for (int row = lo + grp; row < hi; row += kGroups) {
-      const int k0 = Fptr[row], k1 = Fptr[row + 1];
-      double acc = 0.0;
-      for (int k = k0 + lane; k < k1; k += G) acc += Fw[k] * y[Fidx[k]];
-      acc = group_sum<G>(acc);
-      if (lane == 0) y[row] = r[row] + acc;
+// ---------------------------------------------------------------------------
+// The narrow tail of the DAG (the last levels, at most kTailCapV vertices):
+// one workgroup holds y of all tail vertices in LDS and runs the forward
+// sweep, the diagonal scaling and the backward sweep of those levels with a
+// workgroup barrier per level.  The work of each level is a host-packed
+// "slot program" (TailSlot / TailStep of lapgpu_plan.hpp): lane t of step j
+// computes w * y[src], the lanes of an aligned power-of-two segment are
+// summed with xor-shuffles, and the segment's lead lane adds the sum to
+// y[row].  Slots are loaded kTailDepth steps ahead of their use.
+
+constexpr int kTailCapV = 18432;
+constexpr int kTailDepth = 8;
+
+struct alignas(16) TailSlotD {
+  double w;
+  unsigned short src, row;
+  unsigned char seglog, lead;
+  unsigned short pad;
+};
+struct alignas(16) TailStepD {
+  int off, count, end, pad;
+};
+
+__device__ __forceinline__ TailSlotD load_slot(const TailSlotD* __restrict__ slots,
+                                               const TailStepD& st, int t) {
+  TailSlotD z;
+  if (t < st.count) {
+    z = slots[st.off + t];
+  } else {
+    z.w = 0.0;
+    z.src = 0;
+    z.row = 0;
+    z.seglog = 0;
+    z.lead = 0;
+    z.pad = 0;
+  }
+  return z;
+}
+
+__device__ __forceinline__ void run_slot(const TailSlotD& sl, const TailStepD& st,
+                                         double* ly) {
+  if ((int)threadIdx.x < st.count) {
+    double v = sl.w * ly[sl.src];
+    const int seg = 1 << sl.seglog;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const double t = __shfl_xor(v, o, kWave);
+      if (o < seg) v += t;
     }
-    __syncthreads();
+    if (sl.lead == 1)
+      ly[sl.row] += v;
+    else if (sl.lead == 2)
+      atomicAdd(&ly[sl.row], v);
   }
 }
 
-template <int G>
-__global__ void __launch_bounds__(kTailThreads)
-k_bwd_tail(int l0, int l1, const int* __restrict__ lvl,
-           const int* __restrict__ Bptr, const int* __restrict__ Bidx,
-           const double* __restrict__ Bw, const double* __restrict__ s,
-           double* y, const DevState* __restrict__ st) {
-  if (st->stopped) return;
-  const int grp = threadIdx.x / G, lane = threadIdx.x % G;
-  constexpr int kGroups = kTailThreads / G;
-  for (int l = l1 - 1; l >= l0; --l) {
-    const int lo = lvl[l], hi = lvl[l + 1];
-    for (int row = lo + grp; row < hi; row += kGroups) {
-      const int k0 = Bptr[row], k1 = Bptr[row + 1];
-      double acc = 0.0;
-      for (int k = k0 + lane; k < k1; k += G) acc += Bw[k] * y[Bidx[k]];
-      acc = group_sum<G>(acc);
-      if (lane == 0) y[row] = s[row] * y[row] + acc;
-    }
-    __syncthreads();
+// Runs nsteps steps of a slot program on ly.
+__device__ __forceinline__ void run_program(const TailStepD* __restrict__ steps,
+                                            int nsteps,
+                                            const TailSlotD* __restrict__ slots,
+                                            double* ly) {
+  const int t = threadIdx.x;
+  TailStepD ia[kTailDepth], ib[kTailDepth];
+  TailSlotD a[kTailDepth], b[kTailDepth];
+  const TailStepD none = {0, 0, 0, 0};
+#pragma unroll
+  for (int d = 0; d < kTailDepth; ++d) {
+    ia[d] = d < nsteps ? steps[d] : none;
+    a[d] = load_slot(slots, ia[d], t);
   }
+  for (int c = 0; c < nsteps; c += kTailDepth) {
+#pragma unroll
+    for (int d = 0; d < kTailDepth; ++d) {
+      const int j = c + kTailDepth + d;
+      ib[d] = j < nsteps ? steps[j] : none;
+      b[d] = load_slot(slots, ib[d], t);
+    }
+#pragma unroll
+    for (int d = 0; d < kTailDepth; ++d) {
+      if (c + d < nsteps) {
+        run_slot(a[d], ia[d], ly);
+        if (ia[d].end) __syncthreads();
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < kTailDepth; ++d) {
+      a[d] = b[d];
+      ia[d] = ib[d];
+    }
+  }
+}
+
+// ext[i] = r[T0+i] + sum of the forward in-edges of tail row i that come
+// from outside the tail (G lanes per row)
+template <int G>
+__global__ void __launch_bounds__(kBlock)
+k_tail_ext(int T0, int NT, const int* __restrict__ Eptr,
+           const int* __restrict__ Eidx, const double* __restrict__ Ew,
+           const double* __restrict__ r, const double* __restrict__ y,
+           double* __restrict__ ext, const DevState* __restrict__ st) {
+  const int gid = blockIdx.x * kBlock + threadIdx.x;
+  const int row = gid / G;
+  const int lane = gid % G;
+  if (row >= NT) return;
+  const int k0 = Eptr[row], k1 = Eptr[row + 1];
+  double acc = 0.0;
+  for (int k = k0 + lane; k < k1; k += G) acc += Ew[k] * y[Eidx[k]];
+  acc = group_sum<G>(acc);
+  if (lane == 0 && !st->stopped) ext[row] = r[T0 + row] + acc;
+}
+
+// forward + diagonal + backward on the tail; writes yF of the tail to y
+__global__ void __launch_bounds__(kTailThreads)
+k_tail(int T0, int NT, const double* __restrict__ ext,
+       const double* __restrict__ s, double* __restrict__ y,
+       const TailStepD* __restrict__ fsteps, int nfsteps,
+       const TailSlotD* __restrict__ fslots,
+       const TailStepD* __restrict__ bsteps, int nbsteps,
+       const TailSlotD* __restrict__ bslots,
+       const DevState* __restrict__ st) {
+  __shared__ double ly[kTailCapV];
+  if (st->stopped) return;
+  for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] = ext[i];
+  __syncthreads();
+  run_program(fsteps, nfsteps, fslots, ly);
+  __syncthreads();
+  for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] *= s[T0 + i];
+  __syncthreads();
+  run_program(bsteps, nbsteps, bslots, ly);
+  __syncthreads();
+  for (int i = threadIdx.x; i < NT; i += kTailThreads) y[T0 + i] = ly[i];
 }
 
 // partial sum of y (for the mean removal of LDLsolver)

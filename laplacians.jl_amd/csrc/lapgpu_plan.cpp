@@ -68,10 +68,94 @@ bool csc_to_csr(int64_t n, const int64_t* colptr, const int64_t* rowval,
   return true;
 }
 
+namespace {
+
+int64_t pow2_at_least(int64_t v) {
+  int64_t p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+int log2_exact(int64_t v) {
+  int l = 0;
+  while ((int64_t(1) << l) < v) ++l;
+  return l;
+}
+
+// Lanes one row takes: chunks of at most 64 entries, each in an aligned
+// power-of-two segment.
+int64_t row_lanes(int64_t dg) {
+  if (dg == 0) return 0;
+  return 64 * (dg / 64) + ((dg % 64) ? pow2_at_least(dg % 64) : 0);
+}
+
+struct RowEntries {
+  int32_t row;  // local
+  std::vector<std::pair<int32_t, double>> e;  // (local src, weight)
+};
+
+// Pack one level's rows into steps of kTailLanes lanes (see TailSlot).  A row
+// with at most 64 entries is one segment whose lead lane adds its sum to
+// y[row] (lead = 1); a longer row is split into chunks whose leads add with
+// an LDS atomic (lead = 2).  Segments are placed largest first with a bump
+// pointer per step, which keeps every segment aligned to its size.
+void pack_level(std::vector<RowEntries>& rows, std::vector<TailStep>& steps,
+                std::vector<TailSlot>& slots) {
+  struct Chunk {
+    int32_t row;
+    int64_t size;  // segment lanes
+    int64_t k0, k1;
+    const RowEntries* R;
+    bool atomic;
+  };
+  std::vector<Chunk> chunks;
+  for (const RowEntries& R : rows) {
+    const int64_t dg = (int64_t)R.e.size();
+    if (dg == 0) continue;
+    const bool multi = dg > 64;
+    for (int64_t k0 = 0; k0 < dg; k0 += 64) {
+      const int64_t k1 = std::min(dg, k0 + 64);
+      chunks.push_back({R.row, pow2_at_least(k1 - k0), k0, k1, &R, multi});
+    }
+  }
+  std::stable_sort(chunks.begin(), chunks.end(),
+                   [](const Chunk& a, const Chunk& b) { return a.size > b.size; });
+  std::vector<std::vector<TailSlot>> pass;
+  std::vector<int64_t> ptr;
+  for (const Chunk& C : chunks) {
+    size_t p = 0;
+    while (p < pass.size() && ptr[p] + C.size > kTailLanes) ++p;
+    if (p == pass.size()) {
+      pass.emplace_back(kTailLanes, TailSlot{0.0, 0, 0, 0, 0, 0});
+      ptr.push_back(0);
+    }
+    const int64_t off = ptr[p];
+    ptr[p] += C.size;
+    for (int64_t t = 0; t < C.size; ++t) {
+      TailSlot& S = pass[p][off + t];
+      S.seglog = (uint8_t)log2_exact(C.size);
+      S.lead = (t == 0) ? (C.atomic ? 2 : 1) : 0;
+      S.row = (uint16_t)C.row;
+      if (C.k0 + t < C.k1) {
+        S.src = (uint16_t)C.R->e[C.k0 + t].first;
+        S.w = C.R->e[C.k0 + t].second;
+      }
+    }
+  }
+  for (size_t p = 0; p < pass.size(); ++p) {
+    const int64_t cnt = (ptr[p] + 63) / 64 * 64;
+    TailStep st{(int32_t)slots.size(), (int32_t)cnt, p + 1 == pass.size() ? 1 : 0, 0};
+    steps.push_back(st);
+    slots.insert(slots.end(), pass[p].begin(), pass[p].begin() + cnt);
+  }
+}
+
+}  // namespace
+
 bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
                 const int64_t* colptr, const int64_t* rowval,
                 const double* fval, const double* d, int base,
-                int64_t tail_width, Plan& P, std::string& err) {
+                int64_t max_tail_steps, Plan& P, std::string& err) {
   const int64_t n = L.n;
   if (n < 1) {
     err = "upload_ldli: upload the Laplacian first";
@@ -166,12 +250,6 @@ bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
     }
   }
 
-  // tail: the maximal suffix of levels narrower than tail_width
-  {
-    int64_t t = P.nlev;
-    while (t > 0 && (P.lvl[t] - P.lvl[t - 1]) < tail_width) --t;
-    P.tail_level = t;
-  }
 
   // ---- permuted Laplacian --------------------------------------------------
   const int64_t nnzL = (int64_t)L.idx.size();
@@ -237,6 +315,71 @@ bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
         P.Fidx[o] = src;
         P.Fw[o] = w[k];
       }
+    }
+  }
+  // ---- the LDS-resident tail -----------------------------------------------
+  {
+    int64_t t = P.nlev, nt = 0;
+    while (t > 0) {
+      const int64_t l = t - 1;
+      const int64_t w = P.lvl[l + 1] - P.lvl[l];
+      if (nt + w > kTailCap) break;
+      int64_t fl = 0, bl = 0;
+      for (int64_t pos = P.lvl[l]; pos < P.lvl[l + 1]; ++pos) {
+        fl += row_lanes(P.Fptr[pos + 1] - P.Fptr[pos]);
+        bl += row_lanes(P.Bptr[pos + 1] - P.Bptr[pos]);
+      }
+      // bump packing of power-of-two segments in decreasing size fills
+      // whole steps, so lanes / kTailLanes rounded up is the step count
+      const int64_t steps_l = (std::max(fl, bl) + kTailLanes - 1) / kTailLanes;
+      if (steps_l > max_tail_steps) break;
+      nt += w;
+      t = l;
+    }
+    TailProgram& T = P.tail;
+    T = TailProgram();
+    P.tail_level = t;
+    T.T0 = P.lvl[t];
+    T.NT = n - T.T0;
+    const int32_t T0 = (int32_t)T.T0;
+    T.Eptr.assign(T.NT + 1, 0);
+    for (int64_t i = 0; i < T.NT; ++i) {
+      const int64_t pos = T0 + i;
+      for (int32_t k = P.Fptr[pos]; k < P.Fptr[pos + 1]; ++k)
+        if (P.Fidx[k] < T0) {
+          T.Eidx.push_back(P.Fidx[k]);
+          T.Ew.push_back(P.Fw[k]);
+        }
+      T.Eptr[i + 1] = (int32_t)T.Eidx.size();
+    }
+    bool ok = true;
+    for (int64_t l = t; l < P.nlev; ++l) {
+      std::vector<RowEntries> rows;
+      for (int64_t pos = P.lvl[l]; pos < P.lvl[l + 1]; ++pos) {
+        RowEntries R;
+        R.row = (int32_t)(pos - T0);
+        for (int32_t k = P.Fptr[pos]; k < P.Fptr[pos + 1]; ++k)
+          if (P.Fidx[k] >= T0) R.e.emplace_back(P.Fidx[k] - T0, P.Fw[k]);
+        rows.push_back(std::move(R));
+      }
+      pack_level(rows, T.fsteps, T.fslots);
+    }
+    for (int64_t l = P.nlev - 1; l >= t && ok; --l) {
+      std::vector<RowEntries> rows;
+      for (int64_t pos = P.lvl[l]; pos < P.lvl[l + 1]; ++pos) {
+        RowEntries R;
+        R.row = (int32_t)(pos - T0);
+        for (int32_t k = P.Bptr[pos]; k < P.Bptr[pos + 1]; ++k) {
+          if (P.Bidx[k] < T0) ok = false;
+          R.e.emplace_back(P.Bidx[k] - T0, P.Bw[k]);
+        }
+        rows.push_back(std::move(R));
+      }
+      pack_level(rows, T.bsteps, T.bslots);
+    }
+    if (!ok) {
+      err = "upload_ldli: internal error packing the tail program";
+      return false;
     }
   }
   return true;

@@ -16,6 +16,41 @@ struct HostCsr {
   std::vector<double> val;
 };
 
+// One lane's work in the tail kernel: w * y[src] summed over an aligned
+// segment of 2^seglog lanes; the segment's first lane adds the sum to y[row]
+// (lead = 1: plain add; lead = 2: LDS atomic add, for rows split in chunks).
+// Indices are local to the tail (position - T0).
+struct TailSlot {
+  double w;
+  uint16_t src;
+  uint16_t row;
+  uint8_t seglog;
+  uint8_t lead;
+  uint16_t pad;
+};
+static_assert(sizeof(TailSlot) == 16, "TailSlot is 16 bytes");
+
+// A step = up to kTailLanes slots run by the tail workgroup at once; `end`
+// marks the last step of a level (a workgroup barrier follows it).
+struct TailStep {
+  int32_t off;    // first slot
+  int32_t count;  // active lanes (multiple of 64)
+  int32_t end;
+  int32_t pad;
+};
+
+constexpr int kTailLanes = 1024;
+constexpr int64_t kTailCap = 18432;  // tail vertices held in LDS (147 KiB)
+
+struct TailProgram {
+  int64_t T0 = 0, NT = 0;
+  std::vector<TailStep> fsteps, bsteps;
+  std::vector<TailSlot> fslots, bslots;
+  // forward in-edges of tail rows from outside the tail (CSR over tail rows)
+  std::vector<int32_t> Eptr, Eidx;
+  std::vector<double> Ew;
+};
+
 // Everything the device needs, in "level order": vertices sorted by the
 // forward level of the LDLinv column DAG (ties by original index).
 struct Plan {
@@ -40,17 +75,18 @@ struct Plan {
   std::vector<int32_t> lvl;
   int64_t nlev = 0;
   int64_t tail_level = 0;  // first level of the single-workgroup tail
+  TailProgram tail;
 };
 
 // Convert a caller CSC (int64, base 0/1) to 0-based int32 CSR.
 bool csc_to_csr(int64_t n, const int64_t* colptr, const int64_t* rowval,
                 const double* nzval, int base, HostCsr& out, std::string& err);
 
-// Build the plan. `tail_width`: levels at the end narrower than this run in
-// the single-workgroup tail kernel.
+// Build the plan. The tail is the longest suffix of levels that fits in LDS
+// (kTailCap vertices) and whose levels need at most `max_tail_steps` steps.
 bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
                 const int64_t* colptr, const int64_t* rowval,
                 const double* fval, const double* d, int base,
-                int64_t tail_width, Plan& P, std::string& err);
+                int64_t max_tail_steps, Plan& P, std::string& err);
 
 }  // namespace lapgpu
