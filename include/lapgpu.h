@@ -124,6 +124,77 @@ int lapgpu_last_timings(lapgpu_ctx* ctx, double* spmv_ms, double* ldl_ms,
 const char* lapgpu_last_error(lapgpu_ctx* ctx);
 void lapgpu_destroy(lapgpu_ctx* ctx);
 
+/* ---- batched solves: many graphs and many right-hand sides ------------- */
+/* No counterpart in the reference, which solves one b per call; this is the
+ * batched form of the same closure (src/approxChol.jl:1624) for callers that
+ * hold many graphs or many right-hand sides (blockSolver's components,
+ * src/solverInterface.jl:152-188; serving).  Each request runs the complete
+ * pcg of src/pcg.jl:256-445 with LDLsolver in one workgroup, all requests in
+ * one launch. */
+typedef struct lapgpu_batch lapgpu_batch;
+
+int lapgpu_batch_create(lapgpu_batch** out, int device);
+
+/* Upload one graph (la as in lapgpu_upload_csc, LDLinv as in
+ * lapgpu_upload_ldli); *graph_id receives its handle. */
+int lapgpu_batch_add_graph(lapgpu_batch* B, int64_t n, const int64_t* colptr,
+                           const int64_t* rowval, const double* nzval,
+                           int64_t nnzF, const int64_t* col,
+                           const int64_t* fcolptr, const int64_t* frowval,
+                           const double* fval, const double* d, int index_base,
+                           int32_t* graph_id);
+
+/* Free a graph's device memory; its id is not reused. */
+int lapgpu_batch_remove_graph(lapgpu_batch* B, int32_t graph_id);
+
+/* n and the device bytes held by a graph (for admission by HBM footprint). */
+int lapgpu_batch_graph_info(lapgpu_batch* B, int32_t graph_id, int64_t* n,
+                            int64_t* device_bytes);
+
+/* Solve nreq requests; request i solves graph graph_ids[i] with the
+ * right-hand side at b_concat[off_i .. off_i + n_i) (host buffers,
+ * concatenated in request order), writing x_concat likewise.  Stopping
+ * rules, tol/maxits/maxtime/stag_test/flags and the per-request outputs mean
+ * what they mean in lapgpu_solve. */
+int lapgpu_batch_solve(lapgpu_batch* B, int64_t nreq, const int32_t* graph_ids,
+                       const double* b_concat, double* x_concat, double tol,
+                       int64_t maxits, double maxtime, int stag_test, int flags,
+                       int64_t* its_out, double* relres_out, int32_t* stop_out);
+
+/* As lapgpu_batch_solve with per-request device pointers (resident b, x). */
+int lapgpu_batch_solve_device(lapgpu_batch* B, int64_t nreq,
+                              const int32_t* graph_ids,
+                              const double* const* d_b, double* const* d_x,
+                              double tol, int64_t maxits, double maxtime,
+                              int stag_test, int flags, int64_t* its_out,
+                              double* relres_out, int32_t* stop_out);
+
+/* Asynchronous form: enqueue a launch of nreq requests (device pointers) on
+ * one of the batch's launch slots and return at once; *slot_out names it.
+ * Launches on different slots run concurrently on the device.  Returns
+ * LAPGPU_BUSY when every slot holds an uncollected launch. */
+#define LAPGPU_BUSY 1
+int lapgpu_batch_submit(lapgpu_batch* B, int64_t nreq, const int32_t* graph_ids,
+                        const double* const* d_b, double* const* d_x, double tol,
+                        int64_t maxits, double maxtime, int stag_test, int flags,
+                        int32_t* slot_out);
+/* As lapgpu_batch_submit with the right-hand sides concatenated in a host
+ * buffer (copied before the call returns); the solutions come back through
+ * lapgpu_batch_collect's x_concat. */
+int lapgpu_batch_submit_host(lapgpu_batch* B, int64_t nreq, const int32_t* graph_ids,
+                             const double* b_concat, double tol, int64_t maxits,
+                             double maxtime, int stag_test, int flags, int32_t* slot_out);
+/* *done = 1 when the launch on `slot` has finished, else 0. */
+int lapgpu_batch_query(lapgpu_batch* B, int32_t slot, int* done);
+/* Wait for the launch on `slot`, write its per-request results (in the
+ * order of the submit call) and free the slot.  x_concat receives the
+ * solutions of a lapgpu_batch_submit_host launch (NULL otherwise). */
+int lapgpu_batch_collect(lapgpu_batch* B, int32_t slot, double* x_concat, int64_t* its_out,
+                         double* relres_out, int32_t* stop_out);
+
+const char* lapgpu_batch_last_error(lapgpu_batch* B);
+void lapgpu_batch_destroy(lapgpu_batch* B);
+
 #ifdef __cplusplus
 }
 #endif

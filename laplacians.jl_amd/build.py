@@ -64,9 +64,12 @@ def build_gpu(verbose: bool = False) -> str:
     plan_o = os.path.join(OUT, "_lapgpu_plan.o")
     _run(["g++", "-O3", "-std=c++17", "-fPIC", "-c", f"-I{INC}", f"-I{CSRC}", "-o", plan_o, os.path.join(CSRC, "lapgpu_plan.cpp")], verbose)
     objs.append(plan_o)
-    hip_o = os.path.join(OUT, "_lapgpu.o")
-    _run([HIPCC, *HIP_FLAGS, "-c", "-o", hip_o, os.path.join(CSRC, "lapgpu.hip")], verbose)
-    objs.append(hip_o)
+    srcs = ["lapgpu.hip", "lapgpu_batch.hip"]
+    outs = [os.path.join(OUT, f"_{os.path.splitext(s)[0]}.o") for s in srcs]
+    with ThreadPoolExecutor(max_workers=len(srcs)) as ex:
+        list(ex.map(lambda so: _run([HIPCC, *HIP_FLAGS, "-c", "-o", so[1], os.path.join(CSRC, so[0])], verbose),
+                    zip(srcs, outs)))
+    objs.extend(outs)
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out, *objs], verbose)
     for o in objs:
         os.remove(o)

@@ -47,6 +47,7 @@ struct lapgpu_ctx {
   int64_t max_tail_steps = 12;
   int G_spmv = 8, G_ldl = 8;
   int precision = LAPGPU_FP64, replace_every = 20;
+  double switch_tol = 1e-5;  // FP32 mode: residual below which the SpMV is FP64
 
   // device graph
   int *perm = nullptr, *Lptr = nullptr, *Lidx = nullptr;
@@ -59,8 +60,15 @@ struct lapgpu_ctx {
   double *Ew = nullptr, *ext = nullptr;
   TailStepD *fsteps = nullptr, *bsteps = nullptr;
   TailSlotD *fslots = nullptr, *bslots = nullptr;
-  // one PCG iteration captured as a HIP graph (rebuilt after each upload)
-  hipGraphExec_t iter_exec = nullptr;
+  // FP32-inner mode copies (built by lapgpu_set_precision)
+  bool f32_ready = false;
+  float *Lval32 = nullptr, *Fw32 = nullptr, *Bw32 = nullptr, *s32 = nullptr,
+        *Ew32 = nullptr, *y32 = nullptr, *ext32 = nullptr, *pf = nullptr;
+  TailSlotF *fslots32 = nullptr, *bslots32 = nullptr;
+  double* r0 = nullptr;
+  // one PCG iteration captured as a HIP graph (rebuilt after each upload);
+  // iter_exec_rep is the FP32-mode iteration with a residual replacement
+  hipGraphExec_t iter_exec = nullptr, iter_exec_rep = nullptr;
   bool use_graph = true;
 
   // workspace
@@ -136,76 +144,125 @@ int grid_for(int64_t items, int per_block) {
 
 // ---- templated launch helpers ----------------------------------------------
 
+#define LAPGPU_DISPATCH_G(g, F, ...)            \
+  switch (g) {                                  \
+    case 4: F<4>(__VA_ARGS__); break;           \
+    case 8: F<8>(__VA_ARGS__); break;           \
+    case 16: F<16>(__VA_ARGS__); break;         \
+    case 32: F<32>(__VA_ARGS__); break;         \
+    default: F<64>(__VA_ARGS__); break;         \
+  }
+
 template <int G>
 void launch_spmv(lapgpu_ctx* c, const double* p, double* q) {
-  hipLaunchKernelGGL(k_spmv_dot<G>, dim3(kRedBlocks), dim3(kBlock), 0,
-                     c->stream, (int)c->n, c->Lptr, c->Lidx, c->Lval, p, q,
-                     c->part, c->st);
+  if (c->precision == LAPGPU_FP32_INNER)
+    hipLaunchKernelGGL(k_spmv_dot_mixed<G>, dim3(kRedBlocks), dim3(kBlock), 0, c->stream,
+                       (int)c->n, c->Lptr, c->Lidx, c->Lval32, c->pf, c->Lval, p, q, c->part,
+                       c->st);
+  else
+    hipLaunchKernelGGL((k_spmv_dot<G, double>), dim3(kRedBlocks), dim3(kBlock), 0,
+                       c->stream, (int)c->n, c->Lptr, c->Lidx, c->Lval, p, p, q,
+                       c->part, c->st);
 }
 
+// q = la * p (p.q partials); FP32 mode reads the FP32 copies of la and p
 void spmv(lapgpu_ctx* c, const double* p, double* q) {
-  switch (c->G_spmv) {
-    case 4: launch_spmv<4>(c, p, q); break;
-    case 8: launch_spmv<8>(c, p, q); break;
-    case 16: launch_spmv<16>(c, p, q); break;
-    case 32: launch_spmv<32>(c, p, q); break;
-    default: launch_spmv<64>(c, p, q); break;
-  }
+  LAPGPU_DISPATCH_G(c->G_spmv, launch_spmv, c, p, q);
 }
 
 template <int G>
-void launch_sweeps(lapgpu_ctx* c, const double* rin, double* y) {
+void launch_resid(lapgpu_ctx* c, int always) {
+  hipLaunchKernelGGL(k_resid<G>, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, (int)c->n,
+                     always, c->Lptr, c->Lidx, c->Lval, c->x, c->r0, c->r, c->part, c->st);
+}
+
+template <int G, typename T, typename S>
+void launch_sweeps(lapgpu_ctx* c, const double* rin, T* y, const T* Fw, const T* Bw,
+                   const T* sv, const T* Ew, T* ext, const S* fslots, const S* bslots) {
   const Plan& P = c->plan;
   const int64_t tl = P.tail_level;
   for (int64_t l = 0; l < tl; ++l) {
     const int lo = P.lvl[l], hi = P.lvl[l + 1];
-    hipLaunchKernelGGL(k_fwd_level<G>, dim3(grid_for((int64_t)(hi - lo) * G, kBlock)),
-                       dim3(kBlock), 0, c->stream, lo, hi, c->Fptr, c->Fidx,
-                       c->Fw, rin, y, c->st);
+    hipLaunchKernelGGL((k_fwd_level<G, T>), dim3(grid_for((int64_t)(hi - lo) * G, kBlock)),
+                       dim3(kBlock), 0, c->stream, lo, hi, c->Fptr, c->Fidx, Fw, rin, y,
+                       c->st);
   }
-  const TailProgram& T = P.tail;
-  if (T.NT > 0) {
-    hipLaunchKernelGGL(k_tail_ext<G>, dim3(grid_for(T.NT * G, kBlock)), dim3(kBlock), 0,
-                       c->stream, (int)T.T0, (int)T.NT, c->Eptr, c->Eidx, c->Ew, rin,
-                       y, c->ext, c->st);
-    hipLaunchKernelGGL(k_tail, dim3(1), dim3(kTailThreads), 0, c->stream, (int)T.T0,
-                       (int)T.NT, c->ext, c->s, y, c->fsteps, (int)T.fsteps.size(),
-                       c->fslots, c->bsteps, (int)T.bsteps.size(), c->bslots, c->st);
+  const TailProgram& TP = P.tail;
+  if (TP.NT > 0) {
+    hipLaunchKernelGGL((k_tail_ext<G, T>), dim3(grid_for(TP.NT * G, kBlock)), dim3(kBlock), 0,
+                       c->stream, (int)TP.T0, (int)TP.NT, c->Eptr, c->Eidx, Ew, rin, y, ext,
+                       c->st);
+    hipLaunchKernelGGL((k_tail<T, S>), dim3(1), dim3(kTailThreads), 0, c->stream, (int)TP.T0,
+                       (int)TP.NT, ext, sv, y, c->fsteps, (int)TP.fsteps.size(), fslots,
+                       c->bsteps, (int)TP.bsteps.size(), bslots, c->st);
   }
   for (int64_t l = tl - 1; l >= 0; --l) {
     const int lo = P.lvl[l], hi = P.lvl[l + 1];
-    hipLaunchKernelGGL(k_bwd_level<G>, dim3(grid_for((int64_t)(hi - lo) * G, kBlock)),
-                       dim3(kBlock), 0, c->stream, lo, hi, c->Bptr, c->Bidx,
-                       c->Bw, c->s, y, c->st);
+    hipLaunchKernelGGL((k_bwd_level<G, T>), dim3(grid_for((int64_t)(hi - lo) * G, kBlock)),
+                       dim3(kBlock), 0, c->stream, lo, hi, c->Bptr, c->Bidx, Bw, sv, y, c->st);
   }
 }
 
-// y = forward, diagonal and backward sweeps of r (no mean removal)
-void sweeps(lapgpu_ctx* c, const double* rin, double* y) {
-  switch (c->G_ldl) {
-    case 4: launch_sweeps<4>(c, rin, y); break;
-    case 8: launch_sweeps<8>(c, rin, y); break;
-    case 16: launch_sweeps<16>(c, rin, y); break;
-    case 32: launch_sweeps<32>(c, rin, y); break;
-    default: launch_sweeps<64>(c, rin, y); break;
-  }
+template <int G>
+void launch_sweeps_mode(lapgpu_ctx* c, const double* rin) {
+  if (c->precision == LAPGPU_FP32_INNER)
+    launch_sweeps<G, float, TailSlotF>(c, rin, c->y32, c->Fw32, c->Bw32, c->s32, c->Ew32,
+                                       c->ext32, c->fslots32, c->bslots32);
+  else
+    launch_sweeps<G, double, TailSlotD>(c, rin, c->y, c->Fw, c->Bw, c->s, c->Ew, c->ext,
+                                        c->fslots, c->bslots);
+}
+
+// y (or y32) = forward, diagonal and backward sweeps of r (no mean removal)
+void sweeps(lapgpu_ctx* c, const double* rin) {
+  LAPGPU_DISPATCH_G(c->G_ldl, launch_sweeps_mode, c, rin);
+}
+
+// mean removal and z.r (sum|z|, sum|p| every 10th iteration) after sweeps
+void finish_precond(lapgpu_ctx* c, const double* rin) {
+  const int n = (int)c->n;
+  if (c->precision == LAPGPU_FP32_INNER)
+    hipLaunchKernelGGL(k_vsum<float>, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
+                       c->y32, c->part, c->st);
+  else
+    hipLaunchKernelGGL(k_vsum<double>, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
+                       c->y, c->part, c->st);
+  hipLaunchKernelGGL(k_scalar_mu, dim3(1), dim3(kScalarThreads), 0, c->stream, n, c->part,
+                     c->st);
+  if (c->precision == LAPGPU_FP32_INNER)
+    hipLaunchKernelGGL(k_zdot<float>, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
+                       c->y32, rin, c->p, c->z, c->part, c->st);
+  else
+    hipLaunchKernelGGL(k_zdot<double>, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
+                       c->y, rin, c->p, c->z, c->part, c->st);
 }
 
 // z = LDLsolver(r) = sweeps then mean removal; partial z.r etc.
 void precond(lapgpu_ctx* c, const double* rin) {
-  sweeps(c, rin, c->y);
-  const int n = (int)c->n;
-  hipLaunchKernelGGL(k_vsum, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
-                     c->y, c->part, c->st);
-  hipLaunchKernelGGL(k_scalar_mu, dim3(1), dim3(kScalarThreads), 0, c->stream, n,
-                     c->part, c->st);
-  hipLaunchKernelGGL(k_zdot, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
-                     c->y, rin, c->p, c->z, c->part, c->st);
+  sweeps(c, rin);
+  finish_precond(c, rin);
+}
+
+void drop_graphs(lapgpu_ctx* c) {
+  if (c->iter_exec) (void)hipGraphExecDestroy(c->iter_exec);
+  if (c->iter_exec_rep) (void)hipGraphExecDestroy(c->iter_exec_rep);
+  c->iter_exec = c->iter_exec_rep = nullptr;
+}
+
+void free_f32(lapgpu_ctx* c) {
+  void* fp[] = {c->Lval32, c->Fw32, c->Bw32, c->s32, c->Ew32, c->y32, c->ext32, c->pf,
+                c->fslots32, c->bslots32, c->r0};
+  for (void* a : fp)
+    if (a) (void)hipFree(a);
+  c->Lval32 = c->Fw32 = c->Bw32 = c->s32 = c->Ew32 = c->y32 = c->ext32 = c->pf = nullptr;
+  c->fslots32 = c->bslots32 = nullptr;
+  c->r0 = nullptr;
+  c->f32_ready = false;
 }
 
 int free_graph(lapgpu_ctx* c) {
-  if (c->iter_exec) (void)hipGraphExecDestroy(c->iter_exec);
-  c->iter_exec = nullptr;
+  drop_graphs(c);
+  free_f32(c);
   void* tp[] = {c->Eptr, c->Eidx, c->Ew, c->ext, c->fsteps, c->bsteps, c->fslots, c->bslots};
   for (void* a : tp)
     if (a) (void)hipFree(a);
@@ -223,6 +280,40 @@ int free_graph(lapgpu_ctx* c) {
   c->perm = c->Lptr = c->Lidx = c->Fptr = c->Fidx = c->Bptr = c->Bidx = c->lvl = nullptr;
   c->Lval = c->Fw = c->Bw = c->s = c->b = c->r = c->x = c->bestx = c->p =
       c->q = c->y = c->z = c->io = nullptr;
+  return 0;
+}
+
+template <typename D, typename S>
+std::vector<D> narrow(const std::vector<S>& v) {
+  std::vector<D> o(v.size());
+  for (size_t i = 0; i < v.size(); ++i) o[i] = (D)v[i];
+  return o;
+}
+
+// FP32 copies of la, the sweep weights and the tail slots; FP32 sweep and p
+// buffers; r0 for the FP64 residual replacement.
+int prepare_f32(lapgpu_ctx* c) {
+  if (c->f32_ready) return 0;
+  const Plan& P = c->plan;
+  const int64_t n = P.n;
+  int rc;
+  if ((rc = dev_upload(c, &c->Lval32, narrow<float>(P.Lval)))) return rc;
+  if ((rc = dev_upload(c, &c->Fw32, narrow<float>(P.Fw)))) return rc;
+  if ((rc = dev_upload(c, &c->Bw32, narrow<float>(P.Bw)))) return rc;
+  if ((rc = dev_upload(c, &c->s32, narrow<float>(P.s)))) return rc;
+  if ((rc = dev_upload(c, &c->Ew32, narrow<float>(P.tail.Ew)))) return rc;
+  auto slots32 = [](const std::vector<TailSlot>& v) {
+    std::vector<TailSlotF> o(v.size());
+    for (size_t i = 0; i < v.size(); ++i) o[i] = TailSlotF{(float)v[i].w, v[i].src, v[i].meta};
+    return o;
+  };
+  if ((rc = dev_upload(c, &c->fslots32, slots32(P.tail.fslots)))) return rc;
+  if ((rc = dev_upload(c, &c->bslots32, slots32(P.tail.bslots)))) return rc;
+  if ((rc = dev_alloc(c, &c->y32, (size_t)n))) return rc;
+  if ((rc = dev_alloc(c, &c->pf, (size_t)n))) return rc;
+  if ((rc = dev_alloc(c, &c->r0, (size_t)n))) return rc;
+  if ((rc = dev_alloc(c, &c->ext32, (size_t)std::max<int64_t>(P.tail.NT, 1)))) return rc;
+  c->f32_ready = true;
   return 0;
 }
 
@@ -260,9 +351,11 @@ int upload_plan(lapgpu_ctx* c) {
     if ((rc = dev_upload(c, &c->fslots, fl))) return rc;
     if ((rc = dev_upload(c, &c->bslots, bl))) return rc;
   }
-  if (c->iter_exec) {
-    HIPC(c, hipGraphExecDestroy(c->iter_exec));
-    c->iter_exec = nullptr;
+  drop_graphs(c);
+  free_f32(c);
+  if (c->precision == LAPGPU_FP32_INNER) {
+    int rc2 = prepare_f32(c);
+    if (rc2) return rc2;
   }
   double** ws[] = {&c->b, &c->r, &c->x, &c->bestx, &c->p, &c->q, &c->y, &c->z, &c->io};
   for (double** w : ws)
@@ -308,6 +401,8 @@ void init_state(lapgpu_ctx* c, double tol, int64_t maxits, double maxtime,
   h.stag_test = stag_test;
   h.stag_fac = 1.0 - 1.0 / (double)stag_test;  // -Inf for stag_test == 0
   h.center = (flags & LAPGPU_NO_CENTER) ? 0 : 1;
+  h.fp32 = c->precision == LAPGPU_FP32_INNER ? 1 : 0;
+  h.switch_tol = c->switch_tol;
   h.has_maxtime = (maxtime > 0 && std::isfinite(maxtime)) ? 1 : 0;
   h.maxticks = h.has_maxtime ? (uint64_t)(maxtime * c->wallclock_khz * 1e3) : 0;
   h.bestnr = 1.0;
@@ -328,9 +423,11 @@ const char* stop_message(int reason) {
   }
 }
 
-// one PCG iteration (src/pcg.jl:288-431)
-void enqueue_iteration(lapgpu_ctx* c, bool prof) {
+// one PCG iteration (src/pcg.jl:288-431); `replace`: FP32-mode iteration
+// that ends with an FP64 residual replacement
+void enqueue_iteration(lapgpu_ctx* c, bool prof, bool replace) {
   const int n = (int)c->n;
+  const bool f32 = c->precision == LAPGPU_FP32_INNER;
   if (prof) (void)hipEventRecord(c->ev[0], c->stream);
   spmv(c, c->p, c->q);
   if (prof) (void)hipEventRecord(c->ev[1], c->stream);
@@ -338,17 +435,22 @@ void enqueue_iteration(lapgpu_ctx* c, bool prof) {
   hipLaunchKernelGGL(k_update, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
                      c->x, c->bestx, c->p, c->q, c->r, c->part, c->st);
   hipLaunchKernelGGL(k_scalar_nr, dim3(1), dim3(kScalarThreads), 0, c->stream, c->part, c->st);
+  if (f32) {
+    LAPGPU_DISPATCH_G(c->G_spmv, launch_resid, c, replace ? 1 : 0);
+    hipLaunchKernelGGL(k_scalar_resid, dim3(1), dim3(kScalarThreads), 0, c->stream,
+                       replace ? 1 : 0, c->part, c->st);
+  }
   if (prof) (void)hipEventRecord(c->ev[2], c->stream);
-  sweeps(c, c->r, c->y);
+  sweeps(c, c->r);
   if (prof) (void)hipEventRecord(c->ev[3], c->stream);
-  hipLaunchKernelGGL(k_vsum, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
-                     c->y, c->part, c->st);
-  hipLaunchKernelGGL(k_scalar_mu, dim3(1), dim3(kScalarThreads), 0, c->stream, n, c->part, c->st);
-  hipLaunchKernelGGL(k_zdot, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
-                     c->y, c->r, c->p, c->z, c->part, c->st);
+  finish_precond(c, c->r);
   hipLaunchKernelGGL(k_scalar_rho, dim3(1), dim3(kScalarThreads), 0, c->stream, c->part, c->st);
-  hipLaunchKernelGGL(k_pupdate, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
-                     c->z, c->p, c->st);
+  if (f32)
+    hipLaunchKernelGGL(k_pupdate_f, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
+                       c->z, c->p, c->pf, c->st);
+  else
+    hipLaunchKernelGGL(k_pupdate, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
+                       c->z, c->p, c->st);
   if (prof) (void)hipEventRecord(c->ev[4], c->stream);
 }
 
@@ -370,10 +472,17 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
   hipLaunchKernelGGL(k_center, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n, c->b,
                      c->r, c->x, c->bestx, c->part, c->st);
   hipLaunchKernelGGL(k_scalar_nb, dim3(1), dim3(kScalarThreads), 0, c->stream, c->part, c->st);
+  const bool f32 = c->precision == LAPGPU_FP32_INNER;
+  if (f32)
+    HIPC(c, hipMemcpyAsync(c->r0, c->r, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
   precond(c, c->r);
   hipLaunchKernelGGL(k_scalar_rho0, dim3(1), dim3(kScalarThreads), 0, c->stream, c->part, c->st);
-  hipLaunchKernelGGL(k_copy, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n, c->z,
-                     c->p, c->st);
+  if (f32)
+    hipLaunchKernelGGL(k_copy_f, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n, c->z,
+                       c->p, c->pf, c->st);
+  else
+    hipLaunchKernelGGL(k_copy, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n, c->z,
+                       c->p, c->st);
   HIPC(c, hipGetLastError());
 
   // the setup may already have stopped (b == 0, maxits == 0)
@@ -388,20 +497,28 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
   double acc_spmv = 0, acc_ldl = 0, acc_vec = 0;
   int64_t enq = 0, polled = 0;
   bool done = c->hst->stopped != 0;
-  if (!prof && c->use_graph && !c->iter_exec) {
-    hipGraph_t g = nullptr;
-    HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_iteration(c, false);
-    HIPC(c, hipStreamEndCapture(c->stream, &g));
-    hipError_t ie = hipGraphInstantiate(&c->iter_exec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    HIPC(c, ie);
+  const int rep_every = f32 ? c->replace_every : 0;
+  if (!prof && c->use_graph) {
+    for (int v = 0; v < (f32 ? 2 : 1); ++v) {
+      hipGraphExec_t* ex = v ? &c->iter_exec_rep : &c->iter_exec;
+      if (*ex) continue;
+      hipGraph_t g = nullptr;
+      HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_iteration(c, false, v == 1);
+      HIPC(c, hipStreamEndCapture(c->stream, &g));
+      hipError_t ie = hipGraphInstantiate(ex, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIPC(c, ie);
+    }
   }
   while (!done) {
-    if (!prof && c->iter_exec)
-      HIPC(c, hipGraphLaunch(c->iter_exec, c->stream));
+    // iteration number enq + 1 of this solve (itcnt on the device)
+    const bool replace = rep_every > 0 && ((enq + 1) % rep_every == 0);
+    hipGraphExec_t ex = replace ? c->iter_exec_rep : c->iter_exec;
+    if (!prof && ex)
+      HIPC(c, hipGraphLaunch(ex, c->stream));
     else
-      enqueue_iteration(c, prof);
+      enqueue_iteration(c, prof, replace);
     const int slot = (int)(enq % lapgpu_ctx::kRing);
     HIPC(c, hipMemcpyAsync(&c->hflags[slot], &c->st->stopped, sizeof(int32_t),
                            hipMemcpyDeviceToHost, c->stream));
@@ -506,6 +623,7 @@ int lapgpu_create(lapgpu_ctx** out, int device) {
     if ((e = hipEventCreate(&ev)) != hipSuccess) return fail("hipEventCreate", e);
   if (const char* tw = getenv("LAPGPU_TAIL_STEPS")) c->max_tail_steps = atoll(tw);
   if (const char* ug = getenv("LAPGPU_NO_GRAPH")) c->use_graph = atoi(ug) == 0;
+  if (const char* sw = getenv("LAPGPU_SWITCH_TOL")) c->switch_tol = atof(sw);
   *out = c;
   return 0;
 }
@@ -631,6 +749,9 @@ int lapgpu_spmv(lapgpu_ctx* c, const double* xin, double* yout, int64_t n) {
   HIPC(c, hipMemcpyAsync(c->io, xin, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_gather_perm, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, (int)n,
                      c->perm, c->io, c->p, c->part);
+  if (c->precision == LAPGPU_FP32_INNER && c->f32_ready)
+    hipLaunchKernelGGL(k_copy_f, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, (int)n, c->p,
+                       c->p, c->pf, c->st);
   spmv(c, c->p, c->q);
   hipLaunchKernelGGL(k_scatter_perm, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, (int)n,
                      c->perm, c->q, c->io);
@@ -643,9 +764,18 @@ int lapgpu_spmv(lapgpu_ctx* c, const double* xin, double* yout, int64_t n) {
 
 int lapgpu_set_precision(lapgpu_ctx* c, int mode, int replace_every) {
   if (!c) return -1;
-  if (mode != LAPGPU_FP64) return set_err(c, -1, "set_precision: only LAPGPU_FP64 is built");
+  if (mode != LAPGPU_FP64 && mode != LAPGPU_FP32_INNER)
+    return set_err(c, -1, "set_precision: unknown mode");
+  HIPC(c, hipSetDevice(c->device));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  drop_graphs(c);
   c->precision = mode;
   c->replace_every = replace_every > 0 ? replace_every : 20;
+  if (mode == LAPGPU_FP32_INNER && c->has_L) {
+    int rc = prepare_f32(c);
+    if (rc) return rc;
+  }
+  c->err.clear();
   return 0;
 }
 

@@ -1,0 +1,494 @@
+// lapgpu_batch.hip — device-resident graph store and batched solves, one
+// workgroup per (graph, right-hand side) request (kernels: lapgpu_wg.hpp).
+//
+// A batch holds any number of uploaded graphs (la + LDLinv, level-ordered,
+// with the whole LDLinv apply as one slot program).  lapgpu_batch_solve runs
+// a set of requests of uneven sizes in ONE launch: many small graphs, many
+// right-hand sides of one graph, or both; requests are ordered largest
+// first so the long ones start early.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "lapgpu.h"
+#include "lapgpu_plan.hpp"
+#include "lapgpu_wg.hpp"
+
+using namespace lapgpu;
+
+namespace {
+
+struct GraphStore {
+  int n = 0, nf = 0, nb = 0;
+  int64_t bytes = 0;
+  int *perm = nullptr, *Lptr = nullptr, *Lidx = nullptr;
+  double *Lval = nullptr, *s = nullptr;
+  TailStepD *fsteps = nullptr, *bsteps = nullptr;
+  TailSlotD *fslots = nullptr, *bslots = nullptr;
+  bool live = false;
+};
+
+std::string g_batch_err;
+constexpr int kBatchSlots = 4;
+
+}  // namespace
+
+// One launch slot: its own stream, request table and workspace, so that
+// launches on different slots overlap on the device.
+struct BatchSlot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  WgReqD* d_reqs = nullptr;
+  size_t reqs_cap = 0;
+  WgReqD* h_reqs = nullptr;  // pinned
+  size_t h_cap = 0;
+  double* d_work = nullptr;
+  size_t work_cap = 0;
+  std::vector<int64_t> order;
+  int64_t nreq = 0;
+  bool busy = false;
+  // host-buffer launches: device io and pinned staging
+  double* d_io = nullptr;
+  size_t io_cap = 0;
+  double* h_io = nullptr;  // pinned
+  size_t hio_cap = 0;
+  size_t io_len = 0;       // doubles of x to hand back (0: device launch)
+};
+
+struct lapgpu_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;  // uploads and the synchronous entry points
+  double wallclock_khz = 100000.0;
+  std::string err;
+  std::vector<GraphStore> graphs;
+  WgGraphD* d_graphs = nullptr;
+  size_t d_graphs_cap = 0;
+  bool graphs_dirty = true;
+  std::vector<BatchSlot> slots;
+  double* d_io = nullptr;
+  size_t d_io_cap = 0;
+};
+
+namespace {
+
+int berr(lapgpu_batch* B, int code, const std::string& m) {
+  if (B)
+    B->err = m;
+  else
+    g_batch_err = m;
+  return code;
+}
+
+#define BHIP(B, call)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return berr(B, -10, std::string(#call) + ": " + hipGetErrorString(e_));  \
+  } while (0)
+
+template <typename T>
+int up(lapgpu_batch* B, T** dst, const T* src, size_t count, int64_t& bytes) {
+  const size_t nb = sizeof(T) * (count ? count : 1);
+  BHIP(B, hipMalloc((void**)dst, nb));
+  if (count) BHIP(B, hipMemcpy(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice));
+  bytes += (int64_t)nb;
+  return 0;
+}
+
+template <typename T>
+int grow(lapgpu_batch* B, T** p, size_t* cap, size_t want) {
+  if (want <= *cap) return 0;
+  if (*p) BHIP(B, hipFree(*p));
+  *p = nullptr;
+  size_t c = std::max(want, *cap * 2);
+  BHIP(B, hipMalloc((void**)p, sizeof(T) * c));
+  *cap = c;
+  return 0;
+}
+
+void free_store(GraphStore& G) {
+  void* ps[] = {G.perm, G.Lptr, G.Lidx, G.Lval, G.s, G.fsteps, G.bsteps, G.fslots, G.bslots};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  G = GraphStore();
+}
+
+}  // namespace
+
+extern "C" {
+
+int lapgpu_batch_create(lapgpu_batch** out, int device) {
+  if (!out) return berr(nullptr, -1, "lapgpu_batch_create: null output");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    return berr(nullptr, -10, "lapgpu_batch_create: no HIP device visible");
+  if (device < 0 || device >= count)
+    return berr(nullptr, -1, "lapgpu_batch_create: device index out of range");
+  lapgpu_batch* B = new lapgpu_batch();
+  B->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    g_batch_err = std::string("lapgpu_batch_create: ") + hipGetErrorString(e);
+    delete B;
+    return -10;
+  }
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+    B->wallclock_khz = khz;
+  B->slots.resize(kBatchSlots);
+  for (BatchSlot& S : B->slots) {
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&S.done, hipEventDisableTiming);
+  }
+  if (e != hipSuccess) {
+    g_batch_err = std::string("lapgpu_batch_create: ") + hipGetErrorString(e);
+    lapgpu_batch_destroy(B);
+    return -10;
+  }
+  *out = B;
+  return 0;
+}
+
+void lapgpu_batch_destroy(lapgpu_batch* B) {
+  if (!B) return;
+  (void)hipSetDevice(B->device);
+  if (B->stream) (void)hipStreamSynchronize(B->stream);
+  for (BatchSlot& S : B->slots) {
+    if (S.stream) (void)hipStreamSynchronize(S.stream);
+    if (S.d_reqs) (void)hipFree(S.d_reqs);
+    if (S.d_work) (void)hipFree(S.d_work);
+    if (S.h_reqs) (void)hipHostFree(S.h_reqs);
+    if (S.d_io) (void)hipFree(S.d_io);
+    if (S.h_io) (void)hipHostFree(S.h_io);
+    if (S.done) (void)hipEventDestroy(S.done);
+    if (S.stream) (void)hipStreamDestroy(S.stream);
+  }
+  for (GraphStore& G : B->graphs) free_store(G);
+  void* ps[] = {B->d_graphs, B->d_io};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  if (B->stream) (void)hipStreamDestroy(B->stream);
+  delete B;
+}
+
+const char* lapgpu_batch_last_error(lapgpu_batch* B) {
+  return B ? B->err.c_str() : g_batch_err.c_str();
+}
+
+int lapgpu_batch_add_graph(lapgpu_batch* B, int64_t n, const int64_t* colptr,
+                           const int64_t* rowval, const double* nzval,
+                           int64_t nnzF, const int64_t* col,
+                           const int64_t* fcolptr, const int64_t* frowval,
+                           const double* fval, const double* d, int index_base,
+                           int32_t* graph_id) {
+  if (!B || !graph_id) return berr(B, -1, "lapgpu_batch_add_graph: null argument");
+  BHIP(B, hipSetDevice(B->device));
+  std::string err;
+  HostCsr L;
+  if (!csc_to_csr(n, colptr, rowval, nzval, index_base, L, err)) return berr(B, -1, err);
+  Plan P;
+  if (!build_plan(L, nnzF, col, fcolptr, frowval, fval, d, index_base, 0, P, err))
+    return berr(B, -1, err);
+  if (!build_tail(P, INT64_MAX, INT64_MAX) || P.tail.T0 != 0)
+    return berr(B, -1, "lapgpu_batch_add_graph: could not build the slot program");
+  GraphStore G;
+  G.n = (int)n;
+  G.nf = (int)P.tail.fsteps.size();
+  G.nb = (int)P.tail.bsteps.size();
+  static_assert(sizeof(TailStepD) == sizeof(TailStep), "step layout");
+  static_assert(sizeof(TailSlotD) == sizeof(TailSlot), "slot layout");
+  int rc = 0;
+  if ((rc = up(B, &G.perm, P.perm.data(), P.perm.size(), G.bytes)) ||
+      (rc = up(B, &G.Lptr, P.Lptr.data(), P.Lptr.size(), G.bytes)) ||
+      (rc = up(B, &G.Lidx, P.Lidx.data(), P.Lidx.size(), G.bytes)) ||
+      (rc = up(B, &G.Lval, P.Lval.data(), P.Lval.size(), G.bytes)) ||
+      (rc = up(B, &G.s, P.s.data(), P.s.size(), G.bytes)) ||
+      (rc = up(B, &G.fsteps, (const TailStepD*)P.tail.fsteps.data(), P.tail.fsteps.size(), G.bytes)) ||
+      (rc = up(B, &G.bsteps, (const TailStepD*)P.tail.bsteps.data(), P.tail.bsteps.size(), G.bytes)) ||
+      (rc = up(B, &G.fslots, (const TailSlotD*)P.tail.fslots.data(), P.tail.fslots.size(), G.bytes)) ||
+      (rc = up(B, &G.bslots, (const TailSlotD*)P.tail.bslots.data(), P.tail.bslots.size(), G.bytes))) {
+    free_store(G);
+    return rc;
+  }
+  G.live = true;
+  B->graphs.push_back(G);
+  B->graphs_dirty = true;
+  *graph_id = (int32_t)(B->graphs.size() - 1);
+  B->err.clear();
+  return 0;
+}
+
+int lapgpu_batch_remove_graph(lapgpu_batch* B, int32_t graph_id) {
+  if (!B || graph_id < 0 || graph_id >= (int32_t)B->graphs.size() || !B->graphs[graph_id].live)
+    return berr(B, -1, "lapgpu_batch_remove_graph: unknown graph");
+  BHIP(B, hipSetDevice(B->device));
+  BHIP(B, hipStreamSynchronize(B->stream));
+  for (BatchSlot& S : B->slots)
+    if (S.busy) BHIP(B, hipStreamSynchronize(S.stream));
+  free_store(B->graphs[graph_id]);
+  B->graphs_dirty = true;
+  return 0;
+}
+
+int lapgpu_batch_graph_info(lapgpu_batch* B, int32_t graph_id, int64_t* n,
+                            int64_t* device_bytes) {
+  if (!B || graph_id < 0 || graph_id >= (int32_t)B->graphs.size() || !B->graphs[graph_id].live)
+    return berr(B, -1, "lapgpu_batch_graph_info: unknown graph");
+  if (n) *n = B->graphs[graph_id].n;
+  if (device_bytes) *device_bytes = B->graphs[graph_id].bytes;
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+int free_slot(lapgpu_batch* B) {
+  for (int k = 0; k < (int)B->slots.size(); ++k)
+    if (!B->slots[k].busy) return k;
+  return -1;
+}
+
+// Enqueue nreq requests on slot `slot` (stream-ordered after whatever the
+// caller already put on that stream).
+int launch_on(lapgpu_batch* B, int slot, int64_t nreq, const int32_t* graph_ids,
+              const double* const* d_b, double* const* d_x, double tol, int64_t maxits,
+              double maxtime, int stag_test, int flags) {
+  BatchSlot& S = B->slots[slot];
+  const int ng = (int)B->graphs.size();
+  int rc;
+  if (B->graphs_dirty) {
+    // other slots may still read the descriptor table: let them finish
+    for (BatchSlot& o : B->slots)
+      if (o.busy) BHIP(B, hipStreamSynchronize(o.stream));
+    std::vector<WgGraphD> hg(ng);
+    for (int i = 0; i < ng; ++i) {
+      const GraphStore& G = B->graphs[i];
+      hg[i] = WgGraphD{G.n, G.nf, G.nb, 0, G.perm, G.Lptr, G.Lidx, G.Lval, G.s,
+                       G.fsteps, G.fslots, G.bsteps, G.bslots};
+    }
+    if ((rc = grow(B, &B->d_graphs, &B->d_graphs_cap, (size_t)ng))) return rc;
+    BHIP(B, hipMemcpy(B->d_graphs, hg.data(), sizeof(WgGraphD) * ng, hipMemcpyHostToDevice));
+    B->graphs_dirty = false;
+  }
+  // largest first
+  S.order.resize(nreq);
+  std::iota(S.order.begin(), S.order.end(), 0);
+  std::stable_sort(S.order.begin(), S.order.end(), [&](int64_t a, int64_t b) {
+    return B->graphs[graph_ids[a]].n > B->graphs[graph_ids[b]].n;
+  });
+  size_t work = 0;
+  for (int64_t i = 0; i < nreq; ++i) work += 8 * (size_t)B->graphs[graph_ids[i]].n;
+  if ((rc = grow(B, &S.d_work, &S.work_cap, work))) return rc;
+  if ((rc = grow(B, &S.d_reqs, &S.reqs_cap, (size_t)nreq))) return rc;
+  if ((size_t)nreq > S.h_cap) {
+    if (S.h_reqs) BHIP(B, hipHostFree(S.h_reqs));
+    S.h_reqs = nullptr;
+    BHIP(B, hipHostMalloc((void**)&S.h_reqs, sizeof(WgReqD) * nreq, hipHostMallocDefault));
+    S.h_cap = (size_t)nreq;
+  }
+  const bool has_mt = maxtime > 0 && std::isfinite(maxtime);
+  size_t off = 0;
+  for (int64_t k = 0; k < nreq; ++k) {
+    const int64_t i = S.order[k];
+    WgReqD R{};
+    R.b = d_b[i];
+    R.x = d_x[i];
+    R.work = S.d_work + off;
+    off += 8 * (size_t)B->graphs[graph_ids[i]].n;
+    R.tol = tol;
+    R.maxits = maxits;
+    R.maxticks = has_mt ? (unsigned long long)(maxtime * B->wallclock_khz * 1e3) : 0ull;
+    R.graph = graph_ids[i];
+    R.stag_test = stag_test;
+    R.center = (flags & LAPGPU_NO_CENTER) ? 0 : 1;
+    R.has_maxtime = has_mt ? 1 : 0;
+    S.h_reqs[k] = R;
+  }
+  BHIP(B, hipMemcpyAsync(S.d_reqs, S.h_reqs, sizeof(WgReqD) * nreq, hipMemcpyHostToDevice,
+                         S.stream));
+  hipLaunchKernelGGL(k_wg_pcg, dim3((unsigned)nreq), dim3(kWgThreads), 0, S.stream,
+                     B->d_graphs, S.d_reqs);
+  BHIP(B, hipGetLastError());
+  BHIP(B, hipMemcpyAsync(S.h_reqs, S.d_reqs, sizeof(WgReqD) * nreq, hipMemcpyDeviceToHost,
+                         S.stream));
+  BHIP(B, hipEventRecord(S.done, S.stream));
+  S.nreq = nreq;
+  S.busy = true;
+  return 0;
+}
+
+int check_ids(lapgpu_batch* B, int64_t nreq, const int32_t* graph_ids) {
+  const int ng = (int)B->graphs.size();
+  for (int64_t i = 0; i < nreq; ++i)
+    if (graph_ids[i] < 0 || graph_ids[i] >= ng || !B->graphs[graph_ids[i]].live)
+      return berr(B, -1, "lapgpu_batch: request names an unknown graph");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lapgpu_batch_submit(lapgpu_batch* B, int64_t nreq, const int32_t* graph_ids,
+                        const double* const* d_b, double* const* d_x, double tol,
+                        int64_t maxits, double maxtime, int stag_test, int flags,
+                        int32_t* slot_out) {
+  if (!B || nreq <= 0 || !graph_ids || !d_b || !d_x || !slot_out)
+    return berr(B, -1, "lapgpu_batch_submit: bad arguments");
+  BHIP(B, hipSetDevice(B->device));
+  int rc;
+  if ((rc = check_ids(B, nreq, graph_ids))) return rc;
+  const int slot = free_slot(B);
+  if (slot < 0) return LAPGPU_BUSY;
+  B->slots[slot].io_len = 0;
+  if ((rc = launch_on(B, slot, nreq, graph_ids, d_b, d_x, tol, maxits, maxtime, stag_test,
+                      flags)))
+    return rc;
+  *slot_out = slot;
+  B->err.clear();
+  return 0;
+}
+
+int lapgpu_batch_submit_host(lapgpu_batch* B, int64_t nreq, const int32_t* graph_ids,
+                             const double* b_concat, double tol, int64_t maxits,
+                             double maxtime, int stag_test, int flags, int32_t* slot_out) {
+  if (!B || nreq <= 0 || !graph_ids || !b_concat || !slot_out)
+    return berr(B, -1, "lapgpu_batch_submit_host: bad arguments");
+  BHIP(B, hipSetDevice(B->device));
+  int rc;
+  if ((rc = check_ids(B, nreq, graph_ids))) return rc;
+  const int slot = free_slot(B);
+  if (slot < 0) return LAPGPU_BUSY;
+  BatchSlot& S = B->slots[slot];
+  size_t total = 0;
+  for (int64_t i = 0; i < nreq; ++i) total += (size_t)B->graphs[graph_ids[i]].n;
+  if ((rc = grow(B, &S.d_io, &S.io_cap, 2 * total))) return rc;
+  if (total > S.hio_cap) {
+    if (S.h_io) BHIP(B, hipHostFree(S.h_io));
+    S.h_io = nullptr;
+    BHIP(B, hipHostMalloc((void**)&S.h_io, sizeof(double) * total, hipHostMallocDefault));
+    S.hio_cap = total;
+  }
+  std::memcpy(S.h_io, b_concat, sizeof(double) * total);
+  BHIP(B, hipMemcpyAsync(S.d_io, S.h_io, sizeof(double) * total, hipMemcpyHostToDevice, S.stream));
+  std::vector<const double*> pb(nreq);
+  std::vector<double*> px(nreq);
+  size_t off = 0;
+  for (int64_t i = 0; i < nreq; ++i) {
+    pb[i] = S.d_io + off;
+    px[i] = S.d_io + total + off;
+    off += (size_t)B->graphs[graph_ids[i]].n;
+  }
+  if ((rc = launch_on(B, slot, nreq, graph_ids, pb.data(), px.data(), tol, maxits, maxtime,
+                      stag_test, flags)))
+    return rc;
+  // x comes back into the pinned staging after the kernel
+  BHIP(B, hipMemcpyAsync(S.h_io, S.d_io + total, sizeof(double) * total, hipMemcpyDeviceToHost,
+                         S.stream));
+  BHIP(B, hipEventRecord(S.done, S.stream));
+  S.io_len = total;
+  *slot_out = slot;
+  B->err.clear();
+  return 0;
+}
+
+int lapgpu_batch_query(lapgpu_batch* B, int32_t slot, int* done) {
+  if (!B || slot < 0 || slot >= (int32_t)B->slots.size() || !done)
+    return berr(B, -1, "lapgpu_batch_query: bad arguments");
+  BatchSlot& S = B->slots[slot];
+  if (!S.busy) return berr(B, -1, "lapgpu_batch_query: slot has no launch");
+  hipError_t e = hipEventQuery(S.done);
+  if (e == hipSuccess)
+    *done = 1;
+  else if (e == hipErrorNotReady)
+    *done = 0;
+  else
+    return berr(B, -10, std::string("lapgpu_batch_query: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int lapgpu_batch_collect(lapgpu_batch* B, int32_t slot, double* x_concat, int64_t* its_out,
+                         double* relres_out, int32_t* stop_out) {
+  if (!B || slot < 0 || slot >= (int32_t)B->slots.size())
+    return berr(B, -1, "lapgpu_batch_collect: bad arguments");
+  BatchSlot& S = B->slots[slot];
+  if (!S.busy) return berr(B, -1, "lapgpu_batch_collect: slot has no launch");
+  BHIP(B, hipSetDevice(B->device));
+  BHIP(B, hipEventSynchronize(S.done));
+  if (S.io_len && x_concat) std::memcpy(x_concat, S.h_io, sizeof(double) * S.io_len);
+  for (int64_t k = 0; k < S.nreq; ++k) {
+    const int64_t i = S.order[k];
+    if (its_out) its_out[i] = S.h_reqs[k].its;
+    if (relres_out) relres_out[i] = S.h_reqs[k].nr;
+    if (stop_out) stop_out[i] = S.h_reqs[k].stop;
+  }
+  S.busy = false;
+  B->err.clear();
+  return 0;
+}
+
+int lapgpu_batch_solve_device(lapgpu_batch* B, int64_t nreq,
+                              const int32_t* graph_ids,
+                              const double* const* d_b, double* const* d_x,
+                              double tol, int64_t maxits, double maxtime,
+                              int stag_test, int flags, int64_t* its_out,
+                              double* relres_out, int32_t* stop_out) {
+  if (!B || nreq < 0) return berr(B, -1, "lapgpu_batch_solve: bad arguments");
+  if (nreq == 0) return 0;
+  int32_t slot = -1;
+  int rc;
+  rc = lapgpu_batch_submit(B, nreq, graph_ids, d_b, d_x, tol, maxits, maxtime, stag_test,
+                           flags, &slot);
+  if (rc == LAPGPU_BUSY)
+    return berr(B, -2, "lapgpu_batch_solve: all launch slots hold uncollected launches");
+  if (rc) return rc;
+  return lapgpu_batch_collect(B, slot, nullptr, its_out, relres_out, stop_out);
+}
+
+int lapgpu_batch_solve(lapgpu_batch* B, int64_t nreq, const int32_t* graph_ids,
+                       const double* b_concat, double* x_concat, double tol,
+                       int64_t maxits, double maxtime, int stag_test, int flags,
+                       int64_t* its_out, double* relres_out, int32_t* stop_out) {
+  if (!B || nreq < 0 || (nreq && (!graph_ids || !b_concat || !x_concat)))
+    return berr(B, -1, "lapgpu_batch_solve: bad arguments");
+  if (nreq == 0) return 0;
+  BHIP(B, hipSetDevice(B->device));
+  const int ng = (int)B->graphs.size();
+  size_t total = 0;
+  for (int64_t i = 0; i < nreq; ++i) {
+    if (graph_ids[i] < 0 || graph_ids[i] >= ng || !B->graphs[graph_ids[i]].live)
+      return berr(B, -1, "lapgpu_batch_solve: request names an unknown graph");
+    total += (size_t)B->graphs[graph_ids[i]].n;
+  }
+  int rc;
+  if ((rc = grow(B, &B->d_io, &B->d_io_cap, 2 * total))) return rc;
+  BHIP(B, hipMemcpyAsync(B->d_io, b_concat, sizeof(double) * total, hipMemcpyHostToDevice,
+                         B->stream));
+  BHIP(B, hipStreamSynchronize(B->stream));
+  std::vector<const double*> pb(nreq);
+  std::vector<double*> px(nreq);
+  size_t off = 0;
+  for (int64_t i = 0; i < nreq; ++i) {
+    pb[i] = B->d_io + off;
+    px[i] = B->d_io + total + off;
+    off += (size_t)B->graphs[graph_ids[i]].n;
+  }
+  rc = lapgpu_batch_solve_device(B, nreq, graph_ids, pb.data(), px.data(), tol, maxits,
+                                 maxtime, stag_test, flags, its_out, relres_out, stop_out);
+  if (rc) return rc;
+  BHIP(B, hipMemcpyAsync(x_concat, B->d_io + total, sizeof(double) * total,
+                         hipMemcpyDeviceToHost, B->stream));
+  BHIP(B, hipStreamSynchronize(B->stream));
+  return 0;
+}
+
+}  // extern "C"

@@ -9,91 +9,9 @@
 // so the host can enqueue iterations ahead of the stopping decision.
 #pragma once
 
-#include <hip/hip_runtime.h>
-#include <stdint.h>
+#include "lapgpu_device.hpp"
 
 namespace lapgpu {
-
-constexpr int kWave = 64;
-constexpr int kBlock = 256;
-constexpr int kRedBlocks = 1024;  // fixed grid of the reduction kernels
-constexpr int kTailThreads = 1024;
-constexpr double kEps = 2.220446049250313e-16;  // eps(Float64)
-
-// partial-sum slots (each kRedBlocks doubles)
-enum Slot {
-  S_SUM = 0,   // sum of a vector (means)
-  S_RR = 1,    // r.r
-  S_PQ = 2,    // p.q
-  S_ZR = 3,    // z.r
-  S_AP = 4,    // sum|p|  (al check, before x update)
-  S_AX = 5,    // sum|x|
-  S_AQ = 6,    // sum|q|
-  S_AR = 7,    // sum|r|  (before r update)
-  S_AZ = 8,    // sum|z|
-  S_AP2 = 9,   // sum|p|  (beta check)
-  kNumSlots = 10
-};
-
-struct DevState {
-  double nb, rho, best_rho, al, beta, nr, bestnr, mu;
-  double tol, stag_fac, rho0;
-  int64_t itcnt, maxits, stag_test, stag_count;
-  uint64_t t1, maxticks;
-  int32_t stopped, stop_reason, pending_copy, check10;
-  int32_t center, has_maxtime, pad0, pad1;
-};
-
-// ---------------------------------------------------------------------------
-// reductions
-
-template <int G>
-__device__ __forceinline__ double group_sum(double v) {
-#pragma unroll
-  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
-  return v;
-}
-
-// Sum NV values over the block; result valid in thread 0.
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV]) {
-  __shared__ double red[NV][kBlock / kWave];
-#pragma unroll
-  for (int i = 0; i < NV; ++i) v[i] = group_sum<kWave>(v[i]);
-  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
-  if (lane == 0) {
-#pragma unroll
-    for (int i = 0; i < NV; ++i) red[i][wid] = v[i];
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      double s = 0.0;
-      for (int w = 0; w < kBlock / kWave; ++w) s += red[i][w];
-      v[i] = s;
-    }
-  }
-}
-
-// Sum of the kRedBlocks partials of one slot by a block of kScalarThreads
-// (= kRedBlocks) threads, one partial each, in a fixed tree order.  The
-// result is returned to every thread.
-constexpr int kScalarThreads = kRedBlocks;
-__device__ __forceinline__ double wave_total(const double* part, int slot) {
-  __shared__ double red[kScalarThreads / kWave];
-  double v = part[slot * kRedBlocks + threadIdx.x];
-  v = group_sum<kWave>(v);
-  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
-  __syncthreads();
-  if (lane == 0) red[wid] = v;
-  __syncthreads();
-  double s = 0.0;
-#pragma unroll
-  for (int w = 0; w < kScalarThreads / kWave; ++w) s += red[w];
-  return s;
-}
-
 // ---------------------------------------------------------------------------
 // permutation and setup
 
@@ -157,161 +75,69 @@ __global__ void k_scalar_nb(const double* __restrict__ part, DevState* st) {
 // ---------------------------------------------------------------------------
 // LDLinv sweeps (gathers over one level; G lanes per row)
 
-template <int G>
+// T is the sweep precision (double, or float in the FP32-inner mode).
+template <int G, typename T>
 __global__ void __launch_bounds__(kBlock)
 k_fwd_level(int lo, int hi, const int* __restrict__ Fptr,
-            const int* __restrict__ Fidx, const double* __restrict__ Fw,
-            const double* __restrict__ r, double* y,
+            const int* __restrict__ Fidx, const T* __restrict__ Fw,
+            const double* __restrict__ r, T* y,
             const DevState* __restrict__ st) {
   const int gid = blockIdx.x * kBlock + threadIdx.x;
   const int row = lo + gid / G;
   const int lane = gid % G;
   if (row >= hi) return;
   const int k0 = Fptr[row], k1 = Fptr[row + 1];
-  double acc = 0.0;
+  T acc = 0;
   for (int k = k0 + lane; k < k1; k += G) acc += Fw[k] * y[Fidx[k]];
   acc = group_sum<G>(acc);
-  if (lane == 0 && !st->stopped) y[row] = r[row] + acc;
+  if (lane == 0 && !st->stopped) y[row] = (T)r[row] + acc;
 }
 
-template <int G>
+template <int G, typename T>
 __global__ void __launch_bounds__(kBlock)
 k_bwd_level(int lo, int hi, const int* __restrict__ Bptr,
-            const int* __restrict__ Bidx, const double* __restrict__ Bw,
-            const double* __restrict__ s, double* y,
+            const int* __restrict__ Bidx, const T* __restrict__ Bw,
+            const T* __restrict__ s, T* y,
             const DevState* __restrict__ st) {
   const int gid = blockIdx.x * kBlock + threadIdx.x;
   const int row = lo + gid / G;
   const int lane = gid % G;
   if (row >= hi) return;
   const int k0 = Bptr[row], k1 = Bptr[row + 1];
-  double acc = 0.0;
+  T acc = 0;
   for (int k = k0 + lane; k < k1; k += G) acc += Bw[k] * y[Bidx[k]];
   acc = group_sum<G>(acc);
   if (lane == 0 && !st->stopped) y[row] = s[row] * y[row] + acc;
 }
 
-// ---------------------------------------------------------------------------
-// The narrow tail of the DAG (the last levels, at most kTailCapV vertices):
-// one workgroup holds y of all tail vertices in LDS and runs the forward
-// sweep, the diagonal scaling and the backward sweep of those levels with a
-// workgroup barrier per level.  The work of each level is a host-packed
-// "slot program" (TailSlot / TailStep of lapgpu_plan.hpp): lane t of step j
-// computes w * y[src], the lanes of an aligned power-of-two segment are
-// summed with xor-shuffles, and the segment's lead lane adds the sum to
-// y[row].  Slots are loaded kTailDepth steps ahead of their use.
-
-constexpr int kTailCapV = 18432;
-constexpr int kTailDepth = 8;
-
-struct alignas(16) TailSlotD {
-  double w;
-  unsigned short src, row;
-  unsigned char seglog, lead;
-  unsigned short pad;
-};
-struct alignas(16) TailStepD {
-  int off, count, end, pad;
-};
-
-__device__ __forceinline__ TailSlotD load_slot(const TailSlotD* __restrict__ slots,
-                                               const TailStepD& st, int t) {
-  TailSlotD z;
-  if (t < st.count) {
-    z = slots[st.off + t];
-  } else {
-    z.w = 0.0;
-    z.src = 0;
-    z.row = 0;
-    z.seglog = 0;
-    z.lead = 0;
-    z.pad = 0;
-  }
-  return z;
-}
-
-__device__ __forceinline__ void run_slot(const TailSlotD& sl, const TailStepD& st,
-                                         double* ly) {
-  if ((int)threadIdx.x < st.count) {
-    double v = sl.w * ly[sl.src];
-    const int seg = 1 << sl.seglog;
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-      const double t = __shfl_xor(v, o, kWave);
-      if (o < seg) v += t;
-    }
-    if (sl.lead == 1)
-      ly[sl.row] += v;
-    else if (sl.lead == 2)
-      atomicAdd(&ly[sl.row], v);
-  }
-}
-
-// Runs nsteps steps of a slot program on ly.
-__device__ __forceinline__ void run_program(const TailStepD* __restrict__ steps,
-                                            int nsteps,
-                                            const TailSlotD* __restrict__ slots,
-                                            double* ly) {
-  const int t = threadIdx.x;
-  TailStepD ia[kTailDepth], ib[kTailDepth];
-  TailSlotD a[kTailDepth], b[kTailDepth];
-  const TailStepD none = {0, 0, 0, 0};
-#pragma unroll
-  for (int d = 0; d < kTailDepth; ++d) {
-    ia[d] = d < nsteps ? steps[d] : none;
-    a[d] = load_slot(slots, ia[d], t);
-  }
-  for (int c = 0; c < nsteps; c += kTailDepth) {
-#pragma unroll
-    for (int d = 0; d < kTailDepth; ++d) {
-      const int j = c + kTailDepth + d;
-      ib[d] = j < nsteps ? steps[j] : none;
-      b[d] = load_slot(slots, ib[d], t);
-    }
-#pragma unroll
-    for (int d = 0; d < kTailDepth; ++d) {
-      if (c + d < nsteps) {
-        run_slot(a[d], ia[d], ly);
-        if (ia[d].end) __syncthreads();
-      }
-    }
-#pragma unroll
-    for (int d = 0; d < kTailDepth; ++d) {
-      a[d] = b[d];
-      ia[d] = ib[d];
-    }
-  }
-}
-
 // ext[i] = r[T0+i] + sum of the forward in-edges of tail row i that come
 // from outside the tail (G lanes per row)
-template <int G>
+template <int G, typename T>
 __global__ void __launch_bounds__(kBlock)
 k_tail_ext(int T0, int NT, const int* __restrict__ Eptr,
-           const int* __restrict__ Eidx, const double* __restrict__ Ew,
-           const double* __restrict__ r, const double* __restrict__ y,
-           double* __restrict__ ext, const DevState* __restrict__ st) {
+           const int* __restrict__ Eidx, const T* __restrict__ Ew,
+           const double* __restrict__ r, const T* __restrict__ y,
+           T* __restrict__ ext, const DevState* __restrict__ st) {
   const int gid = blockIdx.x * kBlock + threadIdx.x;
   const int row = gid / G;
   const int lane = gid % G;
   if (row >= NT) return;
   const int k0 = Eptr[row], k1 = Eptr[row + 1];
-  double acc = 0.0;
+  T acc = 0;
   for (int k = k0 + lane; k < k1; k += G) acc += Ew[k] * y[Eidx[k]];
   acc = group_sum<G>(acc);
-  if (lane == 0 && !st->stopped) ext[row] = r[T0 + row] + acc;
+  if (lane == 0 && !st->stopped) ext[row] = (T)r[T0 + row] + acc;
 }
 
 // forward + diagonal + backward on the tail; writes yF of the tail to y
+template <typename T, typename S>
 __global__ void __launch_bounds__(kTailThreads)
-k_tail(int T0, int NT, const double* __restrict__ ext,
-       const double* __restrict__ s, double* __restrict__ y,
-       const TailStepD* __restrict__ fsteps, int nfsteps,
-       const TailSlotD* __restrict__ fslots,
-       const TailStepD* __restrict__ bsteps, int nbsteps,
-       const TailSlotD* __restrict__ bslots,
+k_tail(int T0, int NT, const T* __restrict__ ext, const T* __restrict__ s,
+       T* __restrict__ y, const TailStepD* __restrict__ fsteps, int nfsteps,
+       const S* __restrict__ fslots, const TailStepD* __restrict__ bsteps,
+       int nbsteps, const S* __restrict__ bslots,
        const DevState* __restrict__ st) {
-  __shared__ double ly[kTailCapV];
+  __shared__ T ly[kTailCapV];
   if (st->stopped) return;
   for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] = ext[i];
   __syncthreads();
@@ -325,7 +151,8 @@ k_tail(int T0, int NT, const double* __restrict__ ext,
 }
 
 // partial sum of y (for the mean removal of LDLsolver)
-__global__ void k_vsum(int n, const double* __restrict__ y,
+template <typename T>
+__global__ void k_vsum(int n, const T* __restrict__ y,
                        double* __restrict__ part,
                        const DevState* __restrict__ st) {
   if (st->stopped) return;
@@ -344,7 +171,8 @@ __global__ void k_scalar_mu(int n, const double* __restrict__ part,
 }
 
 // z = y - mu; partials z.r, and (every 10th iteration) sum|z|, sum|p|
-__global__ void k_zdot(int n, const double* __restrict__ y,
+template <typename T>
+__global__ void k_zdot(int n, const T* __restrict__ y,
                        const double* __restrict__ r,
                        const double* __restrict__ p, double* __restrict__ z,
                        double* __restrict__ part,
@@ -354,7 +182,7 @@ __global__ void k_zdot(int n, const double* __restrict__ y,
   const bool chk = st->check10 != 0;
   double v[3] = {0.0, 0.0, 0.0};
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    const double t = y[i] - mu;
+    const double t = (double)y[i] - mu;
     z[i] = t;
     v[0] += t * r[i];
     if (chk) {
@@ -368,6 +196,41 @@ __global__ void k_zdot(int n, const double* __restrict__ y,
     part[S_AZ * kRedBlocks + blockIdx.x] = v[1];
     part[S_AP2 * kRedBlocks + blockIdx.x] = v[2];
   }
+}
+
+// FP32-inner mode SpMV: FP32 la and p until st->spmv64, FP64 afterwards
+template <int G>
+__global__ void __launch_bounds__(kBlock)
+k_spmv_dot_mixed(int n, const int* __restrict__ Lptr, const int* __restrict__ Lidx,
+                 const float* __restrict__ Lval32, const float* __restrict__ p32,
+                 const double* __restrict__ Lval, const double* __restrict__ p,
+                 double* __restrict__ q, double* __restrict__ part,
+                 const DevState* __restrict__ st) {
+  if (st->stopped) return;
+  const bool f64 = st->spmv64 != 0;
+  const int lane = threadIdx.x % G;
+  const int rows_per_grid = gridDim.x * (kBlock / G);
+  double v[1] = {0.0};
+  for (int row = blockIdx.x * (kBlock / G) + threadIdx.x / G; row < n;
+       row += rows_per_grid) {
+    const int k0 = Lptr[row], k1 = Lptr[row + 1];
+    double acc;
+    if (f64) {
+      double a = 0.0;
+      for (int k = k0 + lane; k < k1; k += G) a += Lval[k] * p[Lidx[k]];
+      acc = group_sum<G>(a);
+    } else {
+      float a = 0.0f;
+      for (int k = k0 + lane; k < k1; k += G) a += Lval32[k] * p32[Lidx[k]];
+      acc = (double)group_sum<G>(a);
+    }
+    if (lane == 0) {
+      q[row] = acc;
+      v[0] += p[row] * acc;
+    }
+  }
+  block_sum<1>(v);
+  if (threadIdx.x == 0) part[S_PQ * kRedBlocks + blockIdx.x] = v[0];
 }
 
 // rho of the initial residual; p = z follows as a copy
@@ -403,12 +266,14 @@ __global__ void k_copy(int n, const double* __restrict__ src,
 // PCG iteration
 
 // q = la * p, partial p.q (G lanes per row, grid-stride over rows)
-template <int G>
+// T: matrix values and the p copy read (float in the FP32-inner mode);
+// products are accumulated in T, q and p.q in double.
+template <int G, typename T>
 __global__ void __launch_bounds__(kBlock)
 k_spmv_dot(int n, const int* __restrict__ Lptr, const int* __restrict__ Lidx,
-           const double* __restrict__ Lval, const double* __restrict__ p,
-           double* __restrict__ q, double* __restrict__ part,
-           const DevState* __restrict__ st) {
+           const T* __restrict__ Lval, const T* __restrict__ p,
+           const double* __restrict__ pd, double* __restrict__ q,
+           double* __restrict__ part, const DevState* __restrict__ st) {
   if (st->stopped) return;
   const int lane = threadIdx.x % G;
   const int rows_per_grid = gridDim.x * (kBlock / G);
@@ -416,12 +281,12 @@ k_spmv_dot(int n, const int* __restrict__ Lptr, const int* __restrict__ Lidx,
   for (int row = blockIdx.x * (kBlock / G) + threadIdx.x / G; row < n;
        row += rows_per_grid) {
     const int k0 = Lptr[row], k1 = Lptr[row + 1];
-    double acc = 0.0;
+    T acc = 0;
     for (int k = k0 + lane; k < k1; k += G) acc += Lval[k] * p[Lidx[k]];
     acc = group_sum<G>(acc);
     if (lane == 0) {
-      q[row] = acc;
-      v[0] += p[row] * acc;
+      q[row] = (double)acc;
+      v[0] += pd[row] * (double)acc;
     }
   }
   block_sum<1>(v);
@@ -502,14 +367,102 @@ __global__ void k_scalar_nr(const double* __restrict__ part, DevState* st) {
     }
     const double nr = sqrt(rr) / st->nb;
     st->nr = nr;
+    st->bestnr_prev = st->bestnr;
     if (nr < st->bestnr) {
       st->bestnr = nr;
       st->pending_copy = 1;
     }
+    if (st->fp32 && !st->spmv64 && nr < st->switch_tol) {
+      // below switch_tol an FP32 SpMV cannot resolve the residual on
+      // badly scaled graphs: continue with the FP64 SpMV, starting from an
+      // FP64 true residual
+      st->spmv64 = 1;
+      st->verify = 1;
+    }
     if (nr < st->tol) {
+      if (st->fp32) {
+        st->verify = 1;  // confirmed by an FP64 true residual below
+      } else {
+        st->stopped = 1;
+        st->stop_reason = 0;
+      }
+    }
+  }
+}
+
+// FP32-inner mode: r = r0 - L x in FP64 (r0 = b .- mean(b)), partial r.r.
+// Runs when `always` (the periodic residual replacement) or when the
+// recursive residual has just passed tol (st->verify).
+template <int G>
+__global__ void __launch_bounds__(kBlock)
+k_resid(int n, int always, const int* __restrict__ Lptr,
+        const int* __restrict__ Lidx, const double* __restrict__ Lval,
+        const double* __restrict__ x, const double* __restrict__ r0,
+        double* __restrict__ r, double* __restrict__ part,
+        const DevState* __restrict__ st) {
+  if (st->stopped || !(always || st->verify)) return;
+  const int lane = threadIdx.x % G;
+  const int rows_per_grid = gridDim.x * (kBlock / G);
+  double v[1] = {0.0};
+  for (int row = blockIdx.x * (kBlock / G) + threadIdx.x / G; row < n;
+       row += rows_per_grid) {
+    double acc = 0.0;
+    for (int k = Lptr[row] + lane; k < Lptr[row + 1]; k += G) acc += Lval[k] * x[Lidx[k]];
+    acc = group_sum<G>(acc);
+    if (lane == 0) {
+      const double t = r0[row] - acc;
+      r[row] = t;
+      v[0] += t * t;
+    }
+  }
+  block_sum<1>(v);
+  if (threadIdx.x == 0) part[S_RR * kRedBlocks + blockIdx.x] = v[0];
+}
+
+// After k_resid: the true relative residual replaces nr; convergence is
+// accepted only on it (FP32-inner mode).
+__global__ void k_scalar_resid(int always, const double* __restrict__ part, DevState* st) {
+  if (st->stopped || !(always || st->verify)) return;
+  const double rr = wave_total(part, S_RR);
+  if (threadIdx.x == 0) {
+    const double nr = sqrt(rr) / st->nb;
+    st->nr = nr;
+    // the true residual decides whether this iterate is the best so far
+    st->bestnr = st->bestnr_prev;
+    st->pending_copy = 0;
+    if (nr < st->bestnr) {
+      st->bestnr = nr;
+      st->pending_copy = 1;
+    }
+    if (st->verify && nr < st->tol) {
       st->stopped = 1;
       st->stop_reason = 0;
     }
+    st->verify = 0;
+  }
+}
+
+// p = z + beta p with an FP32 copy of p for the FP32 SpMV
+__global__ void k_pupdate_f(int n, const double* __restrict__ z,
+                            double* __restrict__ p, float* __restrict__ pf,
+                            const DevState* __restrict__ st) {
+  if (st->stopped) return;
+  const double beta = st->beta;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double t = z[i] + beta * p[i];
+    p[i] = t;
+    pf[i] = (float)t;
+  }
+}
+
+// p = z (and its FP32 copy) after the setup's preconditioner
+__global__ void k_copy_f(int n, const double* __restrict__ src,
+                         double* __restrict__ dst, float* __restrict__ dstf,
+                         const DevState* __restrict__ st) {
+  if (st->stopped) return;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    dst[i] = src[i];
+    dstf[i] = (float)src[i];
   }
 }
 

@@ -126,18 +126,17 @@ void pack_level(std::vector<RowEntries>& rows, std::vector<TailStep>& steps,
     size_t p = 0;
     while (p < pass.size() && ptr[p] + C.size > kTailLanes) ++p;
     if (p == pass.size()) {
-      pass.emplace_back(kTailLanes, TailSlot{0.0, 0, 0, 0, 0, 0});
+      pass.emplace_back(kTailLanes, TailSlot{0.0, 0, 0});
       ptr.push_back(0);
     }
     const int64_t off = ptr[p];
     ptr[p] += C.size;
     for (int64_t t = 0; t < C.size; ++t) {
       TailSlot& S = pass[p][off + t];
-      S.seglog = (uint8_t)log2_exact(C.size);
-      S.lead = (t == 0) ? (C.atomic ? 2 : 1) : 0;
-      S.row = (uint16_t)C.row;
+      S.meta = slot_meta((uint32_t)C.row, (uint32_t)log2_exact(C.size),
+                         t == 0 ? (C.atomic ? 2u : 1u) : 0u);
       if (C.k0 + t < C.k1) {
-        S.src = (uint16_t)C.R->e[C.k0 + t].first;
+        S.src = (uint32_t)C.R->e[C.k0 + t].first;
         S.w = C.R->e[C.k0 + t].second;
       }
     }
@@ -151,6 +150,69 @@ void pack_level(std::vector<RowEntries>& rows, std::vector<TailStep>& steps,
 }
 
 }  // namespace
+
+bool build_tail(Plan& P, int64_t max_steps, int64_t cap) {
+  const int64_t n = P.n;
+    int64_t t = P.nlev, nt = 0;
+    while (t > 0) {
+      const int64_t l = t - 1;
+      const int64_t w = P.lvl[l + 1] - P.lvl[l];
+      if (nt + w > cap) break;
+      int64_t fl = 0, bl = 0;
+      for (int64_t pos = P.lvl[l]; pos < P.lvl[l + 1]; ++pos) {
+        fl += row_lanes(P.Fptr[pos + 1] - P.Fptr[pos]);
+        bl += row_lanes(P.Bptr[pos + 1] - P.Bptr[pos]);
+      }
+      // bump packing of power-of-two segments in decreasing size fills
+      // whole steps, so lanes / kTailLanes rounded up is the step count
+      const int64_t steps_l = (std::max(fl, bl) + kTailLanes - 1) / kTailLanes;
+      if (steps_l > max_steps) break;
+      nt += w;
+      t = l;
+    }
+    TailProgram& T = P.tail;
+    T = TailProgram();
+    P.tail_level = t;
+    T.T0 = P.lvl[t];
+    T.NT = n - T.T0;
+    const int32_t T0 = (int32_t)T.T0;
+    T.Eptr.assign(T.NT + 1, 0);
+    for (int64_t i = 0; i < T.NT; ++i) {
+      const int64_t pos = T0 + i;
+      for (int32_t k = P.Fptr[pos]; k < P.Fptr[pos + 1]; ++k)
+        if (P.Fidx[k] < T0) {
+          T.Eidx.push_back(P.Fidx[k]);
+          T.Ew.push_back(P.Fw[k]);
+        }
+      T.Eptr[i + 1] = (int32_t)T.Eidx.size();
+    }
+    bool ok = true;
+    for (int64_t l = t; l < P.nlev; ++l) {
+      std::vector<RowEntries> rows;
+      for (int64_t pos = P.lvl[l]; pos < P.lvl[l + 1]; ++pos) {
+        RowEntries R;
+        R.row = (int32_t)(pos - T0);
+        for (int32_t k = P.Fptr[pos]; k < P.Fptr[pos + 1]; ++k)
+          if (P.Fidx[k] >= T0) R.e.emplace_back(P.Fidx[k] - T0, P.Fw[k]);
+        rows.push_back(std::move(R));
+      }
+      pack_level(rows, T.fsteps, T.fslots);
+    }
+    for (int64_t l = P.nlev - 1; l >= t && ok; --l) {
+      std::vector<RowEntries> rows;
+      for (int64_t pos = P.lvl[l]; pos < P.lvl[l + 1]; ++pos) {
+        RowEntries R;
+        R.row = (int32_t)(pos - T0);
+        for (int32_t k = P.Bptr[pos]; k < P.Bptr[pos + 1]; ++k) {
+          if (P.Bidx[k] < T0) ok = false;
+          R.e.emplace_back(P.Bidx[k] - T0, P.Bw[k]);
+        }
+        rows.push_back(std::move(R));
+      }
+      pack_level(rows, T.bsteps, T.bslots);
+    }
+    return ok;
+  }
 
 bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
                 const int64_t* colptr, const int64_t* rowval,
@@ -317,70 +379,9 @@ bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
       }
     }
   }
-  // ---- the LDS-resident tail -----------------------------------------------
-  {
-    int64_t t = P.nlev, nt = 0;
-    while (t > 0) {
-      const int64_t l = t - 1;
-      const int64_t w = P.lvl[l + 1] - P.lvl[l];
-      if (nt + w > kTailCap) break;
-      int64_t fl = 0, bl = 0;
-      for (int64_t pos = P.lvl[l]; pos < P.lvl[l + 1]; ++pos) {
-        fl += row_lanes(P.Fptr[pos + 1] - P.Fptr[pos]);
-        bl += row_lanes(P.Bptr[pos + 1] - P.Bptr[pos]);
-      }
-      // bump packing of power-of-two segments in decreasing size fills
-      // whole steps, so lanes / kTailLanes rounded up is the step count
-      const int64_t steps_l = (std::max(fl, bl) + kTailLanes - 1) / kTailLanes;
-      if (steps_l > max_tail_steps) break;
-      nt += w;
-      t = l;
-    }
-    TailProgram& T = P.tail;
-    T = TailProgram();
-    P.tail_level = t;
-    T.T0 = P.lvl[t];
-    T.NT = n - T.T0;
-    const int32_t T0 = (int32_t)T.T0;
-    T.Eptr.assign(T.NT + 1, 0);
-    for (int64_t i = 0; i < T.NT; ++i) {
-      const int64_t pos = T0 + i;
-      for (int32_t k = P.Fptr[pos]; k < P.Fptr[pos + 1]; ++k)
-        if (P.Fidx[k] < T0) {
-          T.Eidx.push_back(P.Fidx[k]);
-          T.Ew.push_back(P.Fw[k]);
-        }
-      T.Eptr[i + 1] = (int32_t)T.Eidx.size();
-    }
-    bool ok = true;
-    for (int64_t l = t; l < P.nlev; ++l) {
-      std::vector<RowEntries> rows;
-      for (int64_t pos = P.lvl[l]; pos < P.lvl[l + 1]; ++pos) {
-        RowEntries R;
-        R.row = (int32_t)(pos - T0);
-        for (int32_t k = P.Fptr[pos]; k < P.Fptr[pos + 1]; ++k)
-          if (P.Fidx[k] >= T0) R.e.emplace_back(P.Fidx[k] - T0, P.Fw[k]);
-        rows.push_back(std::move(R));
-      }
-      pack_level(rows, T.fsteps, T.fslots);
-    }
-    for (int64_t l = P.nlev - 1; l >= t && ok; --l) {
-      std::vector<RowEntries> rows;
-      for (int64_t pos = P.lvl[l]; pos < P.lvl[l + 1]; ++pos) {
-        RowEntries R;
-        R.row = (int32_t)(pos - T0);
-        for (int32_t k = P.Bptr[pos]; k < P.Bptr[pos + 1]; ++k) {
-          if (P.Bidx[k] < T0) ok = false;
-          R.e.emplace_back(P.Bidx[k] - T0, P.Bw[k]);
-        }
-        rows.push_back(std::move(R));
-      }
-      pack_level(rows, T.bsteps, T.bslots);
-    }
-    if (!ok) {
-      err = "upload_ldli: internal error packing the tail program";
-      return false;
-    }
+  if (!build_tail(P, max_tail_steps, kTailCap)) {
+    err = "upload_ldli: internal error packing the tail program";
+    return false;
   }
   return true;
 }

@@ -16,18 +16,20 @@ struct HostCsr {
   std::vector<double> val;
 };
 
-// One lane's work in the tail kernel: w * y[src] summed over an aligned
+// One lane's work in a slot program: w * y[src] summed over an aligned
 // segment of 2^seglog lanes; the segment's first lane adds the sum to y[row]
-// (lead = 1: plain add; lead = 2: LDS atomic add, for rows split in chunks).
-// Indices are local to the tail (position - T0).
+// (lead = 1: plain add; lead = 2: atomic add, for rows split in chunks).
+// meta packs row (bits 0-25), seglog (bits 26-28) and lead (bits 29-30).
+// Indices are local to the program (position - T0).
 struct TailSlot {
   double w;
-  uint16_t src;
-  uint16_t row;
-  uint8_t seglog;
-  uint8_t lead;
-  uint16_t pad;
+  uint32_t src;
+  uint32_t meta;
 };
+constexpr uint32_t kRowMask = (1u << 26) - 1;
+inline uint32_t slot_meta(uint32_t row, uint32_t seglog, uint32_t lead) {
+  return row | (seglog << 26) | (lead << 29);
+}
 static_assert(sizeof(TailSlot) == 16, "TailSlot is 16 bytes");
 
 // A step = up to kTailLanes slots run by the tail workgroup at once; `end`
@@ -88,5 +90,10 @@ bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
                 const int64_t* colptr, const int64_t* rowval,
                 const double* fval, const double* d, int base,
                 int64_t max_tail_steps, Plan& P, std::string& err);
+
+// (Re)build P.tail as the longest suffix of levels with at most `cap`
+// vertices whose levels need at most `max_steps` steps each.  With both
+// limits at INT64_MAX the program covers the whole sweep (tail_level 0).
+bool build_tail(Plan& P, int64_t max_steps, int64_t cap);
 
 }  // namespace lapgpu

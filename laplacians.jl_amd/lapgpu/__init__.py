@@ -8,7 +8,7 @@ Public surface (mirrors the reference's names):
 
 from .factor import LDLinv, build_ldli, load_ldli, save_ldli  # noqa: F401
 from .graphs import lap  # noqa: F401
-from ._native import Context, LapGPUError  # noqa: F401
+from ._native import Batch, Context, LapGPUError  # noqa: F401
 from .solver import (  # noqa: F401
     ApproxCholParams,
     approxchol_lap1_gpu,

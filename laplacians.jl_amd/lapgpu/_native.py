@@ -47,6 +47,18 @@ EXPORTS = (
     "lapgpu_last_timings",
     "lapgpu_last_error",
     "lapgpu_destroy",
+    "lapgpu_batch_create",
+    "lapgpu_batch_add_graph",
+    "lapgpu_batch_remove_graph",
+    "lapgpu_batch_graph_info",
+    "lapgpu_batch_solve",
+    "lapgpu_batch_solve_device",
+    "lapgpu_batch_submit",
+    "lapgpu_batch_submit_host",
+    "lapgpu_batch_query",
+    "lapgpu_batch_collect",
+    "lapgpu_batch_last_error",
+    "lapgpu_batch_destroy",
 )
 
 _LIB = None
@@ -81,9 +93,24 @@ def load() -> ctypes.CDLL:
     L.lapgpu_last_error.restype = ctypes.c_char_p
     L.lapgpu_destroy.argtypes = [P]
     L.lapgpu_destroy.restype = None
+    pi32 = ctypes.POINTER(ctypes.c_int32)
+    L.lapgpu_batch_create.argtypes = [ctypes.POINTER(P), ci]
+    L.lapgpu_batch_add_graph.argtypes = [P, i64, P, P, P, i64, P, P, P, P, P, ci, pi32]
+    L.lapgpu_batch_remove_graph.argtypes = [P, ctypes.c_int32]
+    L.lapgpu_batch_graph_info.argtypes = [P, ctypes.c_int32, pi64, pi64]
+    L.lapgpu_batch_solve.argtypes = [P, i64, P, P, P, dbl, i64, dbl, ci, ci, P, P, P]
+    L.lapgpu_batch_solve_device.argtypes = [P, i64, P, P, P, dbl, i64, dbl, ci, ci, P, P, P]
+    L.lapgpu_batch_submit.argtypes = [P, i64, P, P, P, dbl, i64, dbl, ci, ci, pi32]
+    L.lapgpu_batch_query.argtypes = [P, ctypes.c_int32, ctypes.POINTER(ci)]
+    L.lapgpu_batch_submit_host.argtypes = [P, i64, P, P, dbl, i64, dbl, ci, ci, pi32]
+    L.lapgpu_batch_collect.argtypes = [P, ctypes.c_int32, P, P, P, P]
+    L.lapgpu_batch_last_error.argtypes = [P]
+    L.lapgpu_batch_last_error.restype = ctypes.c_char_p
+    L.lapgpu_batch_destroy.argtypes = [P]
+    L.lapgpu_batch_destroy.restype = None
     for name in EXPORTS:
         f = getattr(L, name)
-        if name not in ("lapgpu_last_error", "lapgpu_destroy"):
+        if name not in ("lapgpu_last_error", "lapgpu_destroy", "lapgpu_batch_last_error", "lapgpu_batch_destroy"):
             f.restype = ci
     _LIB = L
     return L
@@ -200,3 +227,151 @@ class Context:
                     "lapgpu_last_timings")
         return {"spmv_ms": v[0].value, "ldl_ms": v[1].value, "vec_ms": v[2].value, "total_ms": v[3].value,
                 "its": its.value}
+
+
+class Batch:
+    """Device-resident graph store; solves many (graph, b) requests in one
+    launch, one workgroup per request (lapgpu_batch_* in include/lapgpu.h)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = load()
+        h = ctypes.c_void_p()
+        rc = self._lib.lapgpu_batch_create(ctypes.byref(h), device)
+        if rc != 0:
+            raise LapGPUError(f"lapgpu_batch_create: {self._lib.lapgpu_batch_last_error(None).decode()}")
+        self._h = h
+        self.sizes = {}
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise LapGPUError(f"{what} failed ({rc}): {self._lib.lapgpu_batch_last_error(self._h).decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.lapgpu_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_graph(self, la, f) -> int:
+        n = la.shape[0]
+        cp = np.ascontiguousarray(la.indptr, dtype=np.int64)
+        rv = np.ascontiguousarray(la.indices, dtype=np.int64)
+        nz = np.ascontiguousarray(la.data, dtype=np.float64)
+        col = np.ascontiguousarray(f.col, dtype=np.int64)
+        fcp = np.ascontiguousarray(f.colptr, dtype=np.int64)
+        frv = np.ascontiguousarray(f.rowval, dtype=np.int64)
+        fv = np.ascontiguousarray(f.fval, dtype=np.float64)
+        d = np.ascontiguousarray(f.d, dtype=np.float64)
+        gid = ctypes.c_int32(-1)
+        self._check(self._lib.lapgpu_batch_add_graph(self._h, n, _p(cp), _p(rv), _p(nz), len(fv), _p(col), _p(fcp),
+                                                     _p(frv), _p(fv), _p(d), 0, ctypes.byref(gid)),
+                    "lapgpu_batch_add_graph")
+        self.sizes[gid.value] = n
+        return gid.value
+
+    def remove_graph(self, gid: int) -> None:
+        self._check(self._lib.lapgpu_batch_remove_graph(self._h, int(gid)), "lapgpu_batch_remove_graph")
+        self.sizes.pop(gid, None)
+
+    def graph_info(self, gid: int) -> dict:
+        n = ctypes.c_int64()
+        by = ctypes.c_int64()
+        self._check(self._lib.lapgpu_batch_graph_info(self._h, int(gid), ctypes.byref(n), ctypes.byref(by)),
+                    "lapgpu_batch_graph_info")
+        return {"n": n.value, "device_bytes": by.value}
+
+    def solve(self, graph_ids, bs, tol=1e-6, maxits=1000, maxtime=float("inf"), stag_test=5, flags=0):
+        """Solve request i = (graph_ids[i], bs[i]); returns (xs, infos)."""
+        ids = np.ascontiguousarray(graph_ids, dtype=np.int32)
+        bcat = np.ascontiguousarray(np.concatenate([np.asarray(b, dtype=np.float64) for b in bs]) if len(bs) else
+                                    np.zeros(0))
+        xcat = np.empty_like(bcat)
+        k = len(ids)
+        its = np.zeros(k, dtype=np.int64)
+        nr = np.zeros(k)
+        stop = np.zeros(k, dtype=np.int32)
+        self._check(self._lib.lapgpu_batch_solve(self._h, k, _p(ids), _p(bcat), _p(xcat), float(tol), int(maxits),
+                                                 float(maxtime), int(stag_test), int(flags), _p(its), _p(nr), _p(stop)),
+                    "lapgpu_batch_solve")
+        xs, off = [], 0
+        for b in bs:
+            xs.append(xcat[off:off + len(b)].copy())
+            off += len(b)
+        infos = [{"its": int(its[i]), "nr": float(nr[i]), "stop": STOP_NAMES.get(int(stop[i]), str(stop[i]))}
+                 for i in range(k)]
+        return xs, infos
+
+    def solve_device(self, graph_ids, b_ptrs, x_ptrs, tol=1e-6, maxits=1000, maxtime=float("inf"), stag_test=5,
+                     flags=0):
+        ids = np.ascontiguousarray(graph_ids, dtype=np.int32)
+        k = len(ids)
+        pb = np.ascontiguousarray(b_ptrs, dtype=np.uint64)
+        px = np.ascontiguousarray(x_ptrs, dtype=np.uint64)
+        its = np.zeros(k, dtype=np.int64)
+        nr = np.zeros(k)
+        stop = np.zeros(k, dtype=np.int32)
+        self._check(self._lib.lapgpu_batch_solve_device(self._h, k, _p(ids), _p(pb), _p(px), float(tol), int(maxits),
+                                                        float(maxtime), int(stag_test), int(flags), _p(its), _p(nr),
+                                                        _p(stop)),
+                    "lapgpu_batch_solve_device")
+        return [{"its": int(its[i]), "nr": float(nr[i]), "stop": STOP_NAMES.get(int(stop[i]), str(stop[i]))}
+                for i in range(k)]
+
+    def submit(self, graph_ids, b_ptrs, x_ptrs, tol=1e-6, maxits=1000, maxtime=float("inf"), stag_test=5,
+               flags=0):
+        """Enqueue a launch (device pointers); returns its slot, or None when
+        every slot holds an uncollected launch."""
+        ids = np.ascontiguousarray(graph_ids, dtype=np.int32)
+        pb = np.ascontiguousarray(b_ptrs, dtype=np.uint64)
+        px = np.ascontiguousarray(x_ptrs, dtype=np.uint64)
+        slot = ctypes.c_int32(-1)
+        rc = self._lib.lapgpu_batch_submit(self._h, len(ids), _p(ids), _p(pb), _p(px), float(tol), int(maxits),
+                                           float(maxtime), int(stag_test), int(flags), ctypes.byref(slot))
+        if rc == 1:
+            return None
+        self._check(rc, "lapgpu_batch_submit")
+        return (slot.value, len(ids), None)
+
+    def submit_host(self, graph_ids, bs, tol=1e-6, maxits=1000, maxtime=float("inf"), stag_test=5, flags=0):
+        """Enqueue a launch with host right-hand sides; returns a handle for
+        query/collect, or None when every launch slot is occupied."""
+        ids = np.ascontiguousarray(graph_ids, dtype=np.int32)
+        bcat = np.ascontiguousarray(np.concatenate([np.asarray(b, dtype=np.float64) for b in bs]))
+        slot = ctypes.c_int32(-1)
+        rc = self._lib.lapgpu_batch_submit_host(self._h, len(ids), _p(ids), _p(bcat), float(tol), int(maxits),
+                                                float(maxtime), int(stag_test), int(flags), ctypes.byref(slot))
+        if rc == 1:
+            return None
+        self._check(rc, "lapgpu_batch_submit_host")
+        return (slot.value, len(ids), [len(b) for b in bs])
+
+    def query(self, handle) -> bool:
+        done = ctypes.c_int(0)
+        self._check(self._lib.lapgpu_batch_query(self._h, handle[0], ctypes.byref(done)), "lapgpu_batch_query")
+        return bool(done.value)
+
+    def collect(self, handle):
+        """Wait for a launch; returns (xs or None, infos)."""
+        k = handle[1]
+        lens = handle[2]
+        its = np.zeros(k, dtype=np.int64)
+        nr = np.zeros(k)
+        stop = np.zeros(k, dtype=np.int32)
+        xcat = np.empty(sum(lens)) if lens is not None else None
+        xp = _p(xcat) if xcat is not None else None
+        self._check(self._lib.lapgpu_batch_collect(self._h, handle[0], xp, _p(its), _p(nr), _p(stop)),
+                    "lapgpu_batch_collect")
+        infos = [{"its": int(its[i]), "nr": float(nr[i]), "stop": STOP_NAMES.get(int(stop[i]), str(stop[i]))}
+                 for i in range(k)]
+        if lens is None:
+            return None, infos
+        xs, off = [], 0
+        for ln in lens:
+            xs.append(xcat[off:off + ln])
+            off += ln
+        return xs, infos

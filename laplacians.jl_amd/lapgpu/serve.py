@@ -1,0 +1,269 @@
+"""Serving loop: continuous batching of Laplacian solves on one GPU.
+
+Graphs (la + LDLinv) stay resident in HBM.  Requests (graph, b, tol) arrive
+over time; each is answered by the same solve closure as approxchol_lap's
+(src/approxChol.jl:1624) on the device.  The loop
+
+* admits a request as soon as it arrives: graphs of at most `wg_max_n`
+  vertices are queued for the next batched launch (one workgroup per
+  request, lapgpu_batch_submit_host), larger graphs go to a worker that
+  runs the plain multi-kernel solve (lapgpu_solve) on that graph's own
+  stream; both kinds overlap on the device;
+* keeps up to four batched launches in flight (the batch's launch slots):
+  requests that arrive while others run start in the next free slot instead
+  of waiting for the running launch to finish;
+* admits a new graph by its HBM footprint: when the budget is exceeded,
+  idle non-resident graphs are evicted (least recently used first), and a
+  request waits while its graph cannot be admitted;
+* factors new graphs on host threads (liblapfactor, re-entrant) while the
+  device keeps solving.
+
+Every result carries its latency (arrival to completion); `summary` gives
+solves/s and p50/p99 latency.
+"""
+
+from __future__ import annotations
+
+import time
+from collections import OrderedDict, deque
+from concurrent.futures import ThreadPoolExecutor
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _native
+from .factor import LDLinv, build_ldli
+from .graphs import lap
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float  # seconds after the start of the run
+    b: np.ndarray
+    graph: object = None  # key of a registered graph
+    new_graph: sp.spmatrix | None = None  # adjacency that needs factoring first
+    tol: float = 1e-8
+    maxits: int = 1000
+
+
+@dataclass
+class Result:
+    rid: int
+    latency: float
+    its: int
+    nr: float
+    stop: str
+    path: str
+    x: np.ndarray | None = None
+
+
+@dataclass
+class _Graph:
+    key: object
+    n: int
+    kind: str  # "wg" or "plain"
+    bytes: int
+    resident: bool
+    handle: object = None  # batch graph id or Context
+    busy: int = 0
+    pending: deque = field(default_factory=deque)
+
+
+def graph_bytes_estimate(n: int, nnz_l: int, nnzf: int) -> int:
+    """Device bytes of a graph for admission: la (12 B/entry), the level-
+    ordered factor in both sweep directions (about 16 B/entry each), and
+    8 workspace vectors."""
+    return 12 * nnz_l + 32 * nnzf + 8 * 8 * n + 16 * n
+
+
+class Server:
+    def __init__(self, device: int = 0, hbm_budget: int | None = None, wg_max_n: int = 200_000,
+                 max_launch: int = 512, plain_workers: int = 3, factor_workers: int = 4,
+                 keep_x: bool = False):
+        self.device = device
+        self.batch = _native.Batch(device)
+        self.hbm_budget = hbm_budget if hbm_budget is not None else 200 * 2**30
+        self.wg_max_n = wg_max_n
+        self.max_launch = max_launch
+        self.keep_x = keep_x
+        self.graphs: "OrderedDict[object, _Graph]" = OrderedDict()
+        self.hbm_used = 0
+        self._plain_pool = ThreadPoolExecutor(max_workers=plain_workers)
+        self._factor_pool = ThreadPoolExecutor(max_workers=factor_workers)
+        self._anon = 0
+
+    # ------------------------------------------------------------ graphs ---
+    def register(self, key, a: sp.spmatrix, ldli: LDLinv | None = None, seed: int = 1,
+                 resident: bool = True) -> bool:
+        """Upload a graph (factoring it on the host if no LDLinv is given).
+        Returns False when it does not fit in the HBM budget."""
+        a = sp.csc_matrix(a)
+        la = lap(a)
+        if ldli is None:
+            ldli = build_ldli(a, seed=seed)
+        return self._admit(key, la, ldli, resident)
+
+    def _admit(self, key, la, ldli, resident) -> bool:
+        n = la.shape[0]
+        need = graph_bytes_estimate(n, la.nnz, ldli.nnz)
+        if self.hbm_used + need > self.hbm_budget:
+            self._evict(need)
+            if self.hbm_used + need > self.hbm_budget:
+                return False
+        if n <= self.wg_max_n:
+            gid = self.batch.add_graph(la, ldli)
+            g = _Graph(key, n, "wg", self.batch.graph_info(gid)["device_bytes"] + 64 * n, resident, gid)
+        else:
+            ctx = _native.Context(self.device)
+            ctx.upload_lap(la)
+            ctx.upload_ldli(ldli)
+            g = _Graph(key, n, "plain", need, resident, ctx)
+        self.graphs[key] = g
+        self.hbm_used += g.bytes
+        return True
+
+    def _evict(self, need: int) -> None:
+        for key in list(self.graphs):
+            if self.hbm_used + need <= self.hbm_budget:
+                return
+            g = self.graphs[key]
+            if g.resident or g.busy or g.pending:
+                continue
+            self._drop(g)
+
+    def _drop(self, g: _Graph) -> None:
+        if g.kind == "wg":
+            self.batch.remove_graph(g.handle)
+        else:
+            g.handle.close()
+        self.hbm_used -= g.bytes
+        del self.graphs[g.key]
+
+    # --------------------------------------------------------------- run ---
+    def run(self, requests: list[Request], realtime: bool = True) -> list[Result]:
+        """Serve `requests` (sorted by arrival).  With realtime=False every
+        request is taken to have arrived at time 0 (throughput mode)."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        results: dict[int, Result] = {}
+        arrival_abs = {}
+        small: deque = deque()  # (graph, request) ready for a batched launch
+        launches = []  # (handle, [(graph, request)])
+        plain_futs = {}
+        factor_futs = {}
+        waiting_admission: deque = deque()
+        t0 = time.perf_counter()
+        nxt = 0
+        total = len(reqs)
+
+        def finish(r: Request, info: dict, x, path: str):
+            lat = time.perf_counter() - arrival_abs[r.rid]
+            results[r.rid] = Result(r.rid, lat, info["its"], info["nr"], info["stop"], path,
+                                    x if self.keep_x else None)
+
+        def route(g: _Graph, r: Request):
+            if g.kind == "wg":
+                g.busy += 1
+                small.append((g, r))
+            else:
+                g.pending.append(r)
+                pump_plain(g)
+
+        def pump_plain(g: _Graph):
+            # one solve at a time per context (a context owns one stream)
+            if g.busy or not g.pending:
+                return
+            r = g.pending.popleft()
+            g.busy = 1
+            fut = self._plain_pool.submit(g.handle.solve, r.b, r.tol, r.maxits, float("inf"), 5, 0)
+            plain_futs[fut] = (g, r)
+
+        while len(results) < total:
+            now = time.perf_counter() - t0
+            progressed = False
+            # arrivals
+            while nxt < total and (not realtime or reqs[nxt].arrival <= now):
+                r = reqs[nxt]
+                nxt += 1
+                arrival_abs[r.rid] = t0 + (r.arrival if realtime else 0.0)
+                if r.new_graph is not None:
+                    fut = self._factor_pool.submit(self._factor, r.new_graph)
+                    factor_futs[fut] = r
+                else:
+                    route(self.graphs[r.graph], r)
+                progressed = True
+            # factored graphs: admit by HBM footprint, then route
+            for fut in [f for f in factor_futs if f.done()]:
+                r = factor_futs.pop(fut)
+                waiting_admission.append((r, fut.result()))
+                progressed = True
+            for _ in range(len(waiting_admission)):
+                r, (la, ldli) = waiting_admission.popleft()
+                self._anon += 1
+                key = ("new", self._anon)
+                if self._admit(key, la, ldli, resident=False):
+                    route(self.graphs[key], r)
+                    progressed = True
+                else:
+                    waiting_admission.append((r, (la, ldli)))
+            # finished batched launches
+            for item in [it for it in launches if self.batch.query(it[0])]:
+                launches.remove(item)
+                xs, infos = self.batch.collect(item[0])
+                for (g, r), x, info in zip(item[1], xs, infos):
+                    g.busy -= 1
+                    finish(r, info, x, "batched")
+                progressed = True
+            # finished plain solves
+            for fut in [f for f in plain_futs if f.done()]:
+                g, r = plain_futs.pop(fut)
+                x, info = fut.result()
+                g.busy = 0
+                finish(r, info, x, "plain")
+                pump_plain(g)
+                progressed = True
+            # start a batched launch with everything queued, if a slot is free
+            while small:
+                take = [small.popleft() for _ in range(min(len(small), self.max_launch))]
+                h = self.batch.submit_host([g.handle for g, _ in take], [r.b for _, r in take],
+                                           tol=take[0][1].tol, maxits=take[0][1].maxits)
+                if h is None:
+                    for item in reversed(take):
+                        small.appendleft(item)
+                    break
+                launches.append((h, take))
+                progressed = True
+            # drop finished one-off graphs
+            for key in [k for k, g in self.graphs.items() if not g.resident and not g.busy and not g.pending]:
+                self._drop(self.graphs[key])
+            if not progressed:
+                time.sleep(50e-6)
+        return [results[r.rid] for r in requests]
+
+    @staticmethod
+    def _factor(a):
+        a = sp.csc_matrix(a)
+        return lap(a), build_ldli(a, seed=1)
+
+    def close(self):
+        self._plain_pool.shutdown(wait=True)
+        self._factor_pool.shutdown(wait=True)
+        for g in list(self.graphs.values()):
+            if g.kind == "plain":
+                g.handle.close()
+        self.graphs.clear()
+        self.batch.close()
+
+
+def summary(results: list[Result], wall: float) -> dict:
+    lat = np.array([r.latency for r in results])
+    return {
+        "solves": len(results),
+        "wall_s": wall,
+        "solves_per_s": len(results) / wall if wall > 0 else None,
+        "p50_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "max_nr": float(max(r.nr for r in results)),
+        "stops": sorted({r.stop for r in results}),
+    }

@@ -161,3 +161,29 @@ def test_large_grid3_true_residual(gpu_ctx_factory):
     assert info["stop"] == "converged"
     assert np.linalg.norm(la @ x - b) / np.linalg.norm(b) <= 1e-7
     assert math.isfinite(info["nr"]) and info["nr"] < 1e-8
+
+
+def test_stag_test_one_stops_at_two(gpu_ctx_factory, oracle_mod):
+    a = graphs.grid2(12)
+    la = lapgpu.lap(a)
+    f = lapgpu.build_ldli(a)
+    ctx = gpu_ctx_factory(la, f)
+    b = _rhs(144, 0)
+    _, info = ctx.solve(b, tol=1e-12, maxits=100, stag_test=1)
+    _, io = oracle_mod.approxchol_solve(la, f, b, tol=1e-12, maxits=100, stag_test=1)
+    assert info["stop"] == io["stop"] == "stag_test" and info["its"] == io["its"] == 2
+
+
+def test_tol_zero_ends_by_stagnation(gpu_ctx_factory, oracle_mod):
+    a = graphs.random_tree(200, np.random.default_rng(8))
+    la = lapgpu.lap(a)
+    f = lapgpu.build_ldli(a)
+    ctx = gpu_ctx_factory(la, f)
+    b = _rhs(200, 0)
+    x, info = ctx.solve(b, tol=0.0, maxits=3000, stag_test=5)
+    _, io = oracle_mod.approxchol_solve(la, f, b, tol=0.0, maxits=3000, stag_test=5)
+    assert info["stop"].startswith("stag")
+    if info["stop"] != "stag_test":  # the |p|, |q| and beta checks run every 10th iteration
+        assert info["its"] % 10 == 0
+    assert abs(info["its"] - io["its"]) <= 10
+    assert np.linalg.norm(la @ x - b) <= 1e-12 * np.linalg.norm(b)

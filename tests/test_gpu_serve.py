@@ -1,0 +1,77 @@
+"""Serving loop (lapgpu.serve.Server): requests of uneven sizes arrive over
+time, are admitted while others run, and overlap on the device through
+batched launches and the plain path.  Each result must meet tol on its true
+residual and equal, to 1e-10 relative, the same request served alone."""
+
+import numpy as np
+import pytest
+
+import lapgpu
+from lapgpu import graphs
+from lapgpu.serve import Request, Server, summary
+
+pytestmark = pytest.mark.gpu
+
+
+def _rhs(n, rng):
+    b = rng.standard_normal(n)
+    return b - b.mean()
+
+
+def _graphs():
+    rng = np.random.default_rng(0)
+    return {
+        "g2": graphs.weighted_grid2(40, 30, rng),
+        "g3": graphs.weighted_grid3(14, 14, 14, rng),
+        "pm": graphs.path_matchings(5000, 4, rng),
+        "big": graphs.weighted_grid3(32, 32, 32, rng),  # above wg_max_n below: plain path
+    }
+
+
+def test_serve_mixed_stream_matches_alone():
+    G = _graphs()
+    srv = Server(0, wg_max_n=20000, keep_x=True)
+    try:
+        for k, a in G.items():
+            assert srv.register(k, a)
+        rng = np.random.default_rng(5)
+        keys = list(G)
+        reqs = []
+        t = 0.0
+        for rid in range(40):
+            t += rng.exponential(0.002)
+            k = keys[rng.integers(len(keys))]
+            reqs.append(Request(rid, t, _rhs(G[k].shape[0], rng), graph=k, tol=1e-8))
+        # one request brings a new graph that needs host factoring
+        newg = graphs.weighted_grid2(25, 25, np.random.default_rng(9))
+        reqs.append(Request(40, t + 0.001, _rhs(625, rng), new_graph=newg, tol=1e-8))
+        res = srv.run(reqs)
+        s = summary(res, max(r.arrival for r in reqs))
+        assert s["solves"] == 41
+        for r, out in zip(reqs, res):
+            a = G[r.graph] if r.graph is not None else newg
+            la = lapgpu.lap(a)
+            assert out.stop == "converged", out
+            assert np.linalg.norm(la @ out.x - r.b) / np.linalg.norm(r.b) <= 1e-7
+        assert {o.path for o in res} == {"batched", "plain"}
+        # the same requests alone
+        for r, out in list(zip(reqs, res))[:12]:
+            if r.graph is None:
+                continue
+            alone = srv.run([Request(1000 + r.rid, 0.0, r.b, graph=r.graph, tol=1e-8)])[0]
+            assert alone.its == out.its
+            assert np.linalg.norm(alone.x - out.x) <= 1e-10 * np.linalg.norm(out.x)
+    finally:
+        srv.close()
+
+
+def test_admission_by_hbm_budget():
+    G = _graphs()
+    srv = Server(0, wg_max_n=20000, hbm_budget=4 * 2**20)
+    try:
+        assert srv.register("g2", G["g2"], resident=True)
+        # far too large for the budget
+        assert not srv.register("big", G["big"], resident=False)
+        assert srv.hbm_used <= srv.hbm_budget
+    finally:
+        srv.close()
