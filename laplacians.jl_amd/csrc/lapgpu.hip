@@ -52,9 +52,23 @@ struct lapgpu_ctx {
   // device graph
   int *perm = nullptr, *Lptr = nullptr, *Lidx = nullptr;
   double* Lval = nullptr;
-  int *Fptr = nullptr, *Fidx = nullptr, *Bptr = nullptr, *Bidx = nullptr,
-      *lvl = nullptr;
-  double *Fw = nullptr, *Bw = nullptr, *s = nullptr;
+  int* lvl = nullptr;
+  double* s = nullptr;
+  // level groups before the tail (composite rows; see LevelBlocks)
+  struct GroupLaunch {
+    int lo, hi;
+    int Gf, nhf, hoff_f;
+    int Gb, nhb, hoff_b;
+  };
+  std::vector<GroupLaunch> groups;
+  int *gFptr = nullptr, *gFidx = nullptr, *gBptr = nullptr, *gBidx = nullptr,
+      *heavy = nullptr;
+  double *gFw = nullptr, *gBw = nullptr;
+  float *gFw32 = nullptr, *gBw32 = nullptr, *z32 = nullptr;
+  double* zF = nullptr;  // backward output yF (double path)
+  int block_depth = 8;
+  double block_growth = 3.0;
+  int64_t block_extra = 1 << 22;
   // tail program
   int *Eptr = nullptr, *Eidx = nullptr;
   double *Ew = nullptr, *ext = nullptr;
@@ -62,8 +76,8 @@ struct lapgpu_ctx {
   TailSlotD *fslots = nullptr, *bslots = nullptr;
   // FP32-inner mode copies (built by lapgpu_set_precision)
   bool f32_ready = false;
-  float *Lval32 = nullptr, *Fw32 = nullptr, *Bw32 = nullptr, *s32 = nullptr,
-        *Ew32 = nullptr, *y32 = nullptr, *ext32 = nullptr, *pf = nullptr;
+  float *Lval32 = nullptr, *s32 = nullptr, *Ew32 = nullptr, *y32 = nullptr, *ext32 = nullptr,
+        *pf = nullptr;
   TailSlotF *fslots32 = nullptr, *bslots32 = nullptr;
   double* r0 = nullptr;
   // one PCG iteration captured as a HIP graph (rebuilt after each upload);
@@ -144,13 +158,14 @@ int grid_for(int64_t items, int per_block) {
 
 // ---- templated launch helpers ----------------------------------------------
 
-#define LAPGPU_DISPATCH_G(g, F, ...)            \
-  switch (g) {                                  \
-    case 4: F<4>(__VA_ARGS__); break;           \
-    case 8: F<8>(__VA_ARGS__); break;           \
-    case 16: F<16>(__VA_ARGS__); break;         \
-    case 32: F<32>(__VA_ARGS__); break;         \
-    default: F<64>(__VA_ARGS__); break;         \
+// Calls F with the compile-time lane count G_ = g (F may name G_).
+#define LAPGPU_DISPATCH_G(g, F, ...)                   \
+  switch (g) {                                         \
+    case 4: { constexpr int G_ = 4; F(__VA_ARGS__); break; }    \
+    case 8: { constexpr int G_ = 8; F(__VA_ARGS__); break; }    \
+    case 16: { constexpr int G_ = 16; F(__VA_ARGS__); break; }  \
+    case 32: { constexpr int G_ = 32; F(__VA_ARGS__); break; }  \
+    default: { constexpr int G_ = 64; F(__VA_ARGS__); break; }  \
   }
 
 template <int G>
@@ -167,7 +182,7 @@ void launch_spmv(lapgpu_ctx* c, const double* p, double* q) {
 
 // q = la * p (p.q partials); FP32 mode reads the FP32 copies of la and p
 void spmv(lapgpu_ctx* c, const double* p, double* q) {
-  LAPGPU_DISPATCH_G(c->G_spmv, launch_spmv, c, p, q);
+  LAPGPU_DISPATCH_G(c->G_spmv, launch_spmv<G_>, c, p, q);
 }
 
 template <int G>
@@ -176,46 +191,59 @@ void launch_resid(lapgpu_ctx* c, int always) {
                      always, c->Lptr, c->Lidx, c->Lval, c->x, c->r0, c->r, c->part, c->st);
 }
 
-template <int G, typename T, typename S>
-void launch_sweeps(lapgpu_ctx* c, const double* rin, T* y, const T* Fw, const T* Bw,
+template <int G, typename T, typename TB>
+void launch_group(lapgpu_ctx* c, int lo, int hi, const int* heavy, int nh, const int* ptr,
+                  const int* idx, const T* w, const T* A, const TB* B, T* out) {
+  const int64_t threads = (int64_t)nh * kWave + (int64_t)(hi - lo) * G;
+  hipLaunchKernelGGL((k_group<G, T, TB>), dim3(grid_for(threads, kBlock)), dim3(kBlock), 0,
+                     c->stream, (int)c->n, lo, hi, heavy, nh, ptr, idx, w, A, B, out, c->st);
+}
+
+template <typename T, typename S>
+void launch_sweeps(lapgpu_ctx* c, const double* rin, T* y, T* z, const T* gFw, const T* gBw,
                    const T* sv, const T* Ew, T* ext, const S* fslots, const S* bslots) {
   const Plan& P = c->plan;
-  const int64_t tl = P.tail_level;
-  for (int64_t l = 0; l < tl; ++l) {
-    const int lo = P.lvl[l], hi = P.lvl[l + 1];
-    hipLaunchKernelGGL((k_fwd_level<G, T>), dim3(grid_for((int64_t)(hi - lo) * G, kBlock)),
-                       dim3(kBlock), 0, c->stream, lo, hi, c->Fptr, c->Fidx, Fw, rin, y,
-                       c->st);
+  for (const auto& g : c->groups) {
+    const int* hv = c->heavy + g.hoff_f;
+    LAPGPU_DISPATCH_G(g.Gf, (launch_group<G_, T, double>), c, g.lo, g.hi, hv, g.nhf, c->gFptr,
+                      c->gFidx, gFw, y, rin, y);
   }
   const TailProgram& TP = P.tail;
   if (TP.NT > 0) {
-    hipLaunchKernelGGL((k_tail_ext<G, T>), dim3(grid_for(TP.NT * G, kBlock)), dim3(kBlock), 0,
-                       c->stream, (int)TP.T0, (int)TP.NT, c->Eptr, c->Eidx, Ew, rin, y, ext,
-                       c->st);
+    const int G = c->G_ldl;
+    if (G <= 8)
+      hipLaunchKernelGGL((k_tail_ext<8, T>), dim3(grid_for(TP.NT * 8, kBlock)), dim3(kBlock), 0,
+                         c->stream, (int)TP.T0, (int)TP.NT, c->Eptr, c->Eidx, Ew, rin, y, ext,
+                         c->st);
+    else
+      hipLaunchKernelGGL((k_tail_ext<32, T>), dim3(grid_for(TP.NT * 32, kBlock)), dim3(kBlock),
+                         0, c->stream, (int)TP.T0, (int)TP.NT, c->Eptr, c->Eidx, Ew, rin, y, ext,
+                         c->st);
     hipLaunchKernelGGL((k_tail<T, S>), dim3(1), dim3(kTailThreads), 0, c->stream, (int)TP.T0,
-                       (int)TP.NT, ext, sv, y, c->fsteps, (int)TP.fsteps.size(), fslots,
+                       (int)TP.NT, ext, sv, z, c->fsteps, (int)TP.fsteps.size(), fslots,
                        c->bsteps, (int)TP.bsteps.size(), bslots, c->st);
   }
-  for (int64_t l = tl - 1; l >= 0; --l) {
-    const int lo = P.lvl[l], hi = P.lvl[l + 1];
-    hipLaunchKernelGGL((k_bwd_level<G, T>), dim3(grid_for((int64_t)(hi - lo) * G, kBlock)),
-                       dim3(kBlock), 0, c->stream, lo, hi, c->Bptr, c->Bidx, Bw, sv, y, c->st);
+  for (auto it = c->groups.rbegin(); it != c->groups.rend(); ++it) {
+    const auto& g = *it;
+    const int* hv = c->heavy + g.hoff_b;
+    LAPGPU_DISPATCH_G(g.Gb, (launch_group<G_, T, T>), c, g.lo, g.hi, hv, g.nhb, c->gBptr,
+                      c->gBidx, gBw, z, y, z);
   }
 }
 
 template <int G>
 void launch_sweeps_mode(lapgpu_ctx* c, const double* rin) {
   if (c->precision == LAPGPU_FP32_INNER)
-    launch_sweeps<G, float, TailSlotF>(c, rin, c->y32, c->Fw32, c->Bw32, c->s32, c->Ew32,
-                                       c->ext32, c->fslots32, c->bslots32);
+    launch_sweeps<float, TailSlotF>(c, rin, c->y32, c->z32, c->gFw32, c->gBw32, c->s32,
+                                    c->Ew32, c->ext32, c->fslots32, c->bslots32);
   else
-    launch_sweeps<G, double, TailSlotD>(c, rin, c->y, c->Fw, c->Bw, c->s, c->Ew, c->ext,
-                                        c->fslots, c->bslots);
+    launch_sweeps<double, TailSlotD>(c, rin, c->y, c->zF, c->gFw, c->gBw, c->s, c->Ew, c->ext,
+                                     c->fslots, c->bslots);
 }
 
 // y (or y32) = forward, diagonal and backward sweeps of r (no mean removal)
 void sweeps(lapgpu_ctx* c, const double* rin) {
-  LAPGPU_DISPATCH_G(c->G_ldl, launch_sweeps_mode, c, rin);
+  LAPGPU_DISPATCH_G(c->G_ldl, launch_sweeps_mode<G_>, c, rin);
 }
 
 // mean removal and z.r (sum|z|, sum|p| every 10th iteration) after sweeps
@@ -223,18 +251,18 @@ void finish_precond(lapgpu_ctx* c, const double* rin) {
   const int n = (int)c->n;
   if (c->precision == LAPGPU_FP32_INNER)
     hipLaunchKernelGGL(k_vsum<float>, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
-                       c->y32, c->part, c->st);
+                       c->z32, c->part, c->st);
   else
     hipLaunchKernelGGL(k_vsum<double>, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
-                       c->y, c->part, c->st);
+                       c->zF, c->part, c->st);
   hipLaunchKernelGGL(k_scalar_mu, dim3(1), dim3(kScalarThreads), 0, c->stream, n, c->part,
                      c->st);
   if (c->precision == LAPGPU_FP32_INNER)
     hipLaunchKernelGGL(k_zdot<float>, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
-                       c->y32, rin, c->p, c->z, c->part, c->st);
+                       c->z32, rin, c->p, c->z, c->part, c->st);
   else
     hipLaunchKernelGGL(k_zdot<double>, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
-                       c->y, rin, c->p, c->z, c->part, c->st);
+                       c->zF, rin, c->p, c->z, c->part, c->st);
 }
 
 // z = LDLsolver(r) = sweeps then mean removal; partial z.r etc.
@@ -250,11 +278,11 @@ void drop_graphs(lapgpu_ctx* c) {
 }
 
 void free_f32(lapgpu_ctx* c) {
-  void* fp[] = {c->Lval32, c->Fw32, c->Bw32, c->s32, c->Ew32, c->y32, c->ext32, c->pf,
+  void* fp[] = {c->Lval32, c->gFw32, c->gBw32, c->s32, c->Ew32, c->y32, c->z32, c->ext32, c->pf,
                 c->fslots32, c->bslots32, c->r0};
   for (void* a : fp)
     if (a) (void)hipFree(a);
-  c->Lval32 = c->Fw32 = c->Bw32 = c->s32 = c->Ew32 = c->y32 = c->ext32 = c->pf = nullptr;
+  c->Lval32 = c->gFw32 = c->gBw32 = c->s32 = c->Ew32 = c->y32 = c->z32 = c->ext32 = c->pf = nullptr;
   c->fslots32 = c->bslots32 = nullptr;
   c->r0 = nullptr;
   c->f32_ready = false;
@@ -270,16 +298,19 @@ int free_graph(lapgpu_ctx* c) {
   c->Ew = c->ext = nullptr;
   c->fsteps = c->bsteps = nullptr;
   c->fslots = c->bslots = nullptr;
-  int* ip[] = {c->perm, c->Lptr, c->Lidx, c->Fptr, c->Fidx, c->Bptr, c->Bidx, c->lvl};
+  int* ip[] = {c->perm, c->Lptr, c->Lidx, c->lvl, c->gFptr, c->gFidx, c->gBptr, c->gBidx,
+               c->heavy};
   for (int* a : ip)
     if (a) (void)hipFree(a);
-  double* dp[] = {c->Lval, c->Fw, c->Bw, c->s, c->b, c->r, c->x, c->bestx,
-                  c->p, c->q, c->y, c->z, c->io};
+  double* dp[] = {c->Lval, c->s, c->b, c->r, c->x, c->bestx, c->p, c->q, c->y, c->z, c->io,
+                  c->gFw, c->gBw, c->zF};
   for (double* a : dp)
     if (a) (void)hipFree(a);
-  c->perm = c->Lptr = c->Lidx = c->Fptr = c->Fidx = c->Bptr = c->Bidx = c->lvl = nullptr;
-  c->Lval = c->Fw = c->Bw = c->s = c->b = c->r = c->x = c->bestx = c->p =
-      c->q = c->y = c->z = c->io = nullptr;
+  c->perm = c->Lptr = c->Lidx = c->lvl = nullptr;
+  c->gFptr = c->gFidx = c->gBptr = c->gBidx = c->heavy = nullptr;
+  c->Lval = c->s = c->b = c->r = c->x = c->bestx = c->p = c->q = c->y = c->z = c->io = nullptr;
+  c->gFw = c->gBw = c->zF = nullptr;
+  c->groups.clear();
   return 0;
 }
 
@@ -298,8 +329,9 @@ int prepare_f32(lapgpu_ctx* c) {
   const int64_t n = P.n;
   int rc;
   if ((rc = dev_upload(c, &c->Lval32, narrow<float>(P.Lval)))) return rc;
-  if ((rc = dev_upload(c, &c->Fw32, narrow<float>(P.Fw)))) return rc;
-  if ((rc = dev_upload(c, &c->Bw32, narrow<float>(P.Bw)))) return rc;
+  if ((rc = dev_upload(c, &c->gFw32, narrow<float>(P.blocks.fw)))) return rc;
+  if ((rc = dev_upload(c, &c->gBw32, narrow<float>(P.blocks.bw)))) return rc;
+  if ((rc = dev_alloc(c, &c->z32, (size_t)n))) return rc;
   if ((rc = dev_upload(c, &c->s32, narrow<float>(P.s)))) return rc;
   if ((rc = dev_upload(c, &c->Ew32, narrow<float>(P.tail.Ew)))) return rc;
   auto slots32 = [](const std::vector<TailSlot>& v) {
@@ -317,21 +349,64 @@ int prepare_f32(lapgpu_ctx* c) {
   return 0;
 }
 
+// The level groups before the tail: composite rows, the per-group lane
+// count and the heavy-row lists of both sweeps.
+int upload_groups(lapgpu_ctx* c) {
+  Plan& P = c->plan;
+  build_blocks(P, c->block_depth, c->block_growth, c->block_extra);
+  const LevelBlocks& B = P.blocks;
+  c->groups.clear();
+  std::vector<int32_t> heavy;
+  auto plan_sweep = [&](const std::vector<int32_t>& ptr, int lo, int hi, int& G, int& nh,
+                        int& hoff) {
+    int64_t light_rows = 0, light_ent = 0;
+    int64_t maxdeg = 0;
+    for (int v = lo; v < hi; ++v) maxdeg = std::max<int64_t>(maxdeg, ptr[v + 1] - ptr[v]);
+    // lanes per row from the mean of the rows that are not heavy
+    int64_t tot = ptr[hi] - ptr[lo];
+    G = pick_group(hi > lo ? (double)tot / (double)(hi - lo) : 1.0);
+    if (G > 32) G = 32;
+    const int64_t lim = 8 * (int64_t)G;
+    hoff = (int)heavy.size();
+    for (int v = lo; v < hi; ++v)
+      if (ptr[v + 1] - ptr[v] > lim) heavy.push_back(v);
+      else {
+        ++light_rows;
+        light_ent += ptr[v + 1] - ptr[v];
+      }
+    nh = (int)heavy.size() - hoff;
+    (void)maxdeg;
+  };
+  for (size_t g = 0; g + 1 < B.lev.size(); ++g) {
+    lapgpu_ctx::GroupLaunch gl{};
+    gl.lo = P.lvl[B.lev[g]];
+    gl.hi = P.lvl[B.lev[g + 1]];
+    if (gl.hi <= gl.lo) continue;
+    plan_sweep(B.fptr, gl.lo, gl.hi, gl.Gf, gl.nhf, gl.hoff_f);
+    plan_sweep(B.bptr, gl.lo, gl.hi, gl.Gb, gl.nhb, gl.hoff_b);
+    c->groups.push_back(gl);
+  }
+  int rc;
+  if ((rc = dev_upload(c, &c->gFptr, B.fptr))) return rc;
+  if ((rc = dev_upload(c, &c->gFidx, B.fidx))) return rc;
+  if ((rc = dev_upload(c, &c->gFw, B.fw))) return rc;
+  if ((rc = dev_upload(c, &c->gBptr, B.bptr))) return rc;
+  if ((rc = dev_upload(c, &c->gBidx, B.bidx))) return rc;
+  if ((rc = dev_upload(c, &c->gBw, B.bw))) return rc;
+  if ((rc = dev_upload(c, &c->heavy, heavy))) return rc;
+  return 0;
+}
+
 int upload_plan(lapgpu_ctx* c) {
-  const Plan& P = c->plan;
+  Plan& P = c->plan;
   const int64_t n = P.n;
   int rc;
   if ((rc = dev_upload(c, &c->perm, P.perm))) return rc;
   if ((rc = dev_upload(c, &c->Lptr, P.Lptr))) return rc;
   if ((rc = dev_upload(c, &c->Lidx, P.Lidx))) return rc;
   if ((rc = dev_upload(c, &c->Lval, P.Lval))) return rc;
-  if ((rc = dev_upload(c, &c->Fptr, P.Fptr))) return rc;
-  if ((rc = dev_upload(c, &c->Fidx, P.Fidx))) return rc;
-  if ((rc = dev_upload(c, &c->Fw, P.Fw))) return rc;
-  if ((rc = dev_upload(c, &c->Bptr, P.Bptr))) return rc;
-  if ((rc = dev_upload(c, &c->Bidx, P.Bidx))) return rc;
-  if ((rc = dev_upload(c, &c->Bw, P.Bw))) return rc;
   if ((rc = dev_upload(c, &c->s, P.s))) return rc;
+  if ((rc = upload_groups(c))) return rc;
   if ((rc = dev_upload(c, &c->lvl, P.lvl))) return rc;
   if ((rc = dev_upload(c, &c->Eptr, P.tail.Eptr))) return rc;
   if ((rc = dev_upload(c, &c->Eidx, P.tail.Eidx))) return rc;
@@ -357,7 +432,7 @@ int upload_plan(lapgpu_ctx* c) {
     int rc2 = prepare_f32(c);
     if (rc2) return rc2;
   }
-  double** ws[] = {&c->b, &c->r, &c->x, &c->bestx, &c->p, &c->q, &c->y, &c->z, &c->io};
+  double** ws[] = {&c->b, &c->r, &c->x, &c->bestx, &c->p, &c->q, &c->y, &c->z, &c->io, &c->zF};
   for (double** w : ws)
     if ((rc = dev_alloc(c, w, (size_t)n))) return rc;
   c->n = n;
@@ -436,7 +511,7 @@ void enqueue_iteration(lapgpu_ctx* c, bool prof, bool replace) {
                      c->x, c->bestx, c->p, c->q, c->r, c->part, c->st);
   hipLaunchKernelGGL(k_scalar_nr, dim3(1), dim3(kScalarThreads), 0, c->stream, c->part, c->st);
   if (f32) {
-    LAPGPU_DISPATCH_G(c->G_spmv, launch_resid, c, replace ? 1 : 0);
+    LAPGPU_DISPATCH_G(c->G_spmv, launch_resid<G_>, c, replace ? 1 : 0);
     hipLaunchKernelGGL(k_scalar_resid, dim3(1), dim3(kScalarThreads), 0, c->stream,
                        replace ? 1 : 0, c->part, c->st);
   }
@@ -624,6 +699,8 @@ int lapgpu_create(lapgpu_ctx** out, int device) {
   if (const char* tw = getenv("LAPGPU_TAIL_STEPS")) c->max_tail_steps = atoll(tw);
   if (const char* ug = getenv("LAPGPU_NO_GRAPH")) c->use_graph = atoi(ug) == 0;
   if (const char* sw = getenv("LAPGPU_SWITCH_TOL")) c->switch_tol = atof(sw);
+  if (const char* bd = getenv("LAPGPU_BLOCK_DEPTH")) c->block_depth = atoi(bd);
+  if (const char* bg = getenv("LAPGPU_BLOCK_GROWTH")) c->block_growth = atof(bg);
   *out = c;
   return 0;
 }
@@ -784,7 +861,7 @@ int lapgpu_factor_info(lapgpu_ctx* c, int64_t* nlevels, int64_t* nwide,
   if (!c) return -1;
   if (!c->has_F) return set_err(c, -2, "factor_info: no LDLinv uploaded");
   if (nlevels) *nlevels = c->plan.nlev;
-  if (nwide) *nwide = c->plan.tail_level;
+  if (nwide) *nwide = (int64_t)c->groups.size();
   if (tail_start) *tail_start = c->plan.tail.T0;
   if (nnzF) *nnzF = c->plan.nnzF;
   return 0;

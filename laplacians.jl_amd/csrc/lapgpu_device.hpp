@@ -92,14 +92,21 @@ __device__ __forceinline__ double wave_total(const double* part, int slot) {
 }
 
 // ---------------------------------------------------------------------------
-// The narrow tail of the DAG (the last levels, at most kTailCapV vertices):
-// one workgroup holds y of all tail vertices in LDS and runs the forward
-// sweep, the diagonal scaling and the backward sweep of those levels with a
-// workgroup barrier per level.  The work of each level is a host-packed
-// "slot program" (TailSlot / TailStep of lapgpu_plan.hpp): lane t of step j
-// computes w * y[src], the lanes of an aligned power-of-two segment are
-// summed with xor-shuffles, and the segment's lead lane adds the sum to
-// y[row].  Slots are loaded kTailDepth steps ahead of their use.
+// Slot programs.  The narrow tail of the DAG (the last levels, at most
+// kTailCapV vertices) is run by one workgroup that holds y of all tail
+// vertices in LDS and does the forward sweep, the diagonal scaling and the
+// backward sweep of those levels with one workgroup barrier per level; the
+// batched solver runs whole graphs the same way.  The work of each level is
+// a host-packed "slot program" (TailSlot / TailStep of lapgpu_plan.hpp):
+// lane t of step j computes w * y[src]; the lanes of an aligned
+// power-of-two segment are summed with DPP row shifts so that the segment's
+// LAST lane holds the sum, and that lane adds it to y[row].
+//
+// Latency: every slot load is unconditional (inactive lanes re-read a valid
+// slot of the same step), so the compiler can wait on exactly the loads it
+// needs; slots are loaded D steps ahead, and the step table 64 steps ahead
+// with one vector load per lane, read out with readlane (no scalar loads in
+// the loop, whose counter is shared with LDS).
 
 constexpr int kTailCapV = 18432;
 constexpr int kTailDepth = 8;
@@ -113,33 +120,60 @@ struct alignas(4) TailSlotF {
   unsigned src, meta;
 };
 
-struct alignas(16) TailStepD {
-  int off, count, end, pad;
+// off = first slot; meta = count (active lanes, a multiple of 64) | end << 16
+struct alignas(8) TailStepD {
+  int off, meta;
 };
 
+__device__ __forceinline__ int2 load_step(const TailStepD* __restrict__ steps, int nsteps, int j) {
+  const int jj = j < nsteps ? j : (nsteps > 0 ? nsteps - 1 : 0);
+  const int2 v = *reinterpret_cast<const int2*>(steps + jj);
+  return j < nsteps ? v : make_int2(0, 0);
+}
+
 template <typename S>
-__device__ __forceinline__ S load_slot(const S* __restrict__ slots, const TailStepD& st, int t) {
-  S z;
-  if (t < st.count) {
-    z = slots[st.off + t];
-  } else {
-    z.w = 0;
-    z.src = 0;
-    z.meta = 0;
+__device__ __forceinline__ S load_slot(const S* __restrict__ slots, int off, int count, int t) {
+  const int i = off + (t < count ? t : (count > 0 ? count - 1 : 0));
+  return slots[i];
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_shr(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_shr(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+
+// Sum over aligned segments of 2^seglog lanes; the last lane of each segment
+// receives the segment's sum (other lanes hold partial sums).
+template <typename Y>
+__device__ __forceinline__ Y seg_sum_last(Y v, int seglog) {
+  Y t = dpp_shr<0x111>(v);  // row_shr:1
+  if (seglog >= 1) v += t;
+  t = dpp_shr<0x112>(v);    // row_shr:2
+  if (seglog >= 2) v += t;
+  t = dpp_shr<0x114>(v);    // row_shr:4
+  if (seglog >= 3) v += t;
+  t = dpp_shr<0x118>(v);    // row_shr:8
+  if (seglog >= 4) v += t;
+  if (__any(seglog >= 5)) {
+    t = __shfl_up(v, 16, kWave);
+    if (seglog >= 5) v += t;
+    t = __shfl_up(v, 32, kWave);
+    if (seglog >= 6) v += t;
   }
-  return z;
+  return v;
 }
 
 template <typename S, typename Y>
-__device__ __forceinline__ void run_slot(const S& sl, const TailStepD& st, Y* ly) {
-  if ((int)threadIdx.x < st.count) {
+__device__ __forceinline__ void run_slot(const S& sl, int count, Y* ly) {
+  if ((int)threadIdx.x < count) {  // count is a multiple of 64: wave-uniform
     Y v = sl.w * ly[sl.src];
-    const int seg = 1 << ((sl.meta >> 26) & 7u);
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-      const Y t = __shfl_xor(v, o, kWave);
-      if (o < seg) v += t;
-    }
+    v = seg_sum_last(v, (int)((sl.meta >> 26) & 7u));
     const unsigned lead = sl.meta >> 29;
     const unsigned row = sl.meta & ((1u << 26) - 1);
     if (lead == 1)
@@ -149,40 +183,56 @@ __device__ __forceinline__ void run_slot(const S& sl, const TailStepD& st, Y* ly
   }
 }
 
-// Runs nsteps steps of a slot program on ly, loading slots D steps ahead.
-template <int D = kTailDepth, typename S, typename Y>
-__device__ __forceinline__ void run_program(const TailStepD* __restrict__ steps,
-                                            int nsteps, const S* __restrict__ slots,
-                                            Y* ly) {
-  constexpr int kTailDepth = D;
+// Workgroup barrier for LDS-only communication: waits on LDS operations,
+// not on global loads in flight.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Runs nsteps steps of a slot program on ly (in LDS when LDSY, else in
+// global memory), loading slots D steps ahead.
+template <int D, bool LDSY, typename S, typename Y>
+__device__ __forceinline__ void run_program(const TailStepD* __restrict__ steps, int nsteps,
+                                            const S* __restrict__ slots, Y* ly) {
+  static_assert(64 % D == 0, "D divides 64");
   const int t = threadIdx.x;
-  TailStepD ia[kTailDepth], ib[kTailDepth];
-  S a[kTailDepth], b[kTailDepth];
-  const TailStepD none = {0, 0, 0, 0};
+  const int lane = t & 63;
+  int2 blkA = load_step(steps, nsteps, lane);
+  int2 blkB = load_step(steps, nsteps, 64 + lane);
+  S a[D], b[D];
 #pragma unroll
-  for (int d = 0; d < kTailDepth; ++d) {
-    ia[d] = d < nsteps ? steps[d] : none;
-    a[d] = load_slot(slots, ia[d], t);
-  }
-  for (int c = 0; c < nsteps; c += kTailDepth) {
+  for (int d = 0; d < D; ++d)
+    a[d] = load_slot(slots, __builtin_amdgcn_readlane(blkA.x, d),
+                     __builtin_amdgcn_readlane(blkA.y, d) & 0xffff, t);
+  for (int c = 0; c < nsteps; c += D) {
+    const int cn = c + D;
+    const bool nb = (cn & 63) == 0;
 #pragma unroll
-    for (int d = 0; d < kTailDepth; ++d) {
-      const int j = c + kTailDepth + d;
-      ib[d] = j < nsteps ? steps[j] : none;
-      b[d] = load_slot(slots, ib[d], t);
+    for (int d = 0; d < D; ++d) {
+      const int idx = (cn + d) & 63;
+      const int off = nb ? __builtin_amdgcn_readlane(blkB.x, idx) : __builtin_amdgcn_readlane(blkA.x, idx);
+      const int meta = nb ? __builtin_amdgcn_readlane(blkB.y, idx) : __builtin_amdgcn_readlane(blkA.y, idx);
+      b[d] = load_slot(slots, off, meta & 0xffff, t);
     }
 #pragma unroll
-    for (int d = 0; d < kTailDepth; ++d) {
-      if (c + d < nsteps) {
-        run_slot(a[d], ia[d], ly);
-        if (ia[d].end) __syncthreads();
+    for (int d = 0; d < D; ++d) {
+      const int meta = __builtin_amdgcn_readlane(blkA.y, (c + d) & 63);
+      run_slot(a[d], meta & 0xffff, ly);
+      if (meta >> 16) {
+        if (LDSY)
+          lds_barrier();
+        else
+          __syncthreads();
       }
     }
-#pragma unroll
-    for (int d = 0; d < kTailDepth; ++d) {
-      a[d] = b[d];
-      ia[d] = ib[d];
+    if (nb) {
+      blkA = blkB;
+      blkB = load_step(steps, nsteps, cn + 64 + lane);
     }
+#pragma unroll
+    for (int d = 0; d < D; ++d) a[d] = b[d];
   }
 }
 

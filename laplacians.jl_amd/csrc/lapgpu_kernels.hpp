@@ -75,39 +75,48 @@ __global__ void k_scalar_nb(const double* __restrict__ part, DevState* st) {
 // ---------------------------------------------------------------------------
 // LDLinv sweeps (gathers over one level; G lanes per row)
 
+// One group of consecutive levels (see LevelBlocks in lapgpu_plan.hpp):
+//   out[row] = sum_k w[k] * (idx[k] < n ? A[idx[k]] : B[idx[k] - n])
+// forward: A = y0, B = r, out = y0; backward: A = yF, B = y0, out = yF.
+// Rows with more than kHeavy(G) entries are listed in `heavy` and handled
+// by a whole wave each (threads [0, 64 nh)); the other rows get G lanes each.
 // T is the sweep precision (double, or float in the FP32-inner mode).
-template <int G, typename T>
-__global__ void __launch_bounds__(kBlock)
-k_fwd_level(int lo, int hi, const int* __restrict__ Fptr,
-            const int* __restrict__ Fidx, const T* __restrict__ Fw,
-            const double* __restrict__ r, T* y,
-            const DevState* __restrict__ st) {
-  const int gid = blockIdx.x * kBlock + threadIdx.x;
-  const int row = lo + gid / G;
-  const int lane = gid % G;
-  if (row >= hi) return;
-  const int k0 = Fptr[row], k1 = Fptr[row + 1];
-  T acc = 0;
-  for (int k = k0 + lane; k < k1; k += G) acc += Fw[k] * y[Fidx[k]];
-  acc = group_sum<G>(acc);
-  if (lane == 0 && !st->stopped) y[row] = (T)r[row] + acc;
-}
+template <int G>
+__host__ __device__ constexpr int heavy_limit() { return G >= 64 ? (1 << 30) : 8 * G; }
 
-template <int G, typename T>
+template <int G, typename T, typename TB>
 __global__ void __launch_bounds__(kBlock)
-k_bwd_level(int lo, int hi, const int* __restrict__ Bptr,
-            const int* __restrict__ Bidx, const T* __restrict__ Bw,
-            const T* __restrict__ s, T* y,
-            const DevState* __restrict__ st) {
+k_group(int n, int lo, int hi, const int* __restrict__ heavy, int nh,
+        const int* __restrict__ ptr, const int* __restrict__ idx,
+        const T* __restrict__ w, const T* A, const TB* B, T* out,
+        const DevState* __restrict__ st) {
   const int gid = blockIdx.x * kBlock + threadIdx.x;
-  const int row = lo + gid / G;
-  const int lane = gid % G;
+  if (gid < nh * kWave) {
+    const int row = heavy[gid / kWave];
+    const int lane = gid % kWave;
+    const int k0 = ptr[row], k1 = ptr[row + 1];
+    T acc = 0;
+    for (int k = k0 + lane; k < k1; k += kWave) {
+      const int i = idx[k];
+      acc += w[k] * (i < n ? A[i] : (T)B[i - n]);
+    }
+    acc = group_sum<kWave>(acc);
+    if (lane == 0 && !st->stopped) out[row] = acc;
+    return;
+  }
+  const int g2 = gid - nh * kWave;
+  const int row = lo + g2 / G;
+  const int lane = g2 % G;
   if (row >= hi) return;
-  const int k0 = Bptr[row], k1 = Bptr[row + 1];
+  const int k0 = ptr[row], k1 = ptr[row + 1];
+  if (k1 - k0 > heavy_limit<G>()) return;
   T acc = 0;
-  for (int k = k0 + lane; k < k1; k += G) acc += Bw[k] * y[Bidx[k]];
+  for (int k = k0 + lane; k < k1; k += G) {
+    const int i = idx[k];
+    acc += w[k] * (i < n ? A[i] : (T)B[i - n]);
+  }
   acc = group_sum<G>(acc);
-  if (lane == 0 && !st->stopped) y[row] = s[row] * y[row] + acc;
+  if (lane == 0 && !st->stopped) out[row] = acc;
 }
 
 // ext[i] = r[T0+i] + sum of the forward in-edges of tail row i that come
@@ -133,7 +142,7 @@ k_tail_ext(int T0, int NT, const int* __restrict__ Eptr,
 template <typename T, typename S>
 __global__ void __launch_bounds__(kTailThreads)
 k_tail(int T0, int NT, const T* __restrict__ ext, const T* __restrict__ s,
-       T* __restrict__ y, const TailStepD* __restrict__ fsteps, int nfsteps,
+       T* __restrict__ yF, const TailStepD* __restrict__ fsteps, int nfsteps,
        const S* __restrict__ fslots, const TailStepD* __restrict__ bsteps,
        int nbsteps, const S* __restrict__ bslots,
        const DevState* __restrict__ st) {
@@ -141,13 +150,13 @@ k_tail(int T0, int NT, const T* __restrict__ ext, const T* __restrict__ s,
   if (st->stopped) return;
   for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] = ext[i];
   __syncthreads();
-  run_program(fsteps, nfsteps, fslots, ly);
+  run_program<kTailDepth, true>(fsteps, nfsteps, fslots, ly);
   __syncthreads();
   for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] *= s[T0 + i];
   __syncthreads();
-  run_program(bsteps, nbsteps, bslots, ly);
+  run_program<kTailDepth, true>(bsteps, nbsteps, bslots, ly);
   __syncthreads();
-  for (int i = threadIdx.x; i < NT; i += kTailThreads) y[T0 + i] = ly[i];
+  for (int i = threadIdx.x; i < NT; i += kTailThreads) yF[T0 + i] = ly[i];
 }
 
 // partial sum of y (for the mean removal of LDLsolver)

@@ -134,7 +134,7 @@ void pack_level(std::vector<RowEntries>& rows, std::vector<TailStep>& steps,
     for (int64_t t = 0; t < C.size; ++t) {
       TailSlot& S = pass[p][off + t];
       S.meta = slot_meta((uint32_t)C.row, (uint32_t)log2_exact(C.size),
-                         t == 0 ? (C.atomic ? 2u : 1u) : 0u);
+                         t == C.size - 1 ? (C.atomic ? 2u : 1u) : 0u);
       if (C.k0 + t < C.k1) {
         S.src = (uint32_t)C.R->e[C.k0 + t].first;
         S.w = C.R->e[C.k0 + t].second;
@@ -143,7 +143,7 @@ void pack_level(std::vector<RowEntries>& rows, std::vector<TailStep>& steps,
   }
   for (size_t p = 0; p < pass.size(); ++p) {
     const int64_t cnt = (ptr[p] + 63) / 64 * 64;
-    TailStep st{(int32_t)slots.size(), (int32_t)cnt, p + 1 == pass.size() ? 1 : 0, 0};
+    TailStep st{(int32_t)slots.size(), (int32_t)(cnt | ((p + 1 == pass.size() ? 1 : 0) << 16))};
     steps.push_back(st);
     slots.insert(slots.end(), pass[p].begin(), pass[p].begin() + cnt);
   }
@@ -213,6 +213,138 @@ bool build_tail(Plan& P, int64_t max_steps, int64_t cap) {
     }
     return ok;
   }
+
+namespace {
+
+// Sparse accumulator over [0, 2n) for composite rows.
+struct Accum {
+  std::vector<double> val;
+  std::vector<int32_t> mark, touched;
+  int32_t stamp = 0;
+  explicit Accum(int64_t size) : val(size, 0.0), mark(size, -1) {}
+  void begin() {
+    ++stamp;
+    touched.clear();
+  }
+  void add(int32_t i, double w) {
+    if (mark[i] != stamp) {
+      mark[i] = stamp;
+      val[i] = 0.0;
+      touched.push_back(i);
+    }
+    val[i] += w;
+  }
+};
+
+using Rep = std::vector<std::pair<int32_t, double>>;
+
+void take(Accum& A, Rep& out) {
+  std::sort(A.touched.begin(), A.touched.end());
+  out.clear();
+  out.reserve(A.touched.size());
+  for (int32_t i : A.touched) out.emplace_back(i, A.val[i]);
+}
+
+}  // namespace
+
+void build_blocks(Plan& P, int max_depth, double growth, int64_t max_extra) {
+  const int64_t n = P.n;
+  const int64_t L = P.tail_level;
+  LevelBlocks& B = P.blocks;
+  B = LevelBlocks();
+  const int64_t T0 = P.lvl[L];
+  B.fptr.assign(T0 + 1, 0);
+  B.bptr.assign(T0 + 1, 0);
+  if (L == 0) {
+    B.lev = {0};
+    return;
+  }
+  Accum acc(2 * n);
+  std::vector<Rep> frep(T0), brep(T0);
+  // forward composites and the grouping (forward reps of a group's earlier
+  // levels do not change when the group grows at its end)
+  int64_t L0 = 0;
+  B.lev.push_back(0);
+  int64_t orig_block = 0, comp_block = 0;
+  for (int64_t l = 0; l < L; ++l) {
+    const int64_t lo = P.lvl[l], hi = P.lvl[l + 1];
+    int64_t orig = 0, comp = 0;
+    auto compose = [&](int64_t start) {
+      orig = comp = 0;
+      for (int64_t v = lo; v < hi; ++v) {
+        acc.begin();
+        acc.add((int32_t)(n + v), 1.0);
+        for (int32_t k = P.Fptr[v]; k < P.Fptr[v + 1]; ++k) {
+          const int32_t c = P.Fidx[k];
+          if (c < P.lvl[start]) {
+            acc.add(c, P.Fw[k]);
+          } else {
+            for (const auto& e : frep[c]) acc.add(e.first, P.Fw[k] * e.second);
+          }
+        }
+        take(acc, frep[v]);
+        orig += P.Fptr[v + 1] - P.Fptr[v] + 1;
+        comp += (int64_t)frep[v].size();
+      }
+    };
+    bool joined = false;
+    if (l > L0 && l - L0 < max_depth) {
+      compose(L0);
+      if (comp <= growth * (double)orig && comp - orig <= max_extra &&
+          comp_block + comp <= growth * (double)(orig_block + orig)) {
+        joined = true;
+        orig_block += orig;
+        comp_block += comp;
+      }
+    }
+    if (!joined) {
+      L0 = l;
+      if (l > 0) B.lev.push_back((int32_t)l);
+      compose(L0);  // a fresh group: every in-edge is outside it
+      orig_block = orig;
+      comp_block = comp;
+    }
+  }
+  B.lev.push_back((int32_t)L);
+  // backward composites per group (top level first)
+  for (size_t g = 0; g + 1 < B.lev.size(); ++g) {
+    const int64_t g0 = B.lev[g], g1 = B.lev[g + 1];
+    const int64_t top = P.lvl[g1];
+    for (int64_t l = g1 - 1; l >= g0; --l) {
+      for (int64_t v = P.lvl[l]; v < P.lvl[l + 1]; ++v) {
+        acc.begin();
+        acc.add((int32_t)(n + v), P.s[v]);
+        for (int32_t k = P.Bptr[v]; k < P.Bptr[v + 1]; ++k) {
+          const int32_t j = P.Bidx[k];
+          if (j >= top) {
+            acc.add(j, P.Bw[k]);
+          } else {
+            for (const auto& e : brep[j]) acc.add(e.first, P.Bw[k] * e.second);
+          }
+        }
+        take(acc, brep[v]);
+      }
+    }
+  }
+  for (int64_t v = 0; v < T0; ++v) {
+    B.fptr[v + 1] = B.fptr[v] + (int32_t)frep[v].size();
+    B.bptr[v + 1] = B.bptr[v] + (int32_t)brep[v].size();
+  }
+  B.fidx.reserve(B.fptr[T0]);
+  B.fw.reserve(B.fptr[T0]);
+  B.bidx.reserve(B.bptr[T0]);
+  B.bw.reserve(B.bptr[T0]);
+  for (int64_t v = 0; v < T0; ++v) {
+    for (const auto& e : frep[v]) {
+      B.fidx.push_back(e.first);
+      B.fw.push_back(e.second);
+    }
+    for (const auto& e : brep[v]) {
+      B.bidx.push_back(e.first);
+      B.bw.push_back(e.second);
+    }
+  }
+}
 
 bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
                 const int64_t* colptr, const int64_t* rowval,

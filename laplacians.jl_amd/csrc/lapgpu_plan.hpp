@@ -17,7 +17,7 @@ struct HostCsr {
 };
 
 // One lane's work in a slot program: w * y[src] summed over an aligned
-// segment of 2^seglog lanes; the segment's first lane adds the sum to y[row]
+// segment of 2^seglog lanes; the segment's last lane adds the sum to y[row]
 // (lead = 1: plain add; lead = 2: atomic add, for rows split in chunks).
 // meta packs row (bits 0-25), seglog (bits 26-28) and lead (bits 29-30).
 // Indices are local to the program (position - T0).
@@ -32,13 +32,12 @@ inline uint32_t slot_meta(uint32_t row, uint32_t seglog, uint32_t lead) {
 }
 static_assert(sizeof(TailSlot) == 16, "TailSlot is 16 bytes");
 
-// A step = up to kTailLanes slots run by the tail workgroup at once; `end`
+// A step = up to kTailLanes slots run by the tail workgroup at once.
+// meta = count (active lanes, a multiple of 64) | end << 16, where `end`
 // marks the last step of a level (a workgroup barrier follows it).
 struct TailStep {
-  int32_t off;    // first slot
-  int32_t count;  // active lanes (multiple of 64)
-  int32_t end;
-  int32_t pad;
+  int32_t off;   // first slot
+  int32_t meta;
 };
 
 constexpr int kTailLanes = 1024;
@@ -51,6 +50,20 @@ struct TailProgram {
   // forward in-edges of tail rows from outside the tail (CSR over tail rows)
   std::vector<int32_t> Eptr, Eidx;
   std::vector<double> Ew;
+};
+
+// Groups of consecutive wide levels computed by one launch each.  Row v of
+// a group gets a composite row over the vertices outside the group plus
+// "self" terms of the group's rows, with indices idx < n naming A and
+// idx >= n naming B[idx - n]:
+//   forward:  y0[v] = sum w * (idx < n ? y0[idx] : r[idx - n])
+//   backward: yF[v] = sum w * (idx < n ? yF[idx] : y0[idx - n])
+// (a one-level group holds exactly the original gathers plus the r[v] or
+// s[v]*y0[v] term).
+struct LevelBlocks {
+  std::vector<int32_t> lev;  // group g = levels [lev[g], lev[g+1])
+  std::vector<int32_t> fptr, fidx, bptr, bidx;  // CSR over positions [0, T0)
+  std::vector<double> fw, bw;
 };
 
 // Everything the device needs, in "level order": vertices sorted by the
@@ -78,6 +91,7 @@ struct Plan {
   int64_t nlev = 0;
   int64_t tail_level = 0;  // first level of the single-workgroup tail
   TailProgram tail;
+  LevelBlocks blocks;
 };
 
 // Convert a caller CSC (int64, base 0/1) to 0-based int32 CSR.
@@ -95,5 +109,10 @@ bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
 // vertices whose levels need at most `max_steps` steps each.  With both
 // limits at INT64_MAX the program covers the whole sweep (tail_level 0).
 bool build_tail(Plan& P, int64_t max_steps, int64_t cap);
+
+// Group the levels before the tail into launches of up to `max_depth`
+// levels while a group's composite rows stay within `growth` times the
+// entries of its levels (and `max_extra` extra entries).
+void build_blocks(Plan& P, int max_depth, double growth, int64_t max_extra);
 
 }  // namespace lapgpu

@@ -68,20 +68,20 @@ __device__ __forceinline__ void wg_sum(double (&v)[NV], double* red) {
 
 // z = LDLsolver(r) (without mean removal: y is returned in ybuf); the mean
 // removal is fused into the caller's next pass.
-template <typename Y>
+template <bool LDSY, typename Y>
 __device__ __forceinline__ void wg_ldl(const WgGraphD& G, const double* r, Y* y) {
   const int n = G.n;
   for (int i = threadIdx.x; i < n; i += kWgThreads) y[i] = r[i];
   __syncthreads();
-  run_program<4>(G.fsteps, G.nf, G.fslots, y);
+  run_program<4, LDSY>(G.fsteps, G.nf, G.fslots, y);
   __syncthreads();
   for (int i = threadIdx.x; i < n; i += kWgThreads) y[i] *= G.s[i];
   __syncthreads();
-  run_program<4>(G.bsteps, G.nb, G.bslots, y);
+  run_program<4, LDSY>(G.bsteps, G.nb, G.bslots, y);
   __syncthreads();
 }
 
-template <typename Y>
+template <bool LDSY, typename Y>
 __device__ __forceinline__ void wg_pcg(const WgGraphD& G, WgReqD& R, Y* y, double* red) {
   const int n = G.n;
   const int t = threadIdx.x;
@@ -121,7 +121,7 @@ __device__ __forceinline__ void wg_pcg(const WgGraphD& G, WgReqD& R, Y* y, doubl
       return;
     }
     // z = pre(r); p = z; rho = r.z
-    wg_ldl(G, r, y);
+    wg_ldl<LDSY>(G, r, y);
     double m[1] = {0.0};
     for (int i = t; i < n; i += kWgThreads) m[0] += y[i];
     wg_sum<1>(m, red);
@@ -200,7 +200,7 @@ __device__ __forceinline__ void wg_pcg(const WgGraphD& G, WgReqD& R, Y* y, doubl
         stop = 0;
         break;
       }
-      wg_ldl(G, r, y);
+      wg_ldl<LDSY>(G, r, y);
       double m2[1] = {0.0};
       for (int i = t; i < n; i += kWgThreads) m2[0] += y[i];
       wg_sum<1>(m2, red);
@@ -274,9 +274,9 @@ k_wg_pcg(const WgGraphD* __restrict__ graphs, WgReqD* __restrict__ reqs) {
   WgReqD R = reqs[blockIdx.x];
   const WgGraphD G = graphs[R.graph];
   if (G.n <= kTailCapV)
-    wg_pcg(G, R, ly, red);
+    wg_pcg<true>(G, R, ly, red);
   else
-    wg_pcg(G, R, R.work + 7 * (size_t)G.n, red);
+    wg_pcg<false>(G, R, R.work + 7 * (size_t)G.n, red);
   if (threadIdx.x == 0) {
     reqs[blockIdx.x].its = R.its;
     reqs[blockIdx.x].nr = R.nr;
