@@ -44,7 +44,7 @@ struct lapgpu_ctx {
   bool has_L = false, has_F = false;
   Plan plan;
   int64_t n = 0;
-  int64_t max_tail_steps = 12;
+  int64_t max_tail_steps = 24;
   int G_spmv = 8, G_ldl = 8;
   int precision = LAPGPU_FP64, replace_every = 20;
   double switch_tol = 1e-5;  // FP32 mode: residual below which the SpMV is FP64
@@ -66,8 +66,8 @@ struct lapgpu_ctx {
   double *gFw = nullptr, *gBw = nullptr;
   float *gFw32 = nullptr, *gBw32 = nullptr, *z32 = nullptr;
   double* zF = nullptr;  // backward output yF (double path)
-  int block_depth = 8;
-  double block_growth = 3.0;
+  int block_depth = 16;
+  double block_growth = 4.0;
   int64_t block_extra = 1 << 22;
   // tail program
   int *Eptr = nullptr, *Eidx = nullptr;

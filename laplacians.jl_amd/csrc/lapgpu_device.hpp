@@ -137,35 +137,37 @@ __device__ __forceinline__ S load_slot(const S* __restrict__ slots, int off, int
   return slots[i];
 }
 
-template <int CTRL>
-__device__ __forceinline__ double dpp_shr(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, true);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, true);
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xF, true);
   return __hiloint2double(hi, lo);
 }
-template <int CTRL>
-__device__ __forceinline__ float dpp_shr(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, true));
 }
 
 // Sum over aligned segments of 2^seglog lanes; the last lane of each segment
-// receives the segment's sum (other lanes hold partial sums).
+// receives the segment's sum (other lanes hold partial sums).  Row shifts
+// 1, 2, 4, 8 inside each 16-lane row, then row_bcast:15 (lane 15 of a row
+// into the next row, rows 1 and 3) and row_bcast:31 (lane 31 into rows 2
+// and 3); lanes a DPP move does not write read 0.
 template <typename Y>
 __device__ __forceinline__ Y seg_sum_last(Y v, int seglog) {
-  Y t = dpp_shr<0x111>(v);  // row_shr:1
+  Y t = dpp_mov<0x111, 0xF>(v);  // row_shr:1
   if (seglog >= 1) v += t;
-  t = dpp_shr<0x112>(v);    // row_shr:2
+  t = dpp_mov<0x112, 0xF>(v);    // row_shr:2
   if (seglog >= 2) v += t;
-  t = dpp_shr<0x114>(v);    // row_shr:4
+  t = dpp_mov<0x114, 0xF>(v);    // row_shr:4
   if (seglog >= 3) v += t;
-  t = dpp_shr<0x118>(v);    // row_shr:8
+  t = dpp_mov<0x118, 0xF>(v);    // row_shr:8
   if (seglog >= 4) v += t;
-  if (__any(seglog >= 5)) {
-    t = __shfl_up(v, 16, kWave);
-    if (seglog >= 5) v += t;
-    t = __shfl_up(v, 32, kWave);
-    if (seglog >= 6) v += t;
-  }
+  t = dpp_mov<0x142, 0xA>(v);    // row_bcast:15
+  if (seglog >= 5) v += t;
+  t = dpp_mov<0x143, 0xC>(v);    // row_bcast:31
+  if (seglog >= 6) v += t;
   return v;
 }
 
