@@ -99,6 +99,27 @@ int lapgpu_solve_device(lapgpu_ctx* ctx, const double* d_b, double* d_x,
                         int stag_test, int flags, int64_t* its_out,
                         double* relres_out, int* stop_reason_out);
 
+/* k right-hand sides of the same graph at once: B and X_out are n-by-k,
+ * column-major (column j at B + j*n, a Julia Matrix{Float64}).  Column j
+ * gets the result of lapgpu_solve(B[:, j]) with its own stopping rules
+ * (itcnt, relres and stop reason in its_out[j], relres_out[j],
+ * stop_reason_out[j]; arrays of k, each may be NULL), all columns sharing
+ * every launch.  The reference solves one vector per call of the closure
+ * (src/approxChol.jl:1624); this is the same closure applied column by
+ * column.  Columns are processed 64 at a time, always in FP64 (the
+ * precision mode applies to lapgpu_solve). */
+int lapgpu_solve_multi(lapgpu_ctx* ctx, const double* B, double* X_out, int64_t n,
+                       int64_t k, double tol, int64_t maxits, double maxtime,
+                       int stag_test, int flags, int64_t* its_out,
+                       double* relres_out, int* stop_reason_out);
+
+/* As lapgpu_solve_multi, with B and X_out in device memory. */
+int lapgpu_solve_multi_device(lapgpu_ctx* ctx, const double* d_B, double* d_X,
+                              int64_t n, int64_t k, double tol, int64_t maxits,
+                              double maxtime, int stag_test, int flags,
+                              int64_t* its_out, double* relres_out,
+                              int* stop_reason_out);
+
 /* Apply the preconditioner once: z = LDLsolver(ldli, r)
  * (src/approxChol.jl:1347-1366), host buffers. */
 int lapgpu_apply_ldli(lapgpu_ctx* ctx, const double* r, double* z, int64_t n);

@@ -17,7 +17,7 @@
 #include <vector>
 
 #include "lapgpu.h"
-#include "lapgpu_kernels.hpp"
+#include "lapgpu_multi.hpp"
 #include "lapgpu_plan.hpp"
 
 using namespace lapgpu;
@@ -94,6 +94,20 @@ struct lapgpu_ctx {
   int32_t* hflags = nullptr;      // pinned ring of stop flags
   static constexpr int kRing = 4;
   hipEvent_t ring_ev[kRing] = {};
+
+  // multi-RHS workspace (lapgpu_solve_multi): K interleaved right-hand
+  // sides of length n; the tail CSRs are built on first use
+  struct Multi {
+    int64_t cap = 0;  // elements of each vector (n * K)
+    int exec_K = 0;   // K of the captured iteration
+    double *r = nullptr, *x = nullptr, *bestx = nullptr, *p = nullptr, *q = nullptr,
+           *y = nullptr, *z = nullptr, *io = nullptr, *part = nullptr;
+    DevState* st = nullptr;
+    DevState* hst = nullptr;  // pinned, 64 states
+    int* flags = nullptr;
+    double* ext = nullptr;  // tail inputs, NT * K
+    hipGraphExec_t exec = nullptr;
+  } m;
 
   // profiling
   hipEvent_t ev[8] = {};
@@ -275,6 +289,22 @@ void drop_graphs(lapgpu_ctx* c) {
   if (c->iter_exec) (void)hipGraphExecDestroy(c->iter_exec);
   if (c->iter_exec_rep) (void)hipGraphExecDestroy(c->iter_exec_rep);
   c->iter_exec = c->iter_exec_rep = nullptr;
+  if (c->m.exec) (void)hipGraphExecDestroy(c->m.exec);
+  c->m.exec = nullptr;
+}
+
+// the multi-RHS workspace and tail CSRs (rebuilt on the next multi solve)
+void free_multi(lapgpu_ctx* c) {
+  auto& m = c->m;
+  if (m.exec) (void)hipGraphExecDestroy(m.exec);
+  m.exec = nullptr;
+  double* dp[] = {m.r, m.x, m.bestx, m.p, m.q, m.y, m.z, m.io, m.part, m.ext};
+  for (double* a : dp)
+    if (a) (void)hipFree(a);
+  if (m.flags) (void)hipFree(m.flags);
+  if (m.st) (void)hipFree(m.st);
+  if (m.hst) (void)hipHostFree(m.hst);
+  m = lapgpu_ctx::Multi();
 }
 
 void free_f32(lapgpu_ctx* c) {
@@ -291,6 +321,7 @@ void free_f32(lapgpu_ctx* c) {
 int free_graph(lapgpu_ctx* c) {
   drop_graphs(c);
   free_f32(c);
+  free_multi(c);
   void* tp[] = {c->Eptr, c->Eidx, c->Ew, c->ext, c->fsteps, c->bsteps, c->fslots, c->bslots};
   for (void* a : tp)
     if (a) (void)hipFree(a);
@@ -467,21 +498,26 @@ void identity_plan(lapgpu_ctx* c) {
   P.tail = TailProgram();
 }
 
-void init_state(lapgpu_ctx* c, double tol, int64_t maxits, double maxtime,
-                int stag_test, int flags) {
-  DevState& h = *c->hst;
+void fill_state(const lapgpu_ctx* c, DevState& h, double tol, int64_t maxits,
+                double maxtime, int stag_test, int flags, bool fp32) {
   std::memset(&h, 0, sizeof(DevState));
   h.tol = tol;
   h.maxits = maxits;
   h.stag_test = stag_test;
   h.stag_fac = 1.0 - 1.0 / (double)stag_test;  // -Inf for stag_test == 0
   h.center = (flags & LAPGPU_NO_CENTER) ? 0 : 1;
-  h.fp32 = c->precision == LAPGPU_FP32_INNER ? 1 : 0;
+  h.fp32 = fp32 ? 1 : 0;
   h.switch_tol = c->switch_tol;
   h.has_maxtime = (maxtime > 0 && std::isfinite(maxtime)) ? 1 : 0;
   h.maxticks = h.has_maxtime ? (uint64_t)(maxtime * c->wallclock_khz * 1e3) : 0;
   h.bestnr = 1.0;
   h.nr = 1.0;
+}
+
+void init_state(lapgpu_ctx* c, double tol, int64_t maxits, double maxtime,
+                int stag_test, int flags) {
+  fill_state(c, *c->hst, tol, maxits, maxtime, stag_test, flags,
+             c->precision == LAPGPU_FP32_INNER);
 }
 
 const char* stop_message(int reason) {
@@ -657,6 +693,237 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
   return 0;
 }
 
+// ---- k right-hand sides at once (lapgpu_multi.hpp) ---------------------------
+
+// Calls F with the compile-time rhs count K_ = k (8, 16, 32 or 64).
+#define LAPGPU_DISPATCH_K(k, F, ...)                                \
+  switch (k) {                                                      \
+    case 8: { constexpr int K_ = 8; F(__VA_ARGS__); break; }        \
+    case 16: { constexpr int K_ = 16; F(__VA_ARGS__); break; }      \
+    case 32: { constexpr int K_ = 32; F(__VA_ARGS__); break; }      \
+    default: { constexpr int K_ = 64; F(__VA_ARGS__); break; }      \
+  }
+
+constexpr int kMultiMax = 64;
+
+// Workspace for n * K elements per vector (kept while it is large enough).
+int multi_prepare(lapgpu_ctx* c, int K) {
+  auto& m = c->m;
+  const int64_t n = c->n;
+  int rc;
+  if (!m.part) {
+    if ((rc = dev_alloc(c, &m.part, (size_t)kNumSlots * kMRed * kMultiMax))) return rc;
+    if ((rc = dev_alloc(c, &m.st, (size_t)kMultiMax))) return rc;
+    if ((rc = dev_alloc(c, &m.flags, 2))) return rc;
+    HIPC(c, hipHostMalloc((void**)&m.hst, sizeof(DevState) * kMultiMax, hipHostMallocDefault));
+  }
+  const int64_t need = n * K;
+  if (need > m.cap) {
+    if (m.exec) (void)hipGraphExecDestroy(m.exec);
+    m.exec = nullptr;
+    double** ws[] = {&m.r, &m.x, &m.bestx, &m.p, &m.q, &m.y, &m.z, &m.io};
+    for (double** w : ws)
+      if ((rc = dev_alloc(c, w, (size_t)need))) return rc;
+    if ((rc = dev_alloc(c, &m.ext, (size_t)std::max<int64_t>(c->plan.tail.NT, 1) * K)))
+      return rc;
+    m.cap = need;
+  }
+  return 0;
+}
+
+// z = LDLsolver(rin) for K interleaved right-hand sides: forward groups,
+// tail, backward groups, then the mean removal partials and z.r
+template <int K>
+void msweeps(lapgpu_ctx* c, const double* rin) {
+  auto& m = c->m;
+  const Plan& P = c->plan;
+  const int n = (int)c->n;
+  for (const auto& g : c->groups)
+    hipLaunchKernelGGL(k_mgroup<K>, dim3(grid_for((int64_t)(g.hi - g.lo) * K, kBlock)),
+                       dim3(kBlock), 0, c->stream, n, g.lo, g.hi, c->gFptr, c->gFidx, c->gFw,
+                       m.y, rin, m.y, m.flags);
+  const TailProgram& TP = P.tail;
+  if (TP.NT > 0) {
+    hipLaunchKernelGGL(k_mtail_ext<K>, dim3(grid_for(TP.NT * K, kBlock)), dim3(kBlock), 0,
+                       c->stream, (int)TP.T0, (int)TP.NT, c->Eptr, c->Eidx, c->Ew, rin, m.y,
+                       m.ext, m.flags);
+    hipLaunchKernelGGL(k_mtail<K>, dim3(K), dim3(kTailThreads), 0, c->stream, (int)TP.T0,
+                       (int)TP.NT, m.ext, c->s, m.z, c->fsteps, (int)TP.fsteps.size(), c->fslots,
+                       c->bsteps, (int)TP.bsteps.size(), c->bslots, m.st, m.flags);
+  }
+  for (auto it = c->groups.rbegin(); it != c->groups.rend(); ++it)
+    hipLaunchKernelGGL(k_mgroup<K>, dim3(grid_for((int64_t)(it->hi - it->lo) * K, kBlock)),
+                       dim3(kBlock), 0, c->stream, n, it->lo, it->hi, c->gBptr, c->gBidx,
+                       c->gBw, m.z, m.y, m.z, m.flags);
+}
+
+template <int K>
+void mfinish_precond(lapgpu_ctx* c, const double* rin) {
+  auto& m = c->m;
+  const int n = (int)c->n;
+  hipLaunchKernelGGL(k_mvsum<K>, dim3(kMRed), dim3(kBlock), 0, c->stream, n, m.z, m.part,
+                     m.flags);
+  hipLaunchKernelGGL(k_mscalar_mu<K>, dim3(1), dim3(kScalarThreads), 0, c->stream, n, m.part,
+                     m.st, m.flags);
+  hipLaunchKernelGGL(k_mzdot<K>, dim3(kMRed), dim3(kBlock), 0, c->stream, n, m.z, rin, m.p,
+                     m.part, m.st, m.flags);
+}
+
+// one PCG iteration for K right-hand sides
+template <int K>
+void enqueue_miter(lapgpu_ctx* c, bool prof) {
+  auto& m = c->m;
+  const int n = (int)c->n;
+  if (prof) (void)hipEventRecord(c->ev[0], c->stream);
+  hipLaunchKernelGGL(k_mspmv<K>, dim3(kMRed), dim3(kBlock), 0, c->stream, n, c->Lptr, c->Lidx,
+                     c->Lval, m.p, m.q, m.part, m.flags);
+  if (prof) (void)hipEventRecord(c->ev[1], c->stream);
+  hipLaunchKernelGGL(k_mscalar_al<K>, dim3(1), dim3(kScalarThreads), 0, c->stream, m.part, m.st,
+                     m.flags);
+  hipLaunchKernelGGL(k_mupdate<K>, dim3(kMRed), dim3(kBlock), 0, c->stream, n, m.x, m.bestx,
+                     m.p, m.q, m.r, m.part, m.st, m.flags);
+  hipLaunchKernelGGL(k_mscalar_nr<K>, dim3(1), dim3(kScalarThreads), 0, c->stream, m.part, m.st,
+                     m.flags);
+  if (prof) (void)hipEventRecord(c->ev[2], c->stream);
+  msweeps<K>(c, m.r);
+  if (prof) (void)hipEventRecord(c->ev[3], c->stream);
+  mfinish_precond<K>(c, m.r);
+  hipLaunchKernelGGL(k_mscalar_rho<K>, dim3(1), dim3(kScalarThreads), 0, c->stream, m.part,
+                     m.st, m.flags);
+  hipLaunchKernelGGL(k_mpupdate<K>, dim3(kMRed), dim3(kBlock), 0, c->stream, n, m.z, m.p, m.st,
+                     m.flags);
+  if (prof) (void)hipEventRecord(c->ev[4], c->stream);
+}
+
+// columns [0, k) of d_B (k <= K), results to d_X
+template <int K>
+int solve_multi_chunk(lapgpu_ctx* c, const double* d_B, double* d_X, int k, double tol,
+                      int64_t maxits, double maxtime, int stag_test, int flags,
+                      int64_t* its_out, double* relres_out, int* stop_out, double* acc) {
+  auto& m = c->m;
+  const bool prof = (flags & LAPGPU_PROFILE) != 0;
+  const int n = (int)c->n;
+  int rc = multi_prepare(c, K);
+  if (rc) return rc;
+  for (int j = 0; j < K; ++j)
+    fill_state(c, m.hst[j], tol, maxits, maxtime, stag_test, flags, false);
+  HIPC(c, hipMemcpyAsync(m.st, m.hst, sizeof(DevState) * K, hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemsetAsync(m.flags, 0, 2 * sizeof(int), c->stream));
+
+  // setup: r = b - mean(b), z = pre(r), p = z, rho = r.z (q holds b)
+  hipLaunchKernelGGL(k_mgather<K>, dim3(kMRed), dim3(kBlock), 0, c->stream, n, k, c->perm, d_B,
+                     m.q, m.part);
+  hipLaunchKernelGGL(k_mscalar_mean<K>, dim3(1), dim3(kScalarThreads), 0, c->stream, n, m.part,
+                     m.st);
+  hipLaunchKernelGGL(k_mcenter<K>, dim3(kMRed), dim3(kBlock), 0, c->stream, n, m.q, m.r, m.x,
+                     m.bestx, m.part, m.st);
+  hipLaunchKernelGGL(k_mscalar_nb<K>, dim3(1), dim3(kScalarThreads), 0, c->stream, m.part, m.st,
+                     m.flags);
+  msweeps<K>(c, m.r);
+  mfinish_precond<K>(c, m.r);
+  hipLaunchKernelGGL(k_mscalar_rho0<K>, dim3(1), dim3(kScalarThreads), 0, c->stream, m.part,
+                     m.st, m.flags);
+  hipLaunchKernelGGL(k_mpinit<K>, dim3(kMRed), dim3(kBlock), 0, c->stream, n, m.z, m.p, m.st,
+                     m.flags);
+  HIPC(c, hipGetLastError());
+  HIPC(c, hipMemcpyAsync(&c->hflags[0], m.flags, sizeof(int32_t), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  bool done = c->hflags[0] != 0;
+
+  if (!prof && c->use_graph && (!m.exec || m.exec_K != K)) {
+    if (m.exec) (void)hipGraphExecDestroy(m.exec);
+    m.exec = nullptr;
+    hipGraph_t g = nullptr;
+    HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_miter<K>(c, false);
+    HIPC(c, hipStreamEndCapture(c->stream, &g));
+    hipError_t ie = hipGraphInstantiate(&m.exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPC(c, ie);
+    m.exec_K = K;
+  }
+  int64_t enq = 0, polled = 0;
+  while (!done) {
+    if (!prof && m.exec)
+      HIPC(c, hipGraphLaunch(m.exec, c->stream));
+    else
+      enqueue_miter<K>(c, prof);
+    const int slot = (int)(enq % lapgpu_ctx::kRing);
+    HIPC(c, hipMemcpyAsync(&c->hflags[slot], m.flags, sizeof(int32_t), hipMemcpyDeviceToHost,
+                           c->stream));
+    HIPC(c, hipEventRecord(c->ring_ev[slot], c->stream));
+    ++enq;
+    if (prof) {
+      HIPC(c, hipEventSynchronize(c->ev[4]));
+      float a = 0, b2 = 0, d = 0, e = 0;
+      (void)hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
+      (void)hipEventElapsedTime(&b2, c->ev[2], c->ev[3]);
+      (void)hipEventElapsedTime(&d, c->ev[1], c->ev[2]);
+      (void)hipEventElapsedTime(&e, c->ev[3], c->ev[4]);
+      acc[0] += a;
+      acc[1] += b2;
+      acc[2] += d + e;
+      acc[3] += 1;
+    }
+    while (!done && polled < enq - 1) {
+      const int ps = (int)(polled % lapgpu_ctx::kRing);
+      HIPC(c, hipEventSynchronize(c->ring_ev[ps]));
+      ++polled;
+      if (c->hflags[ps]) done = true;
+    }
+  }
+  hipLaunchKernelGGL(k_mfinal<K>, dim3(kMRed), dim3(kBlock), 0, c->stream, n, k, c->perm, m.x,
+                     m.bestx, d_X, m.st);
+  HIPC(c, hipMemcpyAsync(m.hst, m.st, sizeof(DevState) * K, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, hipGetLastError());
+  for (int j = 0; j < k; ++j) {
+    const DevState& h = m.hst[j];
+    if (its_out) its_out[j] = h.itcnt;
+    if (relres_out) relres_out[j] = h.nr;
+    if (stop_out) stop_out[j] = h.stop_reason;
+    if (h.stop_reason == 3) printf("PCG Stopped by stagnation test %d\n", stag_test);
+    if ((flags & LAPGPU_VERBOSE) && h.stop_reason != 9)
+      printf("rhs %d: PCG stopped after %lld iterations with relative error %.17g (%s).\n", j,
+             (long long)h.itcnt, h.nr, h.stop_reason == 1 ? "maxits" : stop_message(h.stop_reason));
+  }
+  fflush(stdout);
+  return 0;
+}
+
+int solve_multi_device(lapgpu_ctx* c, const double* d_B, double* d_X, int64_t k, double tol,
+                       int64_t maxits, double maxtime, int stag_test, int flags,
+                       int64_t* its_out, double* relres_out, int* stop_out) {
+  if (!c->has_F) return set_err(c, -2, "solve_multi: upload the Laplacian and LDLinv first");
+  const auto t_start = std::chrono::steady_clock::now();
+  const int64_t n = c->n;
+  double acc[4] = {0, 0, 0, 0};
+  int64_t maxit = 0;
+  for (int64_t c0 = 0; c0 < k; c0 += kMultiMax) {
+    const int kc = (int)std::min<int64_t>(kMultiMax, k - c0);
+    int K = 8;
+    while (K < kc) K *= 2;
+    int rc = 0;
+    LAPGPU_DISPATCH_K(K, rc = solve_multi_chunk<K_>, c, d_B + c0 * n, d_X + c0 * n, kc, tol,
+                      maxits, maxtime, stag_test, flags, its_out ? its_out + c0 : nullptr,
+                      relres_out ? relres_out + c0 : nullptr, stop_out ? stop_out + c0 : nullptr,
+                      acc);
+    if (rc) return rc;
+    for (int j = 0; j < kc; ++j) maxit = std::max<int64_t>(maxit, c->m.hst[j].itcnt);
+  }
+  if ((flags & LAPGPU_PROFILE) && acc[3] > 0) {
+    c->t_spmv = acc[0] / acc[3];
+    c->t_ldl = acc[1] / acc[3];
+    c->t_vec = acc[2] / acc[3];
+  }
+  c->t_total =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count() * 1e3;
+  c->t_its = maxit;
+  c->err.clear();
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -793,6 +1060,50 @@ int lapgpu_solve(lapgpu_ctx* c, const double* b, double* x_out, int64_t n,
   if (rc) return rc;
   HIPC(c, hipMemcpyAsync(x_out, c->io, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int lapgpu_solve_multi_device(lapgpu_ctx* c, const double* d_B, double* d_X, int64_t n,
+                              int64_t k, double tol, int64_t maxits, double maxtime,
+                              int stag_test, int flags, int64_t* its_out, double* relres_out,
+                              int* stop_out) {
+  if (!c) return -1;
+  if (n != c->n) return set_err(c, -1, "solve_multi: number of rows of B differs from n");
+  if (k < 0 || (k > 0 && (!d_B || !d_X))) return set_err(c, -1, "solve_multi: bad arguments");
+  if (k == 0) return 0;
+  HIPC(c, hipSetDevice(c->device));
+  return solve_multi_device(c, d_B, d_X, k, tol, maxits, maxtime, stag_test, flags, its_out,
+                            relres_out, stop_out);
+}
+
+int lapgpu_solve_multi(lapgpu_ctx* c, const double* B, double* X_out, int64_t n, int64_t k,
+                       double tol, int64_t maxits, double maxtime, int stag_test, int flags,
+                       int64_t* its_out, double* relres_out, int* stop_out) {
+  if (!c) return -1;
+  if (n != c->n) return set_err(c, -1, "solve_multi: number of rows of B differs from n");
+  if (k < 0 || (k > 0 && (!B || !X_out))) return set_err(c, -1, "solve_multi: bad arguments");
+  if (!c->has_F) return set_err(c, -2, "solve_multi: upload the Laplacian and LDLinv first");
+  if (k == 0) return 0;
+  HIPC(c, hipSetDevice(c->device));
+  // staging: one chunk of up to 64 columns at a time through m.io
+  for (int64_t c0 = 0; c0 < k; c0 += kMultiMax) {
+    const int64_t kc = std::min<int64_t>(kMultiMax, k - c0);
+    int K = 8;
+    while (K < kc) K *= 2;
+    int rc = multi_prepare(c, K);
+    if (rc) return rc;
+    HIPC(c, hipMemcpyAsync(c->m.io, B + c0 * n, sizeof(double) * n * kc, hipMemcpyHostToDevice,
+                           c->stream));
+    rc = solve_multi_device(c, c->m.io, c->m.io, kc, tol, maxits, maxtime, stag_test, flags,
+                            its_out ? its_out + c0 : nullptr,
+                            relres_out ? relres_out + c0 : nullptr,
+                            stop_out ? stop_out + c0 : nullptr);
+    if (rc) return rc;
+    HIPC(c, hipMemcpyAsync(X_out + c0 * n, c->m.io, sizeof(double) * n * kc,
+                           hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+  }
+  c->err.clear();
   return 0;
 }
 

@@ -13,6 +13,127 @@
 
 namespace lapgpu {
 // ---------------------------------------------------------------------------
+// The scalar logic of pcg (src/pcg.jl:256-430), one solve state at a time;
+// the single-RHS kernels run it in thread 0, the multi-RHS kernels in thread
+// j for right-hand side j.
+
+// rho of the initial residual (src/pcg.jl:277-283)
+__device__ inline void step_rho0(DevState* st, double zr) {
+  st->rho = zr;
+  st->rho0 = zr;
+  st->best_rho = zr;
+  st->stag_count = 0;
+  st->bestnr = 1.0;
+  st->nr = 1.0;
+  st->pending_copy = 0;
+  st->itcnt = 0;
+  st->t1 = wall_clock64();
+  if (st->maxits == 0) {
+    st->stopped = 1;
+    st->stop_reason = 1;
+  }
+}
+
+// itcnt += 1; al = rho / (p.q)   (src/pcg.jl:289-304)
+__device__ inline void step_al(DevState* st, double pq) {
+  st->itcnt += 1;
+  st->check10 = (st->itcnt % 10 == 0) ? 1 : 0;
+  st->al = st->rho / pq;
+}
+
+// the two al stagnation checks, nr, bestnr and convergence
+// (src/pcg.jl:308-361)
+__device__ inline void step_nr(DevState* st, double rr, double sp, double sx, double sq,
+                               double sr) {
+  st->pending_copy = 0;  // k_update has consumed it
+  if (st->check10) {
+    if (st->al * sp < kEps * sx) {
+      st->stopped = 1;
+      st->stop_reason = 4;
+      return;
+    }
+    if (st->al * sq < kEps * sr) {
+      st->stopped = 1;
+      st->stop_reason = 5;
+      return;
+    }
+  }
+  const double nr = sqrt(rr) / st->nb;
+  st->nr = nr;
+  st->bestnr_prev = st->bestnr;
+  if (nr < st->bestnr) {
+    st->bestnr = nr;
+    st->pending_copy = 1;
+  }
+  if (st->fp32 && !st->spmv64 && nr < st->switch_tol) {
+    // below switch_tol an FP32 SpMV cannot resolve the residual on badly
+    // scaled graphs: continue with the FP64 SpMV, starting from an FP64
+    // true residual
+    st->spmv64 = 1;
+    st->verify = 1;
+  }
+  if (nr < st->tol) {
+    if (st->fp32) {
+      st->verify = 1;  // confirmed by an FP64 true residual
+    } else {
+      st->stopped = 1;
+      st->stop_reason = 0;
+    }
+  }
+}
+
+// rho, the stag_test counter, the inf checks, beta and its stagnation check,
+// maxtime and maxits (src/pcg.jl:367-430)
+__device__ inline void step_rho(DevState* st, double zr, double sz, double sp) {
+  const double oldrho = st->rho;
+  const double rho = zr;
+  st->rho = rho;
+  if (rho < st->best_rho * st->stag_fac) {
+    st->best_rho = rho;
+    st->stag_count = 0;
+  } else if (st->stag_test > 0) {
+    if (st->best_rho > st->stag_fac * rho) {
+      st->stag_count += 1;
+      if (st->stag_count > st->stag_test) {
+        st->stopped = 1;
+        st->stop_reason = 3;
+        return;
+      }
+    }
+  }
+  if (isinf(rho)) {
+    st->stopped = 1;
+    st->stop_reason = 7;
+    return;
+  }
+  const double beta = rho / oldrho;
+  st->beta = beta;
+  if (isinf(beta)) {
+    st->stopped = 1;
+    st->stop_reason = 8;
+    return;
+  }
+  if (st->check10) {
+    if (beta * sp < kEps * sz) {
+      st->stopped = 1;
+      st->stop_reason = 6;
+      return;
+    }
+  }
+  // the reference tests maxtime after the p update; the update cannot
+  // change bestx, so testing it here returns the same result
+  if (st->has_maxtime && (wall_clock64() - st->t1) > st->maxticks) {
+    st->stopped = 1;
+    st->stop_reason = 2;
+    return;
+  }
+  if (st->maxits >= 0 && st->itcnt >= st->maxits) {
+    st->stopped = 1;
+    st->stop_reason = 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // permutation and setup
 
 __global__ void k_gather_perm(int n, const int* __restrict__ perm,
@@ -246,21 +367,7 @@ k_spmv_dot_mixed(int n, const int* __restrict__ Lptr, const int* __restrict__ Li
 __global__ void k_scalar_rho0(const double* __restrict__ part, DevState* st) {
   if (st->stopped) return;
   const double zr = wave_total(part, S_ZR);
-  if (threadIdx.x == 0) {
-    st->rho = zr;
-    st->rho0 = zr;
-    st->best_rho = zr;
-    st->stag_count = 0;
-    st->bestnr = 1.0;
-    st->nr = 1.0;
-    st->pending_copy = 0;
-    st->itcnt = 0;
-    st->t1 = wall_clock64();
-    if (st->maxits == 0) {
-      st->stopped = 1;
-      st->stop_reason = 1;
-    }
-  }
+  if (threadIdx.x == 0) step_rho0(st, zr);
 }
 
 __global__ void k_copy(int n, const double* __restrict__ src,
@@ -306,11 +413,7 @@ k_spmv_dot(int n, const int* __restrict__ Lptr, const int* __restrict__ Lidx,
 __global__ void k_scalar_al(const double* __restrict__ part, DevState* st) {
   if (st->stopped) return;
   const double pq = wave_total(part, S_PQ);
-  if (threadIdx.x == 0) {
-    st->itcnt += 1;
-    st->check10 = (st->itcnt % 10 == 0) ? 1 : 0;
-    st->al = st->rho / pq;
-  }
+  if (threadIdx.x == 0) step_al(st, pq);
 }
 
 // x += al p; r -= al q; bestx <- x (previous iterate) if it improved;
@@ -360,43 +463,7 @@ __global__ void k_scalar_nr(const double* __restrict__ part, DevState* st) {
   const double sx = wave_total(part, S_AX);
   const double sq = wave_total(part, S_AQ);
   const double sr = wave_total(part, S_AR);
-  if (threadIdx.x == 0) {
-    st->pending_copy = 0;  // k_update has consumed it
-    if (st->check10) {
-      if (st->al * sp < kEps * sx) {
-        st->stopped = 1;
-        st->stop_reason = 4;
-        return;
-      }
-      if (st->al * sq < kEps * sr) {
-        st->stopped = 1;
-        st->stop_reason = 5;
-        return;
-      }
-    }
-    const double nr = sqrt(rr) / st->nb;
-    st->nr = nr;
-    st->bestnr_prev = st->bestnr;
-    if (nr < st->bestnr) {
-      st->bestnr = nr;
-      st->pending_copy = 1;
-    }
-    if (st->fp32 && !st->spmv64 && nr < st->switch_tol) {
-      // below switch_tol an FP32 SpMV cannot resolve the residual on
-      // badly scaled graphs: continue with the FP64 SpMV, starting from an
-      // FP64 true residual
-      st->spmv64 = 1;
-      st->verify = 1;
-    }
-    if (nr < st->tol) {
-      if (st->fp32) {
-        st->verify = 1;  // confirmed by an FP64 true residual below
-      } else {
-        st->stopped = 1;
-        st->stop_reason = 0;
-      }
-    }
-  }
+  if (threadIdx.x == 0) step_nr(st, rr, sp, sx, sq, sr);
 }
 
 // FP32-inner mode: r = r0 - L x in FP64 (r0 = b .- mean(b)), partial r.r.
@@ -482,54 +549,7 @@ __global__ void k_scalar_rho(const double* __restrict__ part, DevState* st) {
   const double zr = wave_total(part, S_ZR);
   const double sz = wave_total(part, S_AZ);
   const double sp = wave_total(part, S_AP2);
-  if (threadIdx.x == 0) {
-    const double oldrho = st->rho;
-    const double rho = zr;
-    st->rho = rho;
-    if (rho < st->best_rho * st->stag_fac) {
-      st->best_rho = rho;
-      st->stag_count = 0;
-    } else if (st->stag_test > 0) {
-      if (st->best_rho > st->stag_fac * rho) {
-        st->stag_count += 1;
-        if (st->stag_count > st->stag_test) {
-          st->stopped = 1;
-          st->stop_reason = 3;
-          return;
-        }
-      }
-    }
-    if (isinf(rho)) {
-      st->stopped = 1;
-      st->stop_reason = 7;
-      return;
-    }
-    const double beta = rho / oldrho;
-    st->beta = beta;
-    if (isinf(beta)) {
-      st->stopped = 1;
-      st->stop_reason = 8;
-      return;
-    }
-    if (st->check10) {
-      if (beta * sp < kEps * sz) {
-        st->stopped = 1;
-        st->stop_reason = 6;
-        return;
-      }
-    }
-    // the reference tests maxtime after the p update; the update cannot
-    // change bestx, so testing it here returns the same result
-    if (st->has_maxtime && (wall_clock64() - st->t1) > st->maxticks) {
-      st->stopped = 1;
-      st->stop_reason = 2;
-      return;
-    }
-    if (st->maxits >= 0 && st->itcnt >= st->maxits) {
-      st->stopped = 1;
-      st->stop_reason = 1;
-    }
-  }
+  if (threadIdx.x == 0) step_rho(st, zr, sz, sp);
 }
 
 // p = z + beta p  (bzbeta!, src/pcg.jl:456-461)

@@ -1,0 +1,483 @@
+// lapgpu_multi.hpp — k right-hand sides of one graph solved together.
+//
+// Each right-hand side runs its own pcg (src/pcg.jl:256-445) with its own
+// scalars and stopping rules (state array mst[K], the same step_* logic as
+// the single-RHS path); they share every launch.  Vectors are interleaved,
+// element (position v, rhs j) at v*K + j with K a power of two in [8, 64]:
+// the sparse structure (indices, weights) is read once per row for all K
+// right-hand sides and every vector access is K contiguous doubles.
+//
+// flags[0] = 1 once every right-hand side has stopped (all kernels then
+// return at once); flags[1] = this iteration is a 10th one (the stagnation
+// checks need the 1-norm partials).  All running right-hand sides share the
+// iteration count, so one flag serves them all.
+// Included by lapgpu.hip only.
+#pragma once
+
+#include "lapgpu_kernels.hpp"
+
+namespace lapgpu {
+
+constexpr int kMRed = 1024;  // fixed grid of the multi-RHS reduction kernels
+
+// Block partial of NV sums per rhs j = threadIdx.x % K, written to
+// part[(slot * kMRed + block) * K + j].
+template <int K, int NV>
+__device__ __forceinline__ void mblock_sum(double (&a)[NV], double* __restrict__ part,
+                                           const int (&slots)[NV]) {
+  __shared__ double red[NV][kBlock / kWave][K];
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+#pragma unroll
+    for (int o = K; o < kWave; o <<= 1) a[i] += __shfl_xor(a[i], o, kWave);
+  }
+  if (lane < K) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) red[i][wid][lane] = a[i];
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int w = 0; w < kBlock / kWave; ++w) s += red[i][w][threadIdx.x];
+      part[((int64_t)slots[i] * kMRed + blockIdx.x) * K + threadIdx.x] = s;
+    }
+  }
+}
+
+// Total of one slot for rhs j = threadIdx.x, valid in threads < K of a
+// kScalarThreads block; fixed summation order.
+template <int K>
+__device__ __forceinline__ double mtotal(const double* __restrict__ part, int slot) {
+  constexpr int SUB = kScalarThreads / K;
+  __shared__ double red[SUB][K];
+  const int j = threadIdx.x % K, sub = threadIdx.x / K;
+  double s = 0.0;
+#pragma unroll 4
+  for (int b = sub; b < kMRed; b += SUB) s += part[((int64_t)slot * kMRed + b) * K + j];
+  __syncthreads();
+  red[sub][j] = s;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x < K) {
+#pragma unroll 4
+    for (int u = 0; u < SUB; ++u) t += red[u][threadIdx.x];
+  }
+  return t;
+}
+
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// sum over the entries [k0, k1) of a row of w[e] * src(idx[e]) for the K
+// lanes (one per rhs) of row group `sub` of a wave.  Indices and weights
+// are loaded K at a time, one per lane, and broadcast within the group, so
+// the gathers of one row are independent loads rather than a chain of
+// index -> value loads.  For K = 64 the row is wave-uniform (k0, k1 must be
+// uniform) and the broadcast is a readlane into scalar registers.
+template <int K>
+__device__ __forceinline__ void mbcast(int myi, double myw, int src, int& i, double& w) {
+  if constexpr (K == kWave) {
+    i = __builtin_amdgcn_readlane(myi, src);
+    w = readlane_d(myw, src);
+  } else {
+    i = __shfl(myi, src, kWave);
+    w = __shfl(myw, src, kWave);
+  }
+}
+
+template <int K, typename F>
+__device__ __forceinline__ double mrow_dot(int k0, int k1, int j, int sub,
+                                           const int* __restrict__ idx,
+                                           const double* __restrict__ w, F&& src) {
+  constexpr int U = K == kWave ? 8 : 4;  // gathers in flight per lane
+  double acc = 0.0;
+  for (int base = k0; base < k1; base += K) {
+    const int kk = base + j;
+    int myi = 0;
+    double myw = 0.0;
+    if (kk < k1) {
+      myi = idx[kk];
+      myw = w[kk];
+    }
+    const int cnt = min(K, k1 - base);
+    const int g0 = sub * K;
+    int e = 0;
+    for (; e + U <= cnt; e += U) {
+      int i[U];
+      double we[U], v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) mbcast<K>(myi, myw, g0 + e + u, i[u], we[u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = src(i[u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc += we[u] * v[u];
+    }
+    for (; e < cnt; ++e) {
+      int i;
+      double we;
+      mbcast<K>(myi, myw, g0 + e, i, we);
+      acc += we * src(i);
+    }
+  }
+  return acc;
+}
+
+template <int K>
+__device__ __forceinline__ int wave_uniform(int v) {
+  if constexpr (K == kWave) return __builtin_amdgcn_readfirstlane(v);
+  return v;
+}
+
+// after the per-rhs steps: flags[0] = every rhs stopped
+template <int K>
+__device__ __forceinline__ void mark_all(const DevState* mst, int* flags) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s = 1;
+    for (int j = 0; j < K; ++j) s &= (mst[j].stopped != 0) ? 1 : 0;
+    flags[0] = s;
+  }
+}
+
+// ---- setup -----------------------------------------------------------------
+
+// mb[v*K+j] = B[j*n + perm[v]] (zero for padding columns j >= k); sums
+template <int K>
+__global__ void __launch_bounds__(kBlock)
+k_mgather(int n, int k, const int* __restrict__ perm, const double* __restrict__ B,
+          double* __restrict__ mb, double* __restrict__ part) {
+  const int j = threadIdx.x % K;
+  const int64_t nK = (int64_t)n * K;
+  double a[1] = {0.0};
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nK;
+       e += (int64_t)gridDim.x * kBlock) {
+    const int v = (int)(e / K);
+    const double t = j < k ? B[(int64_t)j * n + perm[v]] : 0.0;
+    mb[e] = t;
+    a[0] += t;
+  }
+  const int slots[1] = {S_SUM};
+  mblock_sum<K, 1>(a, part, slots);
+}
+
+template <int K>
+__global__ void k_mscalar_mean(int n, const double* __restrict__ part, DevState* mst) {
+  const double s = mtotal<K>(part, S_SUM);
+  if (threadIdx.x < K) {
+    DevState* st = mst + threadIdx.x;
+    st->mu = st->center ? s / (double)n : 0.0;
+  }
+}
+
+// r = b - mean(b); x = bestx = 0; partial r.r
+template <int K>
+__global__ void __launch_bounds__(kBlock)
+k_mcenter(int n, const double* __restrict__ mb, double* __restrict__ r, double* __restrict__ x,
+          double* __restrict__ bestx, double* __restrict__ part,
+          const DevState* __restrict__ mst) {
+  const int j = threadIdx.x % K;
+  const double mu = mst[j].mu;
+  const int64_t nK = (int64_t)n * K;
+  double a[1] = {0.0};
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nK;
+       e += (int64_t)gridDim.x * kBlock) {
+    const double t = mb[e] - mu;
+    r[e] = t;
+    x[e] = 0.0;
+    bestx[e] = 0.0;
+    a[0] += t * t;
+  }
+  const int slots[1] = {S_RR};
+  mblock_sum<K, 1>(a, part, slots);
+}
+
+template <int K>
+__global__ void k_mscalar_nb(const double* __restrict__ part, DevState* mst, int* flags) {
+  const double rr = mtotal<K>(part, S_RR);
+  if (threadIdx.x < K) {
+    DevState* st = mst + threadIdx.x;
+    st->nb = sqrt(rr);
+    if (st->nb == 0.0) {
+      st->stopped = 1;
+      st->stop_reason = 9;  // zero rhs (and every padding column)
+    }
+  }
+  mark_all<K>(mst, flags);
+}
+
+// ---- the LDLinv apply --------------------------------------------------------
+
+// One level group (composite rows, as k_group), K lanes per row:
+//   out[v*K+j] = sum_e w[e] * (idx[e] < n ? A[idx[e]*K+j] : B[(idx[e]-n)*K+j])
+// forward: A = y0, B = r, out = y0; backward: A = yF, B = y0, out = yF.
+template <int K>
+__global__ void __launch_bounds__(kBlock)
+k_mgroup(int n, int lo, int hi, const int* __restrict__ ptr, const int* __restrict__ idx,
+         const double* __restrict__ w, const double* A, const double* B, double* out,
+         const int* __restrict__ flags) {
+  if (flags[0]) return;
+  constexpr int R = kWave / K;  // rows per wave
+  const int lane = threadIdx.x % kWave, j = lane % K, sub = lane / K;
+  const int row = lo + ((blockIdx.x * kBlock + threadIdx.x) / kWave) * R + sub;
+  if (row >= hi) return;  // whole row groups leave together
+  const int k0 = wave_uniform<K>(ptr[row]), k1 = wave_uniform<K>(ptr[row + 1]);
+  const double acc = mrow_dot<K>(k0, k1, j, sub, idx, w, [&](int i) {
+    return i < n ? A[(int64_t)i * K + j] : B[(int64_t)(i - n) * K + j];
+  });
+  out[(int64_t)row * K + j] = acc;
+}
+
+// ext[i*K+j] = r[(T0+i)*K+j] + the forward in-edges of tail row i from
+// outside the tail (as k_tail_ext), one lane per (row, rhs)
+template <int K>
+__global__ void __launch_bounds__(kBlock)
+k_mtail_ext(int T0, int NT, const int* __restrict__ Eptr, const int* __restrict__ Eidx,
+            const double* __restrict__ Ew, const double* __restrict__ r,
+            const double* __restrict__ y, double* __restrict__ ext,
+            const int* __restrict__ flags) {
+  if (flags[0]) return;
+  constexpr int R = kWave / K;
+  const int lane = threadIdx.x % kWave, j = lane % K, sub = lane / K;
+  const int row = ((blockIdx.x * kBlock + threadIdx.x) / kWave) * R + sub;
+  if (row >= NT) return;
+  const int k0 = wave_uniform<K>(Eptr[row]), k1 = wave_uniform<K>(Eptr[row + 1]);
+  const double acc =
+      mrow_dot<K>(k0, k1, j, sub, Eidx, Ew, [&](int i) { return y[(int64_t)i * K + j]; });
+  ext[(int64_t)row * K + j] = r[(int64_t)(T0 + row) * K + j] + acc;
+}
+
+// The tail for rhs j = blockIdx.x: the single-RHS slot programs on an
+// LDS-resident copy of column j (as k_tail); K workgroups run side by side
+// on K CUs.  Columns that have stopped are skipped.
+template <int K>
+__global__ void __launch_bounds__(kTailThreads)
+k_mtail(int T0, int NT, const double* __restrict__ ext, const double* __restrict__ s,
+        double* __restrict__ z, const TailStepD* __restrict__ fsteps, int nfsteps,
+        const TailSlotD* __restrict__ fslots, const TailStepD* __restrict__ bsteps,
+        int nbsteps, const TailSlotD* __restrict__ bslots, const DevState* __restrict__ mst,
+        const int* __restrict__ flags) {
+  __shared__ double ly[kTailCapV];
+  const int j = blockIdx.x;
+  if (flags[0] || mst[j].stopped) return;
+  for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] = ext[(int64_t)i * K + j];
+  __syncthreads();
+  run_program<kTailDepth, true>(fsteps, nfsteps, fslots, ly);
+  __syncthreads();
+  for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] *= s[T0 + i];
+  __syncthreads();
+  run_program<kTailDepth, true>(bsteps, nbsteps, bslots, ly);
+  __syncthreads();
+  for (int i = threadIdx.x; i < NT; i += kTailThreads) z[(int64_t)(T0 + i) * K + j] = ly[i];
+}
+
+// partial sums of yF (mean removal of LDLsolver)
+template <int K>
+__global__ void __launch_bounds__(kBlock)
+k_mvsum(int n, const double* __restrict__ z, double* __restrict__ part,
+        const int* __restrict__ flags) {
+  if (flags[0]) return;
+  const int64_t nK = (int64_t)n * K;
+  double a[1] = {0.0};
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nK;
+       e += (int64_t)gridDim.x * kBlock)
+    a[0] += z[e];
+  const int slots[1] = {S_SUM};
+  mblock_sum<K, 1>(a, part, slots);
+}
+
+template <int K>
+__global__ void k_mscalar_mu(int n, const double* __restrict__ part, DevState* mst,
+                             const int* __restrict__ flags) {
+  if (flags[0]) return;
+  const double s = mtotal<K>(part, S_SUM);
+  if (threadIdx.x < K) mst[threadIdx.x].mu = s / (double)n;
+}
+
+// partials (yF - mu).r and, on 10th iterations, sum|yF - mu|, sum|p|
+template <int K>
+__global__ void __launch_bounds__(kBlock)
+k_mzdot(int n, const double* __restrict__ z, const double* __restrict__ r,
+        const double* __restrict__ p, double* __restrict__ part,
+        const DevState* __restrict__ mst, const int* __restrict__ flags) {
+  if (flags[0]) return;
+  const bool chk = flags[1] != 0;
+  const int j = threadIdx.x % K;
+  const double mu = mst[j].mu;
+  const int64_t nK = (int64_t)n * K;
+  double a[3] = {0.0, 0.0, 0.0};
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nK;
+       e += (int64_t)gridDim.x * kBlock) {
+    const double t = z[e] - mu;
+    a[0] += t * r[e];
+    if (chk) {
+      a[1] += fabs(t);
+      a[2] += fabs(p[e]);
+    }
+  }
+  const int slots[3] = {S_ZR, S_AZ, S_AP2};
+  mblock_sum<K, 3>(a, part, slots);
+}
+
+template <int K>
+__global__ void k_mscalar_rho0(const double* __restrict__ part, DevState* mst, int* flags) {
+  if (flags[0]) return;
+  const double zr = mtotal<K>(part, S_ZR);
+  if (threadIdx.x < K && !mst[threadIdx.x].stopped) step_rho0(mst + threadIdx.x, zr);
+  mark_all<K>(mst, flags);
+}
+
+// p = yF - mu after the setup's preconditioner
+template <int K>
+__global__ void __launch_bounds__(kBlock)
+k_mpinit(int n, const double* __restrict__ z, double* __restrict__ p,
+         const DevState* __restrict__ mst, const int* __restrict__ flags) {
+  if (flags[0]) return;
+  const int j = threadIdx.x % K;
+  const double mu = mst[j].mu;
+  const int64_t nK = (int64_t)n * K;
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nK;
+       e += (int64_t)gridDim.x * kBlock)
+    p[e] = z[e] - mu;
+}
+
+// ---- one PCG iteration ---------------------------------------------------------
+
+// q = la * p, partial p.q; K lanes per row, one per rhs
+template <int K>
+__global__ void __launch_bounds__(kBlock)
+k_mspmv(int n, const int* __restrict__ Lptr, const int* __restrict__ Lidx,
+        const double* __restrict__ Lval, const double* __restrict__ p, double* __restrict__ q,
+        double* __restrict__ part, const int* __restrict__ flags) {
+  if (flags[0]) return;
+  constexpr int R = kWave / K;  // rows per wave
+  const int lane = threadIdx.x % kWave, j = lane % K, sub = lane / K;
+  const int wave = (blockIdx.x * kBlock + threadIdx.x) / kWave;
+  const int nwaves = gridDim.x * (kBlock / kWave);
+  double a[1] = {0.0};
+  for (int row = wave * R + sub; row < n; row += nwaves * R) {
+    const int k0 = wave_uniform<K>(Lptr[row]), k1 = wave_uniform<K>(Lptr[row + 1]);
+    const double acc =
+        mrow_dot<K>(k0, k1, j, sub, Lidx, Lval, [&](int i) { return p[(int64_t)i * K + j]; });
+    q[(int64_t)row * K + j] = acc;
+    a[0] += p[(int64_t)row * K + j] * acc;
+  }
+  const int slots[1] = {S_PQ};
+  mblock_sum<K, 1>(a, part, slots);
+}
+
+// itcnt += 1; al = rho / (p.q); flags[1] = 10th iteration
+template <int K>
+__global__ void k_mscalar_al(const double* __restrict__ part, DevState* mst, int* flags) {
+  if (flags[0]) return;
+  const double pq = mtotal<K>(part, S_PQ);
+  if (threadIdx.x < K && !mst[threadIdx.x].stopped) step_al(mst + threadIdx.x, pq);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int c = 0;
+    for (int j = 0; j < K; ++j)
+      if (!mst[j].stopped) c |= mst[j].check10;
+    flags[1] = c;
+  }
+}
+
+// x += al p; r -= al q; bestx <- x where the previous iterate improved;
+// partials r.r and (10th iterations) sum|p|, sum|x|, sum|q|, sum|r| of the
+// values before the update (src/pcg.jl:308-354).  Stopped rhs are frozen.
+template <int K>
+__global__ void __launch_bounds__(kBlock)
+k_mupdate(int n, double* __restrict__ x, double* __restrict__ bestx,
+          const double* __restrict__ p, const double* __restrict__ q, double* __restrict__ r,
+          double* __restrict__ part, const DevState* __restrict__ mst,
+          const int* __restrict__ flags) {
+  if (flags[0]) return;
+  const bool chk = flags[1] != 0;
+  const int j = threadIdx.x % K;
+  const bool run = mst[j].stopped == 0;
+  const bool cp = mst[j].pending_copy != 0;
+  const double al = mst[j].al;
+  const int64_t nK = (int64_t)n * K;
+  double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (run) {
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nK;
+         e += (int64_t)gridDim.x * kBlock) {
+      const double xo = x[e], pi = p[e], qi = q[e], ro = r[e];
+      if (cp) bestx[e] = xo;
+      x[e] = xo + al * pi;
+      const double rn = ro + (-al) * qi;
+      r[e] = rn;
+      a[0] += rn * rn;
+      if (chk) {
+        a[1] += fabs(pi);
+        a[2] += fabs(xo);
+        a[3] += fabs(qi);
+        a[4] += fabs(ro);
+      }
+    }
+  }
+  const int slots[5] = {S_RR, S_AP, S_AX, S_AQ, S_AR};
+  mblock_sum<K, 5>(a, part, slots);
+}
+
+template <int K>
+__global__ void k_mscalar_nr(const double* __restrict__ part, DevState* mst, int* flags) {
+  if (flags[0]) return;
+  const double rr = mtotal<K>(part, S_RR);
+  const double sp = mtotal<K>(part, S_AP);
+  const double sx = mtotal<K>(part, S_AX);
+  const double sq = mtotal<K>(part, S_AQ);
+  const double sr = mtotal<K>(part, S_AR);
+  if (threadIdx.x < K && !mst[threadIdx.x].stopped)
+    step_nr(mst + threadIdx.x, rr, sp, sx, sq, sr);
+  mark_all<K>(mst, flags);
+}
+
+template <int K>
+__global__ void k_mscalar_rho(const double* __restrict__ part, DevState* mst, int* flags) {
+  if (flags[0]) return;
+  const double zr = mtotal<K>(part, S_ZR);
+  const double sz = mtotal<K>(part, S_AZ);
+  const double sp = mtotal<K>(part, S_AP2);
+  if (threadIdx.x < K && !mst[threadIdx.x].stopped) step_rho(mst + threadIdx.x, zr, sz, sp);
+  mark_all<K>(mst, flags);
+}
+
+// p = (yF - mu) + beta p for the right-hand sides still running
+template <int K>
+__global__ void __launch_bounds__(kBlock)
+k_mpupdate(int n, const double* __restrict__ z, double* __restrict__ p,
+           const DevState* __restrict__ mst, const int* __restrict__ flags) {
+  if (flags[0]) return;
+  const int j = threadIdx.x % K;
+  if (mst[j].stopped) return;
+  const double mu = mst[j].mu, beta = mst[j].beta;
+  const int64_t nK = (int64_t)n * K;
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nK;
+       e += (int64_t)gridDim.x * kBlock)
+    p[e] = (z[e] - mu) + beta * p[e];
+}
+
+// X[j*n + perm[v]] = best iterate (x where the last iteration improved)
+template <int K>
+__global__ void __launch_bounds__(kBlock)
+k_mfinal(int n, int k, const int* __restrict__ perm, const double* __restrict__ x,
+         const double* __restrict__ bestx, double* __restrict__ X,
+         const DevState* __restrict__ mst) {
+  const int j = threadIdx.x % K;
+  if (j >= k) return;
+  const bool cp = mst[j].pending_copy != 0;
+  const double* src = cp ? x : bestx;
+  const int64_t nK = (int64_t)n * K;
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nK;
+       e += (int64_t)gridDim.x * kBlock)
+    X[(int64_t)j * n + perm[(int)(e / K)]] = src[e];
+}
+
+}  // namespace lapgpu

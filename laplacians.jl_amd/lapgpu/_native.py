@@ -40,6 +40,8 @@ EXPORTS = (
     "lapgpu_upload_ldli",
     "lapgpu_solve",
     "lapgpu_solve_device",
+    "lapgpu_solve_multi",
+    "lapgpu_solve_multi_device",
     "lapgpu_apply_ldli",
     "lapgpu_spmv",
     "lapgpu_set_precision",
@@ -84,6 +86,8 @@ def load() -> ctypes.CDLL:
     L.lapgpu_upload_ldli.argtypes = [P, i64, i64, P, P, P, P, P, ci]
     L.lapgpu_solve.argtypes = [P, P, P, i64, dbl, i64, dbl, ci, ci, pi64, pdbl, pint]
     L.lapgpu_solve_device.argtypes = [P, P, P, i64, dbl, i64, dbl, ci, ci, pi64, pdbl, pint]
+    L.lapgpu_solve_multi.argtypes = [P, P, P, i64, i64, dbl, i64, dbl, ci, ci, P, P, P]
+    L.lapgpu_solve_multi_device.argtypes = [P, P, P, i64, i64, dbl, i64, dbl, ci, ci, P, P, P]
     L.lapgpu_apply_ldli.argtypes = [P, P, P, i64]
     L.lapgpu_spmv.argtypes = [P, P, P, i64]
     L.lapgpu_set_precision.argtypes = [P, ci, ci]
@@ -118,6 +122,89 @@ def load() -> ctypes.CDLL:
 
 def _p(a: np.ndarray):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+_HIP = None
+
+
+def _hip() -> ctypes.CDLL:
+    """The HIP runtime liblapgpu.so links (the same soname resolves to the
+    already loaded copy), for device buffers owned by that runtime."""
+    global _HIP
+    if _HIP is None:
+        load()
+        try:
+            H = ctypes.CDLL("libamdhip64.so.7")
+        except OSError:
+            H = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so.7")
+        H.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        H.hipFree.argtypes = [ctypes.c_void_p]
+        H.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        H.hipSetDevice.argtypes = [ctypes.c_int]
+        for f in (H.hipMalloc, H.hipFree, H.hipMemcpy, H.hipSetDevice, H.hipDeviceSynchronize):
+            f.restype = ctypes.c_int
+        _HIP = H
+    return _HIP
+
+
+class DeviceArray:
+    """A float64 array in device memory (HBM) allocated by liblapgpu's own HIP
+    runtime: the resident inputs and outputs of the *_device entry points.
+
+    torch bundles a second HIP runtime; device pointers of either work with
+    the library, but a process that initialises liblapgpu's runtime first
+    may not get a device from torch's afterwards, so tests and the benchmark
+    use this class."""
+
+    def __init__(self, count: int, device: int = 0):
+        H = _hip()
+        self.count = int(count)
+        self.device = device
+        self.ptr = ctypes.c_void_p()
+        if H.hipSetDevice(device) != 0 or H.hipMalloc(ctypes.byref(self.ptr), max(8 * self.count, 8)) != 0:
+            raise LapGPUError(f"hipMalloc of {self.count} doubles failed")
+
+    @classmethod
+    def from_host(cls, a: np.ndarray, device: int = 0) -> "DeviceArray":
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        d = cls(a.size, device)
+        d.upload(a)
+        return d
+
+    @property
+    def address(self) -> int:
+        return int(self.ptr.value or 0)
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.size != self.count:
+            raise ValueError("DeviceArray.upload: size mismatch")
+        if _hip().hipMemcpy(self.ptr, _p(a), 8 * self.count, 1) != 0:
+            raise LapGPUError("hipMemcpy host to device failed")
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.count, dtype=np.float64)
+        if _hip().hipMemcpy(_p(out), self.ptr, 8 * self.count, 2) != 0:
+            raise LapGPUError("hipMemcpy device to host failed")
+        return out
+
+    def free(self) -> None:
+        if self.ptr.value:
+            _hip().hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize(device: int = 0) -> None:
+    H = _hip()
+    H.hipSetDevice(device)
+    if H.hipDeviceSynchronize() != 0:
+        raise LapGPUError("hipDeviceSynchronize failed")
 
 
 class LapGPUError(RuntimeError):
@@ -198,6 +285,39 @@ class Context:
             "lapgpu_solve_device",
         )
         return {"its": its.value, "nr": nr.value, "stop": STOP_NAMES.get(stop.value, str(stop.value))}
+
+    def solve_multi(self, B, tol=1e-6, maxits=1000, maxtime=float("inf"), stag_test=5, flags=0):
+        """Solve for every column of the n-by-k matrix B (each column its own
+        pcg, all sharing the launches).  Returns X (n-by-k) and per-column
+        its / nr / stop lists."""
+        B = np.asfortranarray(B, dtype=np.float64)
+        if B.ndim != 2:
+            raise ValueError("solve_multi: B must be n-by-k")
+        n, k = B.shape
+        X = np.empty((n, k), dtype=np.float64, order="F")
+        its = np.zeros(k, dtype=np.int64)
+        nr = np.zeros(k, dtype=np.float64)
+        stop = np.zeros(k, dtype=np.int32)
+        self._check(
+            self._lib.lapgpu_solve_multi(self._h, _p(B), _p(X), n, k, float(tol), int(maxits), float(maxtime),
+                                         int(stag_test), int(flags), _p(its), _p(nr), _p(stop)),
+            "lapgpu_solve_multi",
+        )
+        return X, {"its": its.tolist(), "nr": nr.tolist(), "stop": [STOP_NAMES.get(int(s), str(s)) for s in stop]}
+
+    def solve_multi_device(self, d_B: int, d_X: int, n: int, k: int, tol=1e-6, maxits=1000,
+                           maxtime=float("inf"), stag_test=5, flags=0):
+        """Multi-RHS solve with device pointers to column-major n-by-k B and X."""
+        its = np.zeros(k, dtype=np.int64)
+        nr = np.zeros(k, dtype=np.float64)
+        stop = np.zeros(k, dtype=np.int32)
+        self._check(
+            self._lib.lapgpu_solve_multi_device(self._h, ctypes.c_void_p(d_B), ctypes.c_void_p(d_X), n, k,
+                                                float(tol), int(maxits), float(maxtime), int(stag_test),
+                                                int(flags), _p(its), _p(nr), _p(stop)),
+            "lapgpu_solve_multi_device",
+        )
+        return {"its": its.tolist(), "nr": nr.tolist(), "stop": [STOP_NAMES.get(int(s), str(s)) for s in stop]}
 
     def apply_ldli(self, r):
         r = np.ascontiguousarray(r, dtype=np.float64)

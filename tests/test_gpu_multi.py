@@ -1,0 +1,113 @@
+"""Many right-hand sides of one graph in one solve (lapgpu_solve_multi).
+
+Each column must be solved as the closure would solve it alone: against the
+CPU oracle's pcg with the same LDLinv, the same stop reason, iteration count
+within +-2, true relres <= 10 tol and the solution within 1e-4 relative (the
+bars of the single-RHS parity tests).  Columns that stop early (zero column,
+maxits) must not disturb the others.
+"""
+
+import numpy as np
+import pytest
+
+import lapgpu
+from lapgpu import graphs
+
+pytestmark = pytest.mark.gpu
+
+
+def _rhs_block(n, k, seed):
+    B = np.random.default_rng(seed).standard_normal((n, k))
+    return B - B.mean(axis=0)
+
+
+CASES = {
+    "grid2_60x50": lambda: graphs.weighted_grid2(60, 50, np.random.default_rng(11), "expN"),
+    "grid3_16_w": lambda: graphs.weighted_grid3(16, 16, 16, np.random.default_rng(2)),
+    "pathmatch_5000": lambda: graphs.path_matchings(5000, 4, np.random.default_rng(3)),
+}
+
+
+@pytest.fixture(scope="module", params=sorted(CASES))
+def case(request):
+    a = CASES[request.param]()
+    la = lapgpu.lap(a)
+    f = lapgpu.build_ldli(a, seed=1)
+    return request.param, a, la, f
+
+
+@pytest.mark.parametrize("k", [1, 5, 16, 40, 70])
+def test_multi_matches_oracle(case, gpu_ctx_factory, oracle_mod, k):
+    name, a, la, f = case
+    ctx = gpu_ctx_factory(la, f)
+    n = la.shape[0]
+    B = _rhs_block(n, k, 100 + k)
+    X, info = ctx.solve_multi(B, tol=1e-8, maxits=1000, stag_test=5)
+    assert X.shape == (n, k)
+    # every column against the oracle for small k, a spread of columns else
+    cols = range(k) if k <= 5 else sorted({0, 1, k // 2, k - 2, k - 1})
+    for j in cols:
+        xo, io = oracle_mod.approxchol_solve(la, f, B[:, j], tol=1e-8, maxits=1000, stag_test=5)
+        assert info["stop"][j] == io["stop"], (name, j, info["stop"][j], io)
+        assert abs(info["its"][j] - io["its"]) <= 2, (name, j, info["its"][j], io)
+        assert np.linalg.norm(X[:, j] - xo) <= 1e-4 * np.linalg.norm(xo), (name, j)
+    relres = np.linalg.norm(la @ X - B, axis=0) / np.linalg.norm(B, axis=0)
+    assert np.all(relres <= 1e-7), (name, relres.max())
+
+
+def test_multi_equals_single(case, gpu_ctx_factory):
+    """The multi path and the single-RHS path agree column by column."""
+    name, a, la, f = case
+    ctx = gpu_ctx_factory(la, f)
+    n = la.shape[0]
+    B = _rhs_block(n, 8, 7)
+    X, info = ctx.solve_multi(B, tol=1e-8, maxits=1000)
+    for j in range(8):
+        x1, i1 = ctx.solve(B[:, j], tol=1e-8, maxits=1000)
+        assert abs(i1["its"] - info["its"][j]) <= 2, (name, j)
+        assert np.linalg.norm(X[:, j] - x1) <= 1e-6 * np.linalg.norm(x1), (name, j)
+
+
+def test_multi_mixed_stops(gpu_ctx_factory, oracle_mod):
+    """A zero column, a tiny-maxits run and normal columns side by side."""
+    a = graphs.weighted_grid2(40, 40, np.random.default_rng(3), "expN")
+    la = lapgpu.lap(a)
+    f = lapgpu.build_ldli(a, seed=1)
+    ctx = gpu_ctx_factory(la, f)
+    n = la.shape[0]
+    B = _rhs_block(n, 6, 9)
+    B[:, 2] = 0.0
+    X, info = ctx.solve_multi(B, tol=1e-10, maxits=1000)
+    assert info["stop"][2] == "zero_rhs" and info["its"][2] == 0
+    assert np.all(X[:, 2] == 0.0)
+    for j in (0, 1, 3, 4, 5):
+        xo, io = oracle_mod.approxchol_solve(la, f, B[:, j], tol=1e-10, maxits=1000, stag_test=5)
+        assert info["stop"][j] == io["stop"]
+        assert abs(info["its"][j] - io["its"]) <= 2
+    # maxits = 3 for all: everyone stops at 3 (zero column at 0)
+    X3, i3 = ctx.solve_multi(B, tol=1e-10, maxits=3)
+    for j in (0, 1, 3, 4, 5):
+        xo, io = oracle_mod.approxchol_solve(la, f, B[:, j], tol=1e-10, maxits=3, stag_test=5)
+        assert i3["its"][j] == 3 and i3["stop"][j] == "maxits" == io["stop"]
+        assert np.linalg.norm(X3[:, j] - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
+def test_multi_device_pointers(gpu_ctx_factory):
+    """B and X resident in device memory (column-major n-by-k)."""
+    from lapgpu._native import DeviceArray
+
+    a = graphs.grid2(50)
+    la = lapgpu.lap(a)
+    f = lapgpu.build_ldli(a, seed=1)
+    ctx = gpu_ctx_factory(la, f)
+    n = la.shape[0]
+    B = _rhs_block(n, 12, 4)
+    dB = DeviceArray.from_host(B.T)  # row j of B.T = column j of B
+    dX = DeviceArray(n * 12)
+    info = ctx.solve_multi_device(dB.address, dX.address, n, 12, tol=1e-8, maxits=500)
+    X = dX.download().reshape(12, n).T
+    Xh, ih = ctx.solve_multi(B, tol=1e-8, maxits=500)
+    assert info["its"] == ih["its"]
+    assert np.array_equal(X, Xh)
+    dB.free()
+    dX.free()
