@@ -106,8 +106,8 @@ int lapgpu_solve_device(lapgpu_ctx* ctx, const double* d_b, double* d_x,
  * stop_reason_out[j]; arrays of k, each may be NULL), all columns sharing
  * every launch.  The reference solves one vector per call of the closure
  * (src/approxChol.jl:1624); this is the same closure applied column by
- * column.  Columns are processed 64 at a time, always in FP64 (the
- * precision mode applies to lapgpu_solve). */
+ * column.  Columns are processed 64 at a time, in the context's precision
+ * mode (lapgpu_set_precision), as lapgpu_solve. */
 int lapgpu_solve_multi(lapgpu_ctx* ctx, const double* B, double* X_out, int64_t n,
                        int64_t k, double tol, int64_t maxits, double maxtime,
                        int stag_test, int flags, int64_t* its_out,
@@ -141,6 +141,19 @@ int lapgpu_factor_info(lapgpu_ctx* ctx, int64_t* nlevels, int64_t* nwide,
 #define LAPGPU_PROFILE 4
 int lapgpu_last_timings(lapgpu_ctx* ctx, double* spmv_ms, double* ldl_ms,
                         double* vec_ms, double* total_ms, int64_t* its);
+
+/* Size of the single-workgroup tail of the uploaded factor: tail vertices,
+ * program steps and slots of both sweeps, and forward in-edges from outside
+ * the tail.  Measurement only (algorithmic bytes of the tail kernel). */
+int lapgpu_tail_info(lapgpu_ctx* ctx, int64_t* nt, int64_t* nsteps, int64_t* nslots,
+                     int64_t* nedges_in);
+
+/* The LDLinv apply of the last LAPGPU_PROFILE single-RHS solve split by
+ * kernel (ms per iteration): forward level groups, the tail's inputs
+ * (k_tail_ext), the tail (k_tail) and backward level groups.  Measurement
+ * only; no reference counterpart. */
+int lapgpu_last_sweep_timings(lapgpu_ctx* ctx, double* fgroups_ms, double* tail_ext_ms,
+                              double* tail_ms, double* bgroups_ms);
 
 const char* lapgpu_last_error(lapgpu_ctx* ctx);
 void lapgpu_destroy(lapgpu_ctx* ctx);

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "lapgpu.h"
+#include "lapgpu_lock.hpp"
 #include "lapgpu_multi.hpp"
 #include "lapgpu_plan.hpp"
 
@@ -47,7 +48,7 @@ struct lapgpu_ctx {
   int64_t max_tail_steps = 24;
   int G_spmv = 8, G_ldl = 8;
   int precision = LAPGPU_FP64, replace_every = 20;
-  double switch_tol = 1e-5;  // FP32 mode: residual below which the SpMV is FP64
+  double switch_tol = 1e-4;  // FP32 mode: residual below which the SpMV is FP64
 
   // device graph
   int *perm = nullptr, *Lptr = nullptr, *Lidx = nullptr;
@@ -99,19 +100,28 @@ struct lapgpu_ctx {
   // sides of length n; the tail CSRs are built on first use
   struct Multi {
     int64_t cap = 0;  // elements of each vector (n * K)
-    int exec_K = 0;   // K of the captured iteration
+    int exec_K = 0;   // K of the captured iterations
     double *r = nullptr, *x = nullptr, *bestx = nullptr, *p = nullptr, *q = nullptr,
            *y = nullptr, *z = nullptr, *io = nullptr, *part = nullptr;
     DevState* st = nullptr;
     DevState* hst = nullptr;  // pinned, 64 states
     int* flags = nullptr;
     double* ext = nullptr;  // tail inputs, NT * K
-    hipGraphExec_t exec = nullptr;
+    // FP32-inner mode: sweep buffers, the FP32 copy of p, r0 = b - mean(b)
+    int64_t cap32 = 0;
+    float *y32 = nullptr, *z32 = nullptr, *pf = nullptr, *ext32 = nullptr;
+    double* r0 = nullptr;
+    // captured iteration (exec[1]: FP32 mode with a residual replacement)
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
   } m;
 
-  // profiling
+  // profiling (LAPGPU_PROFILE): events around the kernel classes of one
+  // iteration; ev[5..7] split the sweeps into forward groups, tail inputs,
+  // tail and backward groups
   hipEvent_t ev[8] = {};
+  bool prof_sweeps = false;
   double t_spmv = 0, t_ldl = 0, t_vec = 0, t_total = 0;
+  double t_fgroups = 0, t_tail_ext = 0, t_tail = 0, t_bgroups = 0;
   int64_t t_its = 0;
 };
 
@@ -135,6 +145,7 @@ int set_err(lapgpu_ctx* c, int code, const std::string& msg) {
 
 template <typename T>
 int dev_upload(lapgpu_ctx* c, T** dst, const std::vector<T>& src) {
+  LAPGPU_LOCK();
   if (*dst) {
     HIPC(c, hipFree(*dst));
     *dst = nullptr;
@@ -149,6 +160,7 @@ int dev_upload(lapgpu_ctx* c, T** dst, const std::vector<T>& src) {
 
 template <typename T>
 int dev_alloc(lapgpu_ctx* c, T** dst, size_t count) {
+  LAPGPU_LOCK();
   if (*dst) {
     HIPC(c, hipFree(*dst));
     *dst = nullptr;
@@ -223,6 +235,7 @@ void launch_sweeps(lapgpu_ctx* c, const double* rin, T* y, T* z, const T* gFw, c
                       c->gFidx, gFw, y, rin, y);
   }
   const TailProgram& TP = P.tail;
+  if (c->prof_sweeps) (void)hipEventRecord(c->ev[5], c->stream);
   if (TP.NT > 0) {
     const int G = c->G_ldl;
     if (G <= 8)
@@ -233,10 +246,14 @@ void launch_sweeps(lapgpu_ctx* c, const double* rin, T* y, T* z, const T* gFw, c
       hipLaunchKernelGGL((k_tail_ext<32, T>), dim3(grid_for(TP.NT * 32, kBlock)), dim3(kBlock),
                          0, c->stream, (int)TP.T0, (int)TP.NT, c->Eptr, c->Eidx, Ew, rin, y, ext,
                          c->st);
+    if (c->prof_sweeps) (void)hipEventRecord(c->ev[6], c->stream);
     hipLaunchKernelGGL((k_tail<T, S>), dim3(1), dim3(kTailThreads), 0, c->stream, (int)TP.T0,
                        (int)TP.NT, ext, sv, z, c->fsteps, (int)TP.fsteps.size(), fslots,
                        c->bsteps, (int)TP.bsteps.size(), bslots, c->st);
+  } else if (c->prof_sweeps) {
+    (void)hipEventRecord(c->ev[6], c->stream);
   }
+  if (c->prof_sweeps) (void)hipEventRecord(c->ev[7], c->stream);
   for (auto it = c->groups.rbegin(); it != c->groups.rend(); ++it) {
     const auto& g = *it;
     const int* hv = c->heavy + g.hoff_b;
@@ -289,17 +306,21 @@ void drop_graphs(lapgpu_ctx* c) {
   if (c->iter_exec) (void)hipGraphExecDestroy(c->iter_exec);
   if (c->iter_exec_rep) (void)hipGraphExecDestroy(c->iter_exec_rep);
   c->iter_exec = c->iter_exec_rep = nullptr;
-  if (c->m.exec) (void)hipGraphExecDestroy(c->m.exec);
-  c->m.exec = nullptr;
+  for (auto& e : c->m.exec) {
+    if (e) (void)hipGraphExecDestroy(e);
+    e = nullptr;
+  }
 }
 
 // the multi-RHS workspace and tail CSRs (rebuilt on the next multi solve)
 void free_multi(lapgpu_ctx* c) {
+  LAPGPU_LOCK();
   auto& m = c->m;
-  if (m.exec) (void)hipGraphExecDestroy(m.exec);
-  m.exec = nullptr;
-  double* dp[] = {m.r, m.x, m.bestx, m.p, m.q, m.y, m.z, m.io, m.part, m.ext};
-  for (double* a : dp)
+  for (auto& e : m.exec)
+    if (e) (void)hipGraphExecDestroy(e);
+  void* dp[] = {m.r, m.x, m.bestx, m.p, m.q, m.y, m.z, m.io, m.part, m.ext,
+                m.y32, m.z32, m.pf, m.ext32, m.r0};
+  for (void* a : dp)
     if (a) (void)hipFree(a);
   if (m.flags) (void)hipFree(m.flags);
   if (m.st) (void)hipFree(m.st);
@@ -308,6 +329,7 @@ void free_multi(lapgpu_ctx* c) {
 }
 
 void free_f32(lapgpu_ctx* c) {
+  LAPGPU_LOCK();
   void* fp[] = {c->Lval32, c->gFw32, c->gBw32, c->s32, c->Ew32, c->y32, c->z32, c->ext32, c->pf,
                 c->fslots32, c->bslots32, c->r0};
   for (void* a : fp)
@@ -319,6 +341,7 @@ void free_f32(lapgpu_ctx* c) {
 }
 
 int free_graph(lapgpu_ctx* c) {
+  LAPGPU_LOCK();
   drop_graphs(c);
   free_f32(c);
   free_multi(c);
@@ -552,7 +575,9 @@ void enqueue_iteration(lapgpu_ctx* c, bool prof, bool replace) {
                        replace ? 1 : 0, c->part, c->st);
   }
   if (prof) (void)hipEventRecord(c->ev[2], c->stream);
+  c->prof_sweeps = prof;
   sweeps(c, c->r);
+  c->prof_sweeps = false;
   if (prof) (void)hipEventRecord(c->ev[3], c->stream);
   finish_precond(c, c->r);
   hipLaunchKernelGGL(k_scalar_rho, dim3(1), dim3(kScalarThreads), 0, c->stream, c->part, c->st);
@@ -605,7 +630,8 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
   }
 
   // ---- iterations: poll the stop flag one iteration behind ---------------
-  double acc_spmv = 0, acc_ldl = 0, acc_vec = 0;
+  double acc_spmv = 0, acc_ldl = 0, acc_vec = 0, acc_fg = 0, acc_ext = 0, acc_tail = 0,
+         acc_bg = 0;
   int64_t enq = 0, polled = 0;
   bool done = c->hst->stopped != 0;
   const int rep_every = f32 ? c->replace_every : 0;
@@ -613,6 +639,7 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
     for (int v = 0; v < (f32 ? 2 : 1); ++v) {
       hipGraphExec_t* ex = v ? &c->iter_exec_rep : &c->iter_exec;
       if (*ex) continue;
+      LAPGPU_LOCK();
       hipGraph_t g = nullptr;
       HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       enqueue_iteration(c, false, v == 1);
@@ -645,6 +672,15 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
       acc_spmv += a;
       acc_ldl += b2;
       acc_vec += d + e;
+      float f1 = 0, f2 = 0, f3 = 0, f4 = 0;
+      (void)hipEventElapsedTime(&f1, c->ev[2], c->ev[5]);
+      (void)hipEventElapsedTime(&f2, c->ev[5], c->ev[6]);
+      (void)hipEventElapsedTime(&f3, c->ev[6], c->ev[7]);
+      (void)hipEventElapsedTime(&f4, c->ev[7], c->ev[3]);
+      acc_fg += f1;
+      acc_ext += f2;
+      acc_tail += f3;
+      acc_bg += f4;
     }
     // keep one iteration in flight: wait for the flags of all older ones
     while (!done && polled < enq - 1) {
@@ -671,6 +707,10 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
     c->t_spmv = acc_spmv / (double)enq;
     c->t_ldl = acc_ldl / (double)enq;
     c->t_vec = acc_vec / (double)enq;
+    c->t_fgroups = acc_fg / (double)enq;
+    c->t_tail_ext = acc_ext / (double)enq;
+    c->t_tail = acc_tail / (double)enq;
+    c->t_bgroups = acc_bg / (double)enq;
   }
   c->t_total = secs * 1e3;
   c->t_its = h.itcnt;
@@ -706,77 +746,140 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
 
 constexpr int kMultiMax = 64;
 
-// Workspace for n * K elements per vector (kept while it is large enough).
+// Workspace for n * K elements per vector (kept while it is large enough);
+// FP32 buffers only in the FP32-inner mode.
 int multi_prepare(lapgpu_ctx* c, int K) {
+  LAPGPU_LOCK();
   auto& m = c->m;
   const int64_t n = c->n;
+  const int64_t NT = std::max<int64_t>(c->plan.tail.NT, 1);
   int rc;
   if (!m.part) {
     if ((rc = dev_alloc(c, &m.part, (size_t)kNumSlots * kMRed * kMultiMax))) return rc;
     if ((rc = dev_alloc(c, &m.st, (size_t)kMultiMax))) return rc;
-    if ((rc = dev_alloc(c, &m.flags, 2))) return rc;
+    if ((rc = dev_alloc(c, &m.flags, 4))) return rc;
     HIPC(c, hipHostMalloc((void**)&m.hst, sizeof(DevState) * kMultiMax, hipHostMallocDefault));
   }
   const int64_t need = n * K;
   if (need > m.cap) {
-    if (m.exec) (void)hipGraphExecDestroy(m.exec);
-    m.exec = nullptr;
+    drop_graphs(c);
     double** ws[] = {&m.r, &m.x, &m.bestx, &m.p, &m.q, &m.y, &m.z, &m.io};
     for (double** w : ws)
       if ((rc = dev_alloc(c, w, (size_t)need))) return rc;
-    if ((rc = dev_alloc(c, &m.ext, (size_t)std::max<int64_t>(c->plan.tail.NT, 1) * K)))
-      return rc;
+    if ((rc = dev_alloc(c, &m.ext, (size_t)(NT * K)))) return rc;
     m.cap = need;
+  }
+  if (c->precision == LAPGPU_FP32_INNER && need > m.cap32) {
+    drop_graphs(c);
+    float** ws[] = {&m.y32, &m.z32, &m.pf};
+    for (float** w : ws)
+      if ((rc = dev_alloc(c, w, (size_t)need))) return rc;
+    if ((rc = dev_alloc(c, &m.ext32, (size_t)(NT * K)))) return rc;
+    if ((rc = dev_alloc(c, &m.r0, (size_t)need))) return rc;
+    m.cap32 = need;
   }
   return 0;
 }
 
-// z = LDLsolver(rin) for K interleaved right-hand sides: forward groups,
-// tail, backward groups, then the mean removal partials and z.r
-template <int K>
-void msweeps(lapgpu_ctx* c, const double* rin) {
+// z = LDLsolver(rin) without the mean removal for K interleaved right-hand
+// sides: forward groups, the tail (one workgroup per column), backward
+// groups.  T, S: sweep precision and tail slot type.
+template <int K, typename T, typename S>
+void msweeps_t(lapgpu_ctx* c, const double* rin, T* y, T* z, T* ext, const T* gFw, const T* gBw,
+               const T* sv, const T* Ew, const S* fslots, const S* bslots) {
   auto& m = c->m;
   const Plan& P = c->plan;
   const int n = (int)c->n;
   for (const auto& g : c->groups)
-    hipLaunchKernelGGL(k_mgroup<K>, dim3(grid_for((int64_t)(g.hi - g.lo) * K, kBlock)),
-                       dim3(kBlock), 0, c->stream, n, g.lo, g.hi, c->gFptr, c->gFidx, c->gFw,
-                       m.y, rin, m.y, m.flags);
+    hipLaunchKernelGGL((k_mgroup<K, T, double>),
+                       dim3(grid_for((int64_t)(g.hi - g.lo) * K, kBlock)), dim3(kBlock), 0,
+                       c->stream, n, g.lo, g.hi, c->gFptr, c->gFidx, gFw, y, rin, y, m.flags);
   const TailProgram& TP = P.tail;
   if (TP.NT > 0) {
-    hipLaunchKernelGGL(k_mtail_ext<K>, dim3(grid_for(TP.NT * K, kBlock)), dim3(kBlock), 0,
-                       c->stream, (int)TP.T0, (int)TP.NT, c->Eptr, c->Eidx, c->Ew, rin, m.y,
-                       m.ext, m.flags);
-    hipLaunchKernelGGL(k_mtail<K>, dim3(K), dim3(kTailThreads), 0, c->stream, (int)TP.T0,
-                       (int)TP.NT, m.ext, c->s, m.z, c->fsteps, (int)TP.fsteps.size(), c->fslots,
-                       c->bsteps, (int)TP.bsteps.size(), c->bslots, m.st, m.flags);
+    hipLaunchKernelGGL((k_mtail_ext<K, T>), dim3(grid_for(TP.NT * K, kBlock)), dim3(kBlock), 0,
+                       c->stream, (int)TP.T0, (int)TP.NT, c->Eptr, c->Eidx, Ew, rin, y, ext,
+                       m.flags);
+    hipLaunchKernelGGL((k_mtail<K, T, S>), dim3(K), dim3(kTailThreads), 0, c->stream,
+                       (int)TP.T0, (int)TP.NT, ext, sv, z, c->fsteps, (int)TP.fsteps.size(),
+                       fslots, c->bsteps, (int)TP.bsteps.size(), bslots, m.st, m.flags);
   }
   for (auto it = c->groups.rbegin(); it != c->groups.rend(); ++it)
-    hipLaunchKernelGGL(k_mgroup<K>, dim3(grid_for((int64_t)(it->hi - it->lo) * K, kBlock)),
-                       dim3(kBlock), 0, c->stream, n, it->lo, it->hi, c->gBptr, c->gBidx,
-                       c->gBw, m.z, m.y, m.z, m.flags);
+    hipLaunchKernelGGL((k_mgroup<K, T, T>),
+                       dim3(grid_for((int64_t)(it->hi - it->lo) * K, kBlock)), dim3(kBlock), 0,
+                       c->stream, n, it->lo, it->hi, c->gBptr, c->gBidx, gBw, z, y, z, m.flags);
 }
 
+template <int K>
+void msweeps(lapgpu_ctx* c, const double* rin) {
+  auto& m = c->m;
+  if (c->precision == LAPGPU_FP32_INNER)
+    msweeps_t<K, float, TailSlotF>(c, rin, m.y32, m.z32, m.ext32, c->gFw32, c->gBw32, c->s32,
+                                   c->Ew32, c->fslots32, c->bslots32);
+  else
+    msweeps_t<K, double, TailSlotD>(c, rin, m.y, m.z, m.ext, c->gFw, c->gBw, c->s, c->Ew,
+                                    c->fslots, c->bslots);
+}
+
+// mean removal partials and (yF - mu).r after the sweeps
 template <int K>
 void mfinish_precond(lapgpu_ctx* c, const double* rin) {
   auto& m = c->m;
   const int n = (int)c->n;
-  hipLaunchKernelGGL(k_mvsum<K>, dim3(kMRed), dim3(kBlock), 0, c->stream, n, m.z, m.part,
-                     m.flags);
-  hipLaunchKernelGGL(k_mscalar_mu<K>, dim3(1), dim3(kScalarThreads), 0, c->stream, n, m.part,
-                     m.st, m.flags);
-  hipLaunchKernelGGL(k_mzdot<K>, dim3(kMRed), dim3(kBlock), 0, c->stream, n, m.z, rin, m.p,
-                     m.part, m.st, m.flags);
+  if (c->precision == LAPGPU_FP32_INNER) {
+    hipLaunchKernelGGL((k_mvsum<K, float>), dim3(kMRed), dim3(kBlock), 0, c->stream, n, m.z32,
+                       m.part, m.flags);
+    hipLaunchKernelGGL(k_mscalar_mu<K>, dim3(1), dim3(kScalarThreads), 0, c->stream, n, m.part,
+                       m.st, m.flags);
+    hipLaunchKernelGGL((k_mzdot<K, float>), dim3(kMRed), dim3(kBlock), 0, c->stream, n, m.z32,
+                       rin, m.p, m.part, m.st, m.flags);
+  } else {
+    hipLaunchKernelGGL((k_mvsum<K, double>), dim3(kMRed), dim3(kBlock), 0, c->stream, n, m.z,
+                       m.part, m.flags);
+    hipLaunchKernelGGL(k_mscalar_mu<K>, dim3(1), dim3(kScalarThreads), 0, c->stream, n, m.part,
+                       m.st, m.flags);
+    hipLaunchKernelGGL((k_mzdot<K, double>), dim3(kMRed), dim3(kBlock), 0, c->stream, n, m.z,
+                       rin, m.p, m.part, m.st, m.flags);
+  }
 }
 
-// one PCG iteration for K right-hand sides
+// p = yF - mu (FP64 and, in the FP32-inner mode, the FP32 copy) or, with
+// `update`, p = (yF - mu) + beta p
 template <int K>
-void enqueue_miter(lapgpu_ctx* c, bool prof) {
+void mp_step(lapgpu_ctx* c, bool update) {
   auto& m = c->m;
   const int n = (int)c->n;
+  const bool f32 = c->precision == LAPGPU_FP32_INNER;
+  if (update) {
+    if (f32)
+      hipLaunchKernelGGL((k_mpupdate<K, float>), dim3(kMRed), dim3(kBlock), 0, c->stream, n,
+                         m.z32, m.p, m.pf, m.st, m.flags);
+    else
+      hipLaunchKernelGGL((k_mpupdate<K, double>), dim3(kMRed), dim3(kBlock), 0, c->stream, n,
+                         m.z, m.p, (float*)nullptr, m.st, m.flags);
+  } else {
+    if (f32)
+      hipLaunchKernelGGL((k_mpinit<K, float>), dim3(kMRed), dim3(kBlock), 0, c->stream, n, m.z32,
+                         m.p, m.pf, m.st, m.flags);
+    else
+      hipLaunchKernelGGL((k_mpinit<K, double>), dim3(kMRed), dim3(kBlock), 0, c->stream, n, m.z,
+                         m.p, (float*)nullptr, m.st, m.flags);
+  }
+}
+
+// one PCG iteration for K right-hand sides; `replace`: FP32-mode iteration
+// that ends with an FP64 residual replacement
+template <int K>
+void enqueue_miter(lapgpu_ctx* c, bool prof, bool replace) {
+  auto& m = c->m;
+  const int n = (int)c->n;
+  const bool f32 = c->precision == LAPGPU_FP32_INNER;
   if (prof) (void)hipEventRecord(c->ev[0], c->stream);
-  hipLaunchKernelGGL(k_mspmv<K>, dim3(kMRed), dim3(kBlock), 0, c->stream, n, c->Lptr, c->Lidx,
-                     c->Lval, m.p, m.q, m.part, m.flags);
+  if (f32)
+    hipLaunchKernelGGL(k_mspmv_mixed<K>, dim3(kMRed), dim3(kBlock), 0, c->stream, n, c->Lptr,
+                       c->Lidx, c->Lval32, m.pf, c->Lval, m.p, m.q, m.part, m.st, m.flags);
+  else
+    hipLaunchKernelGGL(k_mspmv<K>, dim3(kMRed), dim3(kBlock), 0, c->stream, n, c->Lptr, c->Lidx,
+                       c->Lval, m.p, m.q, m.part, m.flags);
   if (prof) (void)hipEventRecord(c->ev[1], c->stream);
   hipLaunchKernelGGL(k_mscalar_al<K>, dim3(1), dim3(kScalarThreads), 0, c->stream, m.part, m.st,
                      m.flags);
@@ -784,14 +887,19 @@ void enqueue_miter(lapgpu_ctx* c, bool prof) {
                      m.p, m.q, m.r, m.part, m.st, m.flags);
   hipLaunchKernelGGL(k_mscalar_nr<K>, dim3(1), dim3(kScalarThreads), 0, c->stream, m.part, m.st,
                      m.flags);
+  if (f32) {
+    hipLaunchKernelGGL(k_mresid<K>, dim3(kMRed), dim3(kBlock), 0, c->stream, n, replace ? 1 : 0,
+                       c->Lptr, c->Lidx, c->Lval, m.x, m.r0, m.r, m.part, m.st, m.flags);
+    hipLaunchKernelGGL(k_mscalar_resid<K>, dim3(1), dim3(kScalarThreads), 0, c->stream,
+                       replace ? 1 : 0, m.part, m.st, m.flags);
+  }
   if (prof) (void)hipEventRecord(c->ev[2], c->stream);
   msweeps<K>(c, m.r);
   if (prof) (void)hipEventRecord(c->ev[3], c->stream);
   mfinish_precond<K>(c, m.r);
   hipLaunchKernelGGL(k_mscalar_rho<K>, dim3(1), dim3(kScalarThreads), 0, c->stream, m.part,
                      m.st, m.flags);
-  hipLaunchKernelGGL(k_mpupdate<K>, dim3(kMRed), dim3(kBlock), 0, c->stream, n, m.z, m.p, m.st,
-                     m.flags);
+  mp_step<K>(c, true);
   if (prof) (void)hipEventRecord(c->ev[4], c->stream);
 }
 
@@ -802,13 +910,14 @@ int solve_multi_chunk(lapgpu_ctx* c, const double* d_B, double* d_X, int k, doub
                       int64_t* its_out, double* relres_out, int* stop_out, double* acc) {
   auto& m = c->m;
   const bool prof = (flags & LAPGPU_PROFILE) != 0;
+  const bool f32 = c->precision == LAPGPU_FP32_INNER;
   const int n = (int)c->n;
   int rc = multi_prepare(c, K);
   if (rc) return rc;
   for (int j = 0; j < K; ++j)
-    fill_state(c, m.hst[j], tol, maxits, maxtime, stag_test, flags, false);
+    fill_state(c, m.hst[j], tol, maxits, maxtime, stag_test, flags, f32);
   HIPC(c, hipMemcpyAsync(m.st, m.hst, sizeof(DevState) * K, hipMemcpyHostToDevice, c->stream));
-  HIPC(c, hipMemsetAsync(m.flags, 0, 2 * sizeof(int), c->stream));
+  HIPC(c, hipMemsetAsync(m.flags, 0, 4 * sizeof(int), c->stream));
 
   // setup: r = b - mean(b), z = pre(r), p = z, rho = r.z (q holds b)
   hipLaunchKernelGGL(k_mgather<K>, dim3(kMRed), dim3(kBlock), 0, c->stream, n, k, c->perm, d_B,
@@ -819,36 +928,46 @@ int solve_multi_chunk(lapgpu_ctx* c, const double* d_B, double* d_X, int k, doub
                      m.bestx, m.part, m.st);
   hipLaunchKernelGGL(k_mscalar_nb<K>, dim3(1), dim3(kScalarThreads), 0, c->stream, m.part, m.st,
                      m.flags);
+  if (f32)
+    HIPC(c, hipMemcpyAsync(m.r0, m.r, sizeof(double) * n * K, hipMemcpyDeviceToDevice,
+                           c->stream));
   msweeps<K>(c, m.r);
   mfinish_precond<K>(c, m.r);
   hipLaunchKernelGGL(k_mscalar_rho0<K>, dim3(1), dim3(kScalarThreads), 0, c->stream, m.part,
                      m.st, m.flags);
-  hipLaunchKernelGGL(k_mpinit<K>, dim3(kMRed), dim3(kBlock), 0, c->stream, n, m.z, m.p, m.st,
-                     m.flags);
+  mp_step<K>(c, false);
   HIPC(c, hipGetLastError());
   HIPC(c, hipMemcpyAsync(&c->hflags[0], m.flags, sizeof(int32_t), hipMemcpyDeviceToHost,
                          c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
   bool done = c->hflags[0] != 0;
 
-  if (!prof && c->use_graph && (!m.exec || m.exec_K != K)) {
-    if (m.exec) (void)hipGraphExecDestroy(m.exec);
-    m.exec = nullptr;
-    hipGraph_t g = nullptr;
-    HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_miter<K>(c, false);
-    HIPC(c, hipStreamEndCapture(c->stream, &g));
-    hipError_t ie = hipGraphInstantiate(&m.exec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    HIPC(c, ie);
+  const int rep_every = f32 ? c->replace_every : 0;
+  if (!prof && c->use_graph && m.exec_K != K) {
+    drop_graphs(c);
     m.exec_K = K;
+  }
+  if (!prof && c->use_graph) {
+    for (int v = 0; v < (f32 ? 2 : 1); ++v) {
+      if (m.exec[v]) continue;
+      LAPGPU_LOCK();
+      hipGraph_t g = nullptr;
+      HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_miter<K>(c, false, v == 1);
+      HIPC(c, hipStreamEndCapture(c->stream, &g));
+      hipError_t ie = hipGraphInstantiate(&m.exec[v], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIPC(c, ie);
+    }
   }
   int64_t enq = 0, polled = 0;
   while (!done) {
-    if (!prof && m.exec)
-      HIPC(c, hipGraphLaunch(m.exec, c->stream));
+    const bool replace = rep_every > 0 && ((enq + 1) % rep_every == 0);
+    hipGraphExec_t ex = m.exec[replace ? 1 : 0];
+    if (!prof && ex)
+      HIPC(c, hipGraphLaunch(ex, c->stream));
     else
-      enqueue_miter<K>(c, prof);
+      enqueue_miter<K>(c, prof, replace);
     const int slot = (int)(enq % lapgpu_ctx::kRing);
     HIPC(c, hipMemcpyAsync(&c->hflags[slot], m.flags, sizeof(int32_t), hipMemcpyDeviceToHost,
                            c->stream));
@@ -936,6 +1055,7 @@ int lapgpu_create(lapgpu_ctx** out, int device) {
     return set_err(nullptr, -10, "lapgpu_create: no HIP device visible");
   if (device < 0 || device >= count)
     return set_err(nullptr, -1, "lapgpu_create: device index out of range");
+  LAPGPU_LOCK();
   lapgpu_ctx* c = new lapgpu_ctx();
   c->device = device;
   auto fail = [&](const char* what, hipError_t e) {
@@ -974,6 +1094,7 @@ int lapgpu_create(lapgpu_ctx** out, int device) {
 
 void lapgpu_destroy(lapgpu_ctx* c) {
   if (!c) return;
+  LAPGPU_LOCK();
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_graph(c);
@@ -1175,6 +1296,28 @@ int lapgpu_factor_info(lapgpu_ctx* c, int64_t* nlevels, int64_t* nwide,
   if (nwide) *nwide = (int64_t)c->groups.size();
   if (tail_start) *tail_start = c->plan.tail.T0;
   if (nnzF) *nnzF = c->plan.nnzF;
+  return 0;
+}
+
+int lapgpu_tail_info(lapgpu_ctx* c, int64_t* nt, int64_t* nsteps, int64_t* nslots,
+                     int64_t* nedges_in) {
+  if (!c) return -1;
+  if (!c->has_F) return set_err(c, -2, "tail_info: no LDLinv uploaded");
+  const TailProgram& TP = c->plan.tail;
+  if (nt) *nt = TP.NT;
+  if (nsteps) *nsteps = (int64_t)(TP.fsteps.size() + TP.bsteps.size());
+  if (nslots) *nslots = (int64_t)(TP.fslots.size() + TP.bslots.size());
+  if (nedges_in) *nedges_in = (int64_t)TP.Eidx.size();
+  return 0;
+}
+
+int lapgpu_last_sweep_timings(lapgpu_ctx* c, double* fgroups_ms, double* tail_ext_ms,
+                              double* tail_ms, double* bgroups_ms) {
+  if (!c) return -1;
+  if (fgroups_ms) *fgroups_ms = c->t_fgroups;
+  if (tail_ext_ms) *tail_ext_ms = c->t_tail_ext;
+  if (tail_ms) *tail_ms = c->t_tail;
+  if (bgroups_ms) *bgroups_ms = c->t_bgroups;
   return 0;
 }
 

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "lapgpu.h"
+#include "lapgpu_lock.hpp"
 #include "lapgpu_plan.hpp"
 #include "lapgpu_wg.hpp"
 
@@ -94,6 +95,7 @@ int berr(lapgpu_batch* B, int code, const std::string& m) {
 
 template <typename T>
 int up(lapgpu_batch* B, T** dst, const T* src, size_t count, int64_t& bytes) {
+  LAPGPU_LOCK();
   const size_t nb = sizeof(T) * (count ? count : 1);
   BHIP(B, hipMalloc((void**)dst, nb));
   if (count) BHIP(B, hipMemcpy(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice));
@@ -104,6 +106,7 @@ int up(lapgpu_batch* B, T** dst, const T* src, size_t count, int64_t& bytes) {
 template <typename T>
 int grow(lapgpu_batch* B, T** p, size_t* cap, size_t want) {
   if (want <= *cap) return 0;
+  LAPGPU_LOCK();
   if (*p) BHIP(B, hipFree(*p));
   *p = nullptr;
   size_t c = std::max(want, *cap * 2);
@@ -113,6 +116,7 @@ int grow(lapgpu_batch* B, T** p, size_t* cap, size_t want) {
 }
 
 void free_store(GraphStore& G) {
+  LAPGPU_LOCK();
   void* ps[] = {G.perm, G.Lptr, G.Lidx, G.Lval, G.s, G.fsteps, G.bsteps, G.fslots, G.bslots};
   for (void* p : ps)
     if (p) (void)hipFree(p);
@@ -131,6 +135,7 @@ int lapgpu_batch_create(lapgpu_batch** out, int device) {
     return berr(nullptr, -10, "lapgpu_batch_create: no HIP device visible");
   if (device < 0 || device >= count)
     return berr(nullptr, -1, "lapgpu_batch_create: device index out of range");
+  LAPGPU_LOCK();
   lapgpu_batch* B = new lapgpu_batch();
   B->device = device;
   hipError_t e = hipSetDevice(device);
@@ -159,6 +164,7 @@ int lapgpu_batch_create(lapgpu_batch** out, int device) {
 
 void lapgpu_batch_destroy(lapgpu_batch* B) {
   if (!B) return;
+  LAPGPU_LOCK();
   (void)hipSetDevice(B->device);
   if (B->stream) (void)hipStreamSynchronize(B->stream);
   for (BatchSlot& S : B->slots) {
@@ -266,6 +272,7 @@ int launch_on(lapgpu_batch* B, int slot, int64_t nreq, const int32_t* graph_ids,
   const int ng = (int)B->graphs.size();
   int rc;
   if (B->graphs_dirty) {
+    LAPGPU_LOCK();
     // other slots may still read the descriptor table: let them finish
     for (BatchSlot& o : B->slots)
       if (o.busy) BHIP(B, hipStreamSynchronize(o.stream));
@@ -290,6 +297,7 @@ int launch_on(lapgpu_batch* B, int slot, int64_t nreq, const int32_t* graph_ids,
   if ((rc = grow(B, &S.d_work, &S.work_cap, work))) return rc;
   if ((rc = grow(B, &S.d_reqs, &S.reqs_cap, (size_t)nreq))) return rc;
   if ((size_t)nreq > S.h_cap) {
+    LAPGPU_LOCK();
     if (S.h_reqs) BHIP(B, hipHostFree(S.h_reqs));
     S.h_reqs = nullptr;
     BHIP(B, hipHostMalloc((void**)&S.h_reqs, sizeof(WgReqD) * nreq, hipHostMallocDefault));
@@ -373,6 +381,7 @@ int lapgpu_batch_submit_host(lapgpu_batch* B, int64_t nreq, const int32_t* graph
   for (int64_t i = 0; i < nreq; ++i) total += (size_t)B->graphs[graph_ids[i]].n;
   if ((rc = grow(B, &S.d_io, &S.io_cap, 2 * total))) return rc;
   if (total > S.hio_cap) {
+    LAPGPU_LOCK();
     if (S.h_io) BHIP(B, hipHostFree(S.h_io));
     S.h_io = nullptr;
     BHIP(B, hipHostMalloc((void**)&S.h_io, sizeof(double) * total, hipHostMallocDefault));

@@ -95,9 +95,18 @@ __device__ inline void step_rho(DevState* st, double zr, double sz, double sp) {
     if (st->best_rho > st->stag_fac * rho) {
       st->stag_count += 1;
       if (st->stag_count > st->stag_test) {
-        st->stopped = 1;
-        st->stop_reason = 3;
-        return;
+        if (st->fp32 && !st->spmv64) {
+          // FP32-inner mode: the FP32 SpMV's accuracy floor has been
+          // reached above switch_tol; continue with the FP64 SpMV from an
+          // FP64 true residual instead of stopping
+          st->spmv64 = 1;
+          st->verify = 1;
+          st->stag_count = 0;
+        } else {
+          st->stopped = 1;
+          st->stop_reason = 3;
+          return;
+        }
       }
     }
   }
@@ -503,6 +512,9 @@ __global__ void k_scalar_resid(int always, const double* __restrict__ part, DevS
   if (threadIdx.x == 0) {
     const double nr = sqrt(rr) / st->nb;
     st->nr = nr;
+    // a true residual below switch_tol also ends the FP32 SpMV (the
+    // recursive one may hover above it at the FP32 SpMV's accuracy floor)
+    if (!st->spmv64 && nr < st->switch_tol) st->spmv64 = 1;
     // the true residual decides whether this iterate is the best so far
     st->bestnr = st->bestnr_prev;
     st->pending_copy = 0;

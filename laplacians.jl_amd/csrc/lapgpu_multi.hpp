@@ -9,8 +9,14 @@
 //
 // flags[0] = 1 once every right-hand side has stopped (all kernels then
 // return at once); flags[1] = this iteration is a 10th one (the stagnation
-// checks need the 1-norm partials).  All running right-hand sides share the
-// iteration count, so one flag serves them all.
+// checks need the 1-norm partials; all running right-hand sides share the
+// iteration count, so one flag serves them all); FP32-inner mode:
+// flags[3] = some running column waits for an FP64 true-residual check.
+//
+// FP32-inner mode (as the single-RHS path): SpMV and the LDLinv apply in
+// FP32 (weights, y, z), everything else in FP64 — r, x, p, the dots,
+// alpha/beta, and the true residual r = r0 - la x recomputed every k
+// iterations and before a column is accepted as converged.
 // Included by lapgpu.hip only.
 #pragma once
 
@@ -69,40 +75,43 @@ __device__ __forceinline__ double mtotal(const double* __restrict__ part, int sl
   return t;
 }
 
-__device__ __forceinline__ double readlane_d(double v, int lane) {
+__device__ __forceinline__ double readlane_t(double v, int lane) {
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
   const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
+__device__ __forceinline__ float readlane_t(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
 
-// sum over the entries [k0, k1) of a row of w[e] * src(idx[e]) for the K
-// lanes (one per rhs) of row group `sub` of a wave.  Indices and weights
-// are loaded K at a time, one per lane, and broadcast within the group, so
-// the gathers of one row are independent loads rather than a chain of
-// index -> value loads.  For K = 64 the row is wave-uniform (k0, k1 must be
-// uniform) and the broadcast is a readlane into scalar registers.
-template <int K>
-__device__ __forceinline__ void mbcast(int myi, double myw, int src, int& i, double& w) {
+template <int K, typename T>
+__device__ __forceinline__ void mbcast(int myi, T myw, int src, int& i, T& w) {
   if constexpr (K == kWave) {
     i = __builtin_amdgcn_readlane(myi, src);
-    w = readlane_d(myw, src);
+    w = readlane_t(myw, src);
   } else {
     i = __shfl(myi, src, kWave);
     w = __shfl(myw, src, kWave);
   }
 }
 
-template <int K, typename F>
-__device__ __forceinline__ double mrow_dot(int k0, int k1, int j, int sub,
-                                           const int* __restrict__ idx,
-                                           const double* __restrict__ w, F&& src) {
+// sum over the entries [k0, k1) of a row of w[e] * src(idx[e]) for the K
+// lanes (one per rhs) of row group `sub` of a wave, in entry order, in T.
+// Indices and weights are loaded K at a time, one per lane, and broadcast
+// within the group, so the gathers of one row are independent loads rather
+// than a chain of index -> value loads.  For K = 64 the row is wave-uniform
+// (k0, k1 must be uniform) and the broadcast is a readlane into scalar
+// registers.
+template <int K, typename T, typename F>
+__device__ __forceinline__ T mrow_dot(int k0, int k1, int j, int sub, const int* __restrict__ idx,
+                                      const T* __restrict__ w, F&& src) {
   constexpr int U = K == kWave ? 8 : 4;  // gathers in flight per lane
-  double acc = 0.0;
+  T acc = 0;
   for (int base = k0; base < k1; base += K) {
     const int kk = base + j;
     int myi = 0;
-    double myw = 0.0;
+    T myw = 0;
     if (kk < k1) {
       myi = idx[kk];
       myw = w[kk];
@@ -112,7 +121,7 @@ __device__ __forceinline__ double mrow_dot(int k0, int k1, int j, int sub,
     int e = 0;
     for (; e + U <= cnt; e += U) {
       int i[U];
-      double we[U], v[U];
+      T we[U], v[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) mbcast<K>(myi, myw, g0 + e + u, i[u], we[u]);
 #pragma unroll
@@ -122,7 +131,7 @@ __device__ __forceinline__ double mrow_dot(int k0, int k1, int j, int sub,
     }
     for (; e < cnt; ++e) {
       int i;
-      double we;
+      T we;
       mbcast<K>(myi, myw, g0 + e, i, we);
       acc += we * src(i);
     }
@@ -218,10 +227,10 @@ __global__ void k_mscalar_nb(const double* __restrict__ part, DevState* mst, int
 // One level group (composite rows, as k_group), K lanes per row:
 //   out[v*K+j] = sum_e w[e] * (idx[e] < n ? A[idx[e]*K+j] : B[(idx[e]-n)*K+j])
 // forward: A = y0, B = r, out = y0; backward: A = yF, B = y0, out = yF.
-template <int K>
+template <int K, typename T, typename TB>
 __global__ void __launch_bounds__(kBlock)
 k_mgroup(int n, int lo, int hi, const int* __restrict__ ptr, const int* __restrict__ idx,
-         const double* __restrict__ w, const double* A, const double* B, double* out,
+         const T* __restrict__ w, const T* A, const TB* B, T* out,
          const int* __restrict__ flags) {
   if (flags[0]) return;
   constexpr int R = kWave / K;  // rows per wave
@@ -229,42 +238,40 @@ k_mgroup(int n, int lo, int hi, const int* __restrict__ ptr, const int* __restri
   const int row = lo + ((blockIdx.x * kBlock + threadIdx.x) / kWave) * R + sub;
   if (row >= hi) return;  // whole row groups leave together
   const int k0 = wave_uniform<K>(ptr[row]), k1 = wave_uniform<K>(ptr[row + 1]);
-  const double acc = mrow_dot<K>(k0, k1, j, sub, idx, w, [&](int i) {
-    return i < n ? A[(int64_t)i * K + j] : B[(int64_t)(i - n) * K + j];
+  const T acc = mrow_dot<K, T>(k0, k1, j, sub, idx, w, [&](int i) -> T {
+    return i < n ? A[(int64_t)i * K + j] : (T)B[(int64_t)(i - n) * K + j];
   });
   out[(int64_t)row * K + j] = acc;
 }
 
 // ext[i*K+j] = r[(T0+i)*K+j] + the forward in-edges of tail row i from
 // outside the tail (as k_tail_ext), one lane per (row, rhs)
-template <int K>
+template <int K, typename T>
 __global__ void __launch_bounds__(kBlock)
 k_mtail_ext(int T0, int NT, const int* __restrict__ Eptr, const int* __restrict__ Eidx,
-            const double* __restrict__ Ew, const double* __restrict__ r,
-            const double* __restrict__ y, double* __restrict__ ext,
-            const int* __restrict__ flags) {
+            const T* __restrict__ Ew, const double* __restrict__ r, const T* __restrict__ y,
+            T* __restrict__ ext, const int* __restrict__ flags) {
   if (flags[0]) return;
   constexpr int R = kWave / K;
   const int lane = threadIdx.x % kWave, j = lane % K, sub = lane / K;
   const int row = ((blockIdx.x * kBlock + threadIdx.x) / kWave) * R + sub;
   if (row >= NT) return;
   const int k0 = wave_uniform<K>(Eptr[row]), k1 = wave_uniform<K>(Eptr[row + 1]);
-  const double acc =
-      mrow_dot<K>(k0, k1, j, sub, Eidx, Ew, [&](int i) { return y[(int64_t)i * K + j]; });
-  ext[(int64_t)row * K + j] = r[(int64_t)(T0 + row) * K + j] + acc;
+  const T acc =
+      mrow_dot<K, T>(k0, k1, j, sub, Eidx, Ew, [&](int i) -> T { return y[(int64_t)i * K + j]; });
+  ext[(int64_t)row * K + j] = (T)r[(int64_t)(T0 + row) * K + j] + acc;
 }
 
 // The tail for rhs j = blockIdx.x: the single-RHS slot programs on an
 // LDS-resident copy of column j (as k_tail); K workgroups run side by side
 // on K CUs.  Columns that have stopped are skipped.
-template <int K>
+template <int K, typename T, typename S>
 __global__ void __launch_bounds__(kTailThreads)
-k_mtail(int T0, int NT, const double* __restrict__ ext, const double* __restrict__ s,
-        double* __restrict__ z, const TailStepD* __restrict__ fsteps, int nfsteps,
-        const TailSlotD* __restrict__ fslots, const TailStepD* __restrict__ bsteps,
-        int nbsteps, const TailSlotD* __restrict__ bslots, const DevState* __restrict__ mst,
-        const int* __restrict__ flags) {
-  __shared__ double ly[kTailCapV];
+k_mtail(int T0, int NT, const T* __restrict__ ext, const T* __restrict__ s, T* __restrict__ z,
+        const TailStepD* __restrict__ fsteps, int nfsteps, const S* __restrict__ fslots,
+        const TailStepD* __restrict__ bsteps, int nbsteps, const S* __restrict__ bslots,
+        const DevState* __restrict__ mst, const int* __restrict__ flags) {
+  __shared__ T ly[kTailCapV];
   const int j = blockIdx.x;
   if (flags[0] || mst[j].stopped) return;
   for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] = ext[(int64_t)i * K + j];
@@ -279,16 +286,16 @@ k_mtail(int T0, int NT, const double* __restrict__ ext, const double* __restrict
 }
 
 // partial sums of yF (mean removal of LDLsolver)
-template <int K>
+template <int K, typename T>
 __global__ void __launch_bounds__(kBlock)
-k_mvsum(int n, const double* __restrict__ z, double* __restrict__ part,
+k_mvsum(int n, const T* __restrict__ z, double* __restrict__ part,
         const int* __restrict__ flags) {
   if (flags[0]) return;
   const int64_t nK = (int64_t)n * K;
   double a[1] = {0.0};
   for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nK;
        e += (int64_t)gridDim.x * kBlock)
-    a[0] += z[e];
+    a[0] += (double)z[e];
   const int slots[1] = {S_SUM};
   mblock_sum<K, 1>(a, part, slots);
 }
@@ -302,9 +309,9 @@ __global__ void k_mscalar_mu(int n, const double* __restrict__ part, DevState* m
 }
 
 // partials (yF - mu).r and, on 10th iterations, sum|yF - mu|, sum|p|
-template <int K>
+template <int K, typename T>
 __global__ void __launch_bounds__(kBlock)
-k_mzdot(int n, const double* __restrict__ z, const double* __restrict__ r,
+k_mzdot(int n, const T* __restrict__ z, const double* __restrict__ r,
         const double* __restrict__ p, double* __restrict__ part,
         const DevState* __restrict__ mst, const int* __restrict__ flags) {
   if (flags[0]) return;
@@ -315,7 +322,7 @@ k_mzdot(int n, const double* __restrict__ z, const double* __restrict__ r,
   double a[3] = {0.0, 0.0, 0.0};
   for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nK;
        e += (int64_t)gridDim.x * kBlock) {
-    const double t = z[e] - mu;
+    const double t = (double)z[e] - mu;
     a[0] += t * r[e];
     if (chk) {
       a[1] += fabs(t);
@@ -334,18 +341,21 @@ __global__ void k_mscalar_rho0(const double* __restrict__ part, DevState* mst, i
   mark_all<K>(mst, flags);
 }
 
-// p = yF - mu after the setup's preconditioner
-template <int K>
+// p = yF - mu after the setup's preconditioner (and its FP32 copy when pf)
+template <int K, typename T>
 __global__ void __launch_bounds__(kBlock)
-k_mpinit(int n, const double* __restrict__ z, double* __restrict__ p,
+k_mpinit(int n, const T* __restrict__ z, double* __restrict__ p, float* __restrict__ pf,
          const DevState* __restrict__ mst, const int* __restrict__ flags) {
   if (flags[0]) return;
   const int j = threadIdx.x % K;
   const double mu = mst[j].mu;
   const int64_t nK = (int64_t)n * K;
   for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nK;
-       e += (int64_t)gridDim.x * kBlock)
-    p[e] = z[e] - mu;
+       e += (int64_t)gridDim.x * kBlock) {
+    const double t = (double)z[e] - mu;
+    p[e] = t;
+    if (pf) pf[e] = (float)t;
+  }
 }
 
 // ---- one PCG iteration ---------------------------------------------------------
@@ -366,6 +376,61 @@ k_mspmv(int n, const int* __restrict__ Lptr, const int* __restrict__ Lidx,
     const int k0 = wave_uniform<K>(Lptr[row]), k1 = wave_uniform<K>(Lptr[row + 1]);
     const double acc =
         mrow_dot<K>(k0, k1, j, sub, Lidx, Lval, [&](int i) { return p[(int64_t)i * K + j]; });
+    q[(int64_t)row * K + j] = acc;
+    a[0] += p[(int64_t)row * K + j] * acc;
+  }
+  const int slots[1] = {S_PQ};
+  mblock_sum<K, 1>(a, part, slots);
+}
+
+// FP32-inner mode: q = la * p per column from the FP32 copies of la and p
+// (sums in FP32) until that column leaves the FP32 SpMV (its spmv64, as the
+// single-RHS k_spmv_dot_mixed), from the FP64 ones afterwards; p.q in FP64.
+// Both weight arrays are broadcast; each lane gathers from its column's
+// precision.
+template <int K>
+__global__ void __launch_bounds__(kBlock)
+k_mspmv_mixed(int n, const int* __restrict__ Lptr, const int* __restrict__ Lidx,
+              const float* __restrict__ Lval32, const float* __restrict__ pf,
+              const double* __restrict__ Lval, const double* __restrict__ p,
+              double* __restrict__ q, double* __restrict__ part,
+              const DevState* __restrict__ mst, const int* __restrict__ flags) {
+  if (flags[0]) return;
+  constexpr int R = kWave / K;
+  const int lane = threadIdx.x % kWave, j = lane % K, sub = lane / K;
+  const bool f64 = mst[j].spmv64 != 0;
+  const int wave = (blockIdx.x * kBlock + threadIdx.x) / kWave;
+  const int nwaves = gridDim.x * (kBlock / kWave);
+  double a[1] = {0.0};
+  for (int row = wave * R + sub; row < n; row += nwaves * R) {
+    const int k0 = wave_uniform<K>(Lptr[row]), k1 = wave_uniform<K>(Lptr[row + 1]);
+    double a64 = 0.0;
+    float a32 = 0.0f;
+    for (int base = k0; base < k1; base += K) {
+      const int kk = base + j;
+      int myi = 0;
+      double myw = 0.0;
+      float myw32 = 0.0f;
+      if (kk < k1) {
+        myi = Lidx[kk];
+        myw = Lval[kk];
+        myw32 = Lval32[kk];
+      }
+      const int cnt = min(K, k1 - base);
+      for (int e = 0; e < cnt; ++e) {
+        int i;
+        double w;
+        float w32;
+        mbcast<K>(myi, myw, sub * K + e, i, w);
+        int i2;
+        mbcast<K>(myi, myw32, sub * K + e, i2, w32);
+        if (f64)
+          a64 += w * p[(int64_t)i * K + j];
+        else
+          a32 += w32 * pf[(int64_t)i * K + j];
+      }
+    }
+    const double acc = f64 ? a64 : (double)a32;
     q[(int64_t)row * K + j] = acc;
     a[0] += p[(int64_t)row * K + j] * acc;
   }
@@ -437,6 +502,72 @@ __global__ void k_mscalar_nr(const double* __restrict__ part, DevState* mst, int
   if (threadIdx.x < K && !mst[threadIdx.x].stopped)
     step_nr(mst + threadIdx.x, rr, sp, sx, sq, sr);
   mark_all<K>(mst, flags);
+  if (threadIdx.x == 0) {
+    int anyv = 0;
+    for (int j = 0; j < K; ++j)
+      if (!mst[j].stopped) anyv |= mst[j].verify;
+    flags[3] = anyv;
+  }
+}
+
+// FP32-inner mode: r = r0 - la x in FP64 for the running columns, when
+// `always` (the periodic residual replacement) or for the columns whose
+// recursive residual has just passed tol (verify); partial r.r
+template <int K>
+__global__ void __launch_bounds__(kBlock)
+k_mresid(int n, int always, const int* __restrict__ Lptr, const int* __restrict__ Lidx,
+         const double* __restrict__ Lval, const double* __restrict__ x,
+         const double* __restrict__ r0, double* __restrict__ r, double* __restrict__ part,
+         const DevState* __restrict__ mst, const int* __restrict__ flags) {
+  if (flags[0] || !(always || flags[3])) return;
+  constexpr int R = kWave / K;
+  const int lane = threadIdx.x % kWave, j = lane % K, sub = lane / K;
+  const bool run = !mst[j].stopped && (always || mst[j].verify);
+  const int wave = (blockIdx.x * kBlock + threadIdx.x) / kWave;
+  const int nwaves = gridDim.x * (kBlock / kWave);
+  double a[1] = {0.0};
+  for (int row = wave * R + sub; row < n; row += nwaves * R) {
+    const int k0 = wave_uniform<K>(Lptr[row]), k1 = wave_uniform<K>(Lptr[row + 1]);
+    const double acc = mrow_dot<K, double>(k0, k1, j, sub, Lidx, Lval,
+                                           [&](int i) { return x[(int64_t)i * K + j]; });
+    if (run) {
+      const double t = r0[(int64_t)row * K + j] - acc;
+      r[(int64_t)row * K + j] = t;
+      a[0] += t * t;
+    }
+  }
+  const int slots[1] = {S_RR};
+  mblock_sum<K, 1>(a, part, slots);
+}
+
+// after k_mresid: the true relative residual replaces nr (as
+// k_scalar_resid, per column); convergence is accepted only on it
+template <int K>
+__global__ void k_mscalar_resid(int always, const double* __restrict__ part, DevState* mst,
+                                int* flags) {
+  if (flags[0] || !(always || flags[3])) return;
+  const double rr = mtotal<K>(part, S_RR);
+  if (threadIdx.x < K) {
+    DevState* st = mst + threadIdx.x;
+    if (!st->stopped && (always || st->verify)) {
+      const double nr = sqrt(rr) / st->nb;
+      st->nr = nr;
+      if (!st->spmv64 && nr < st->switch_tol) st->spmv64 = 1;
+      st->bestnr = st->bestnr_prev;
+      st->pending_copy = 0;
+      if (nr < st->bestnr) {
+        st->bestnr = nr;
+        st->pending_copy = 1;
+      }
+      if (st->verify && nr < st->tol) {
+        st->stopped = 1;
+        st->stop_reason = 0;
+      }
+      st->verify = 0;
+    }
+  }
+  mark_all<K>(mst, flags);
+  if (threadIdx.x == 0) flags[3] = 0;
 }
 
 template <int K>
@@ -449,10 +580,11 @@ __global__ void k_mscalar_rho(const double* __restrict__ part, DevState* mst, in
   mark_all<K>(mst, flags);
 }
 
-// p = (yF - mu) + beta p for the right-hand sides still running
-template <int K>
+// p = (yF - mu) + beta p for the right-hand sides still running (and its
+// FP32 copy when pf)
+template <int K, typename T>
 __global__ void __launch_bounds__(kBlock)
-k_mpupdate(int n, const double* __restrict__ z, double* __restrict__ p,
+k_mpupdate(int n, const T* __restrict__ z, double* __restrict__ p, float* __restrict__ pf,
            const DevState* __restrict__ mst, const int* __restrict__ flags) {
   if (flags[0]) return;
   const int j = threadIdx.x % K;
@@ -460,8 +592,11 @@ k_mpupdate(int n, const double* __restrict__ z, double* __restrict__ p,
   const double mu = mst[j].mu, beta = mst[j].beta;
   const int64_t nK = (int64_t)n * K;
   for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nK;
-       e += (int64_t)gridDim.x * kBlock)
-    p[e] = (z[e] - mu) + beta * p[e];
+       e += (int64_t)gridDim.x * kBlock) {
+    const double t = ((double)z[e] - mu) + beta * p[e];
+    p[e] = t;
+    if (pf) pf[e] = (float)t;
+  }
 }
 
 // X[j*n + perm[v]] = best iterate (x where the last iteration improved)

@@ -15,6 +15,7 @@
 // on vertices of lower (forward) or higher (backward) DAG level.  Vertices
 // are renumbered by forward level so every level is a contiguous range.
 #include "lapgpu_plan.hpp"
+#include "lapgpu_lock.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -518,4 +519,11 @@ bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
   return true;
 }
 
+}  // namespace lapgpu
+
+namespace lapgpu {
+std::recursive_mutex& device_lock() {
+  static std::recursive_mutex m;
+  return m;
+}
 }  // namespace lapgpu

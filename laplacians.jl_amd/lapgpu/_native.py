@@ -47,6 +47,8 @@ EXPORTS = (
     "lapgpu_set_precision",
     "lapgpu_factor_info",
     "lapgpu_last_timings",
+    "lapgpu_last_sweep_timings",
+    "lapgpu_tail_info",
     "lapgpu_last_error",
     "lapgpu_destroy",
     "lapgpu_batch_create",
@@ -93,6 +95,8 @@ def load() -> ctypes.CDLL:
     L.lapgpu_set_precision.argtypes = [P, ci, ci]
     L.lapgpu_factor_info.argtypes = [P, pi64, pi64, pi64, pi64]
     L.lapgpu_last_timings.argtypes = [P, pdbl, pdbl, pdbl, pdbl, pi64]
+    L.lapgpu_last_sweep_timings.argtypes = [P, pdbl, pdbl, pdbl, pdbl]
+    L.lapgpu_tail_info.argtypes = [P, pi64, pi64, pi64, pi64]
     L.lapgpu_last_error.argtypes = [P]
     L.lapgpu_last_error.restype = ctypes.c_char_p
     L.lapgpu_destroy.argtypes = [P]
@@ -181,6 +185,13 @@ class DeviceArray:
             raise ValueError("DeviceArray.upload: size mismatch")
         if _hip().hipMemcpy(self.ptr, _p(a), 8 * self.count, 1) != 0:
             raise LapGPUError("hipMemcpy host to device failed")
+
+    def copy_from(self, other: "DeviceArray") -> None:
+        """Device-to-device copy (a plain copy kernel of known size)."""
+        if other.count != self.count:
+            raise ValueError("DeviceArray.copy_from: size mismatch")
+        if _hip().hipMemcpy(self.ptr, other.ptr, 8 * self.count, 3) != 0:
+            raise LapGPUError("hipMemcpy device to device failed")
 
     def download(self) -> np.ndarray:
         out = np.empty(self.count, dtype=np.float64)
@@ -347,6 +358,19 @@ class Context:
                     "lapgpu_last_timings")
         return {"spmv_ms": v[0].value, "ldl_ms": v[1].value, "vec_ms": v[2].value, "total_ms": v[3].value,
                 "its": its.value}
+
+    def tail_info(self) -> dict:
+        v = [ctypes.c_int64() for _ in range(4)]
+        self._check(self._lib.lapgpu_tail_info(self._h, *(ctypes.byref(t) for t in v)), "lapgpu_tail_info")
+        return {"nt": v[0].value, "nsteps": v[1].value, "nslots": v[2].value, "nedges_in": v[3].value}
+
+    def last_sweep_timings(self) -> dict:
+        """Per-iteration ms of the sweep kernels of the last PROFILE solve."""
+        v = [ctypes.c_double() for _ in range(4)]
+        self._check(self._lib.lapgpu_last_sweep_timings(self._h, *(ctypes.byref(t) for t in v)),
+                    "lapgpu_last_sweep_timings")
+        return {"fgroups_ms": v[0].value, "tail_ext_ms": v[1].value, "tail_ms": v[2].value,
+                "bgroups_ms": v[3].value}
 
 
 class Batch:

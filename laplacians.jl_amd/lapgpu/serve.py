@@ -6,9 +6,11 @@ over time; each is answered by the same solve closure as approxchol_lap's
 
 * admits a request as soon as it arrives: graphs of at most `wg_max_n`
   vertices are queued for the next batched launch (one workgroup per
-  request, lapgpu_batch_submit_host), larger graphs go to a worker that
-  runs the plain multi-kernel solve (lapgpu_solve) on that graph's own
-  stream; both kinds overlap on the device;
+  request, lapgpu_batch_submit_host); larger graphs queue per graph, and a
+  worker solves everything queued for that graph in one multi-RHS solve
+  (lapgpu_solve_multi, up to the graph's `max_rhs` columns sharing every
+  launch; lapgpu_solve when only one is queued) on the graph's own stream;
+  all kinds overlap on the device;
 * keeps up to four batched launches in flight (the batch's launch slots):
   requests that arrive while others run start in the next free slot instead
   of waiting for the running launch to finish;
@@ -69,6 +71,7 @@ class _Graph:
     handle: object = None  # batch graph id or Context
     busy: int = 0
     pending: deque = field(default_factory=deque)
+    max_rhs: int = 1  # columns per multi-RHS solve on this graph
 
 
 def graph_bytes_estimate(n: int, nnz_l: int, nnzf: int) -> int:
@@ -78,16 +81,26 @@ def graph_bytes_estimate(n: int, nnz_l: int, nnzf: int) -> int:
     return 12 * nnz_l + 32 * nnzf + 8 * 8 * n + 16 * n
 
 
+def multi_bytes_estimate(n: int, k: int) -> int:
+    """Workspace of a k-column multi-RHS solve: 9 interleaved n-by-K vectors
+    (K = k rounded up to a power of two, at least 8)."""
+    K = 8
+    while K < k:
+        K *= 2
+    return 9 * 8 * n * K
+
+
 class Server:
-    def __init__(self, device: int = 0, hbm_budget: int | None = None, wg_max_n: int = 200_000,
-                 max_launch: int = 512, plain_workers: int = 3, factor_workers: int = 4,
-                 keep_x: bool = False):
+    def __init__(self, device: int = 0, hbm_budget: int | None = None, wg_max_n: int = 20_000,
+                 max_launch: int = 512, plain_workers: int = 4, factor_workers: int = 4,
+                 keep_x: bool = False, max_rhs: int = 64):
         self.device = device
         self.batch = _native.Batch(device)
         self.hbm_budget = hbm_budget if hbm_budget is not None else 200 * 2**30
         self.wg_max_n = wg_max_n
         self.max_launch = max_launch
         self.keep_x = keep_x
+        self.max_rhs = max_rhs
         self.graphs: "OrderedDict[object, _Graph]" = OrderedDict()
         self.hbm_used = 0
         self._plain_pool = ThreadPoolExecutor(max_workers=plain_workers)
@@ -116,10 +129,16 @@ class Server:
             gid = self.batch.add_graph(la, ldli)
             g = _Graph(key, n, "wg", self.batch.graph_info(gid)["device_bytes"] + 64 * n, resident, gid)
         else:
+            # the widest multi-RHS workspace that still fits the budget
+            k = self.max_rhs
+            while k > 1 and self.hbm_used + need + multi_bytes_estimate(n, k) > self.hbm_budget:
+                k //= 2
+            if k > 1:
+                need += multi_bytes_estimate(n, k)
             ctx = _native.Context(self.device)
             ctx.upload_lap(la)
             ctx.upload_ldli(ldli)
-            g = _Graph(key, n, "plain", need, resident, ctx)
+            g = _Graph(key, n, "plain", need, resident, ctx, max_rhs=k)
         self.graphs[key] = g
         self.hbm_used += g.bytes
         return True
@@ -171,13 +190,22 @@ class Server:
                 pump_plain(g)
 
         def pump_plain(g: _Graph):
-            # one solve at a time per context (a context owns one stream)
+            # one solve at a time per context (a context owns one stream);
+            # everything queued with the same tol and maxits goes together
             if g.busy or not g.pending:
                 return
-            r = g.pending.popleft()
+            take = [g.pending.popleft()]
+            while (g.pending and len(take) < g.max_rhs and g.pending[0].tol == take[0].tol
+                   and g.pending[0].maxits == take[0].maxits):
+                take.append(g.pending.popleft())
             g.busy = 1
-            fut = self._plain_pool.submit(g.handle.solve, r.b, r.tol, r.maxits, float("inf"), 5, 0)
-            plain_futs[fut] = (g, r)
+            r0 = take[0]
+            if len(take) == 1:
+                fut = self._plain_pool.submit(g.handle.solve, r0.b, r0.tol, r0.maxits, float("inf"), 5, 0)
+            else:
+                B = np.stack([r.b for r in take], axis=1)
+                fut = self._plain_pool.submit(g.handle.solve_multi, B, r0.tol, r0.maxits, float("inf"), 5, 0)
+            plain_futs[fut] = (g, take)
 
         while len(results) < total:
             now = time.perf_counter() - t0
@@ -217,10 +245,15 @@ class Server:
                 progressed = True
             # finished plain solves
             for fut in [f for f in plain_futs if f.done()]:
-                g, r = plain_futs.pop(fut)
+                g, take = plain_futs.pop(fut)
                 x, info = fut.result()
                 g.busy = 0
-                finish(r, info, x, "plain")
+                if len(take) == 1:
+                    finish(take[0], info, x, "plain")
+                else:
+                    for j, r in enumerate(take):
+                        finish(r, {"its": info["its"][j], "nr": info["nr"][j], "stop": info["stop"][j]},
+                               x[:, j].copy(), "multi")
                 pump_plain(g)
                 progressed = True
             # start a batched launch with everything queued, if a slot is free

@@ -53,3 +53,30 @@ def test_fp32_apply_close_to_oracle(gpu_ctx_factory, oracle_mod):
     assert np.linalg.norm(y - la @ r) <= 1e-5 * np.linalg.norm(la @ r)
     ctx.set_precision(_native.FP64)
     assert np.linalg.norm(ctx.apply_ldli(r) - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("k", [3, 16])
+def test_fp32_inner_multi(k, gpu_ctx_factory):
+    """FP32-inner mode with k right-hand sides at once: every column meets
+    tol on its FP64 true residual and stays close to its FP64 solution."""
+    rng = np.random.default_rng(5)
+    a = graphs.path_matchings(20000, 4, rng, "logU3")
+    la = lapgpu.lap(a)
+    f = lapgpu.build_ldli(a, seed=1)
+    n = a.shape[0]
+    B = rng.standard_normal((n, k))
+    B -= B.mean(axis=0)
+    c64 = gpu_ctx_factory(la, f)
+    X64, i64 = c64.solve_multi(B, tol=1e-8, maxits=1000)
+    c32 = gpu_ctx_factory(la, f)
+    c32.set_precision(_native.FP32_INNER, 20)
+    X32, i32 = c32.solve_multi(B, tol=1e-8, maxits=1000)
+    rel = np.linalg.norm(la @ X32 - B, axis=0) / np.linalg.norm(B, axis=0)
+    assert all(s == "converged" for s in i32["stop"]), i32
+    assert np.all(rel <= 1e-8), rel
+    for j in range(k):
+        assert np.linalg.norm(X32[:, j] - X64[:, j]) <= 1e-5 * np.linalg.norm(X64[:, j])
+        assert i32["its"][j] <= 4 * i64["its"][j] + 10
+    # one column alone in FP32 mode agrees with its multi-RHS result
+    x1, i1 = c32.solve(B[:, 0], tol=1e-8, maxits=1000)
+    assert np.linalg.norm(x1 - X32[:, 0]) <= 1e-5 * np.linalg.norm(x1)
