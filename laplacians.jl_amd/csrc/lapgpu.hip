@@ -23,6 +23,8 @@
 
 using namespace lapgpu;
 
+static_assert(kTailThreads == kTailLanes, "tail workgroup width = lanes of a tail step");
+
 namespace {
 
 std::string g_create_err;
@@ -194,6 +196,15 @@ int grid_for(int64_t items, int per_block) {
     default: { constexpr int G_ = 64; F(__VA_ARGS__); break; }  \
   }
 
+// Calls F with the compile-time entries per lane E_ = 2^elog of a tail
+// program (F may name E_).
+#define LAPGPU_DISPATCH_E(elog, F, ...)                              \
+  switch (elog) {                                                    \
+    case 0: { constexpr int E_ = 1; F(__VA_ARGS__); break; }         \
+    case 1: { constexpr int E_ = 2; F(__VA_ARGS__); break; }         \
+    default: { constexpr int E_ = 4; F(__VA_ARGS__); break; }        \
+  }
+
 template <int G>
 void launch_spmv(lapgpu_ctx* c, const double* p, double* q) {
   if (c->precision == LAPGPU_FP32_INNER)
@@ -247,9 +258,10 @@ void launch_sweeps(lapgpu_ctx* c, const double* rin, T* y, T* z, const T* gFw, c
                          0, c->stream, (int)TP.T0, (int)TP.NT, c->Eptr, c->Eidx, Ew, rin, y, ext,
                          c->st);
     if (c->prof_sweeps) (void)hipEventRecord(c->ev[6], c->stream);
-    hipLaunchKernelGGL((k_tail<T, S>), dim3(1), dim3(kTailThreads), 0, c->stream, (int)TP.T0,
-                       (int)TP.NT, ext, sv, z, c->fsteps, (int)TP.fsteps.size(), fslots,
-                       c->bsteps, (int)TP.bsteps.size(), bslots, c->st);
+    LAPGPU_DISPATCH_E(TP.elog, hipLaunchKernelGGL, (k_tail<T, S, E_>), dim3(1),
+                      dim3(kTailThreads), 0, c->stream, (int)TP.T0, (int)TP.NT, ext, sv, z,
+                      c->fsteps, (int)TP.fsteps.size(), fslots, c->bsteps,
+                      (int)TP.bsteps.size(), bslots, c->st);
   } else if (c->prof_sweeps) {
     (void)hipEventRecord(c->ev[6], c->stream);
   }
@@ -799,9 +811,10 @@ void msweeps_t(lapgpu_ctx* c, const double* rin, T* y, T* z, T* ext, const T* gF
     hipLaunchKernelGGL((k_mtail_ext<K, T>), dim3(grid_for(TP.NT * K, kBlock)), dim3(kBlock), 0,
                        c->stream, (int)TP.T0, (int)TP.NT, c->Eptr, c->Eidx, Ew, rin, y, ext,
                        m.flags);
-    hipLaunchKernelGGL((k_mtail<K, T, S>), dim3(K), dim3(kTailThreads), 0, c->stream,
-                       (int)TP.T0, (int)TP.NT, ext, sv, z, c->fsteps, (int)TP.fsteps.size(),
-                       fslots, c->bsteps, (int)TP.bsteps.size(), bslots, m.st, m.flags);
+    LAPGPU_DISPATCH_E(TP.elog, hipLaunchKernelGGL, (k_mtail<K, T, S, E_>), dim3(K),
+                      dim3(kTailThreads), 0, c->stream, (int)TP.T0, (int)TP.NT, ext, sv, z,
+                      c->fsteps, (int)TP.fsteps.size(), fslots, c->bsteps,
+                      (int)TP.bsteps.size(), bslots, m.st, m.flags);
   }
   for (auto it = c->groups.rbegin(); it != c->groups.rend(); ++it)
     hipLaunchKernelGGL((k_mgroup<K, T, T>),

@@ -203,7 +203,7 @@ int lapgpu_batch_add_graph(lapgpu_batch* B, int64_t n, const int64_t* colptr,
   Plan P;
   if (!build_plan(L, nnzF, col, fcolptr, frowval, fval, d, index_base, 0, P, err))
     return berr(B, -1, err);
-  if (!build_tail(P, INT64_MAX, INT64_MAX) || P.tail.T0 != 0)
+  if (!build_tail(P, INT64_MAX, INT64_MAX, kWgElog, kWgThreads) || P.tail.T0 != 0)
     return berr(B, -1, "lapgpu_batch_add_graph: could not build the slot program");
   GraphStore G;
   G.n = (int)n;

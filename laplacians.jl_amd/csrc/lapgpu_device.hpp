@@ -10,7 +10,10 @@ namespace lapgpu {
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
 constexpr int kRedBlocks = 1024;  // fixed grid of the reduction kernels
-constexpr int kTailThreads = 1024;
+#ifndef LAPGPU_TAIL_THREADS
+#define LAPGPU_TAIL_THREADS 512
+#endif
+constexpr int kTailThreads = LAPGPU_TAIL_THREADS;  // = kTailLanes of the plan
 constexpr double kEps = 2.220446049250313e-16;  // eps(Float64)
 
 // partial-sum slots (each kRedBlocks doubles)
@@ -98,18 +101,20 @@ __device__ __forceinline__ double wave_total(const double* part, int slot) {
 // backward sweep of those levels with one workgroup barrier per level; the
 // batched solver runs whole graphs the same way.  The work of each level is
 // a host-packed "slot program" (TailSlot / TailStep of lapgpu_plan.hpp):
-// lane t of step j computes w * y[src]; the lanes of an aligned
-// power-of-two segment are summed with DPP row shifts so that the segment's
-// LAST lane holds the sum, and that lane adds it to y[row].
+// lane t of step j sums E = 2^elog products w * y[src] (the step's planes),
+// the lanes of an aligned power-of-two segment are summed with DPP row
+// shifts so that the segment's LAST lane holds the sum, and that lane adds
+// it to y[row].  Several entries per lane cut the cross-lane stages, which
+// are the VALU cost of a step, and the lanes a level needs.
 //
-// Latency: every slot load is unconditional (inactive lanes re-read a valid
-// slot of the same step), so the compiler can wait on exactly the loads it
-// needs; slots are loaded D steps ahead, and the step table 64 steps ahead
-// with one vector load per lane, read out with readlane (no scalar loads in
-// the loop, whose counter is shared with LDS).
+// Latency: every slot load is unconditional (inactive lanes re-read a
+// valid slot of the same step), so the compiler can wait on exactly the
+// loads it needs; slots are loaded D steps ahead, and the step table 64
+// steps ahead with one vector load per lane, read out with readlane (no
+// scalar loads in the loop, whose counter is shared with LDS).  A step's
+// largest segment bounds the DPP stages run.
 
 constexpr int kTailCapV = 18432;
-constexpr int kTailDepth = 8;
 
 struct alignas(16) TailSlotD {
   double w;
@@ -120,7 +125,7 @@ struct alignas(4) TailSlotF {
   unsigned src, meta;
 };
 
-// off = first slot; meta = count (active lanes, a multiple of 64) | end << 16
+// off = first slot; meta = count | end << 16 | elog << 17 | maxlog << 19
 struct alignas(8) TailStepD {
   int off, meta;
 };
@@ -131,10 +136,18 @@ __device__ __forceinline__ int2 load_step(const TailStepD* __restrict__ steps, i
   return j < nsteps ? v : make_int2(0, 0);
 }
 
-template <typename S>
-__device__ __forceinline__ S load_slot(const S* __restrict__ slots, int off, int count, int t) {
-  const int i = off + (t < count ? t : (count > 0 ? count - 1 : 0));
-  return slots[i];
+// lane t's EM planes of the step (off, meta), addressed from a wave-uniform
+// base per plane; lanes >= count re-read a valid slot of the same step
+template <int EM, typename S>
+__device__ __forceinline__ void load_planes(S (&r)[EM], const S* __restrict__ slots, int off,
+                                            int meta, int t) {
+  const int cnt = meta & 0xffff;
+  const unsigned tt = min((unsigned)t, (unsigned)max(cnt - 1, 0));
+#pragma unroll
+  for (int e = 0; e < EM; ++e) {
+    const S* __restrict__ base = slots + (off + e * cnt);
+    r[e] = base[tt];
+  }
 }
 
 template <int CTRL, int ROWS>
@@ -153,35 +166,61 @@ __device__ __forceinline__ float dpp_mov(float v) {
 // receives the segment's sum (other lanes hold partial sums).  Row shifts
 // 1, 2, 4, 8 inside each 16-lane row, then row_bcast:15 (lane 15 of a row
 // into the next row, rows 1 and 3) and row_bcast:31 (lane 31 into rows 2
-// and 3); lanes a DPP move does not write read 0.
+// and 3); lanes a DPP move does not write read 0.  Stages at or above the
+// wave-uniform bound maxlog are skipped.
 template <typename Y>
-__device__ __forceinline__ Y seg_sum_last(Y v, int seglog) {
-  Y t = dpp_mov<0x111, 0xF>(v);  // row_shr:1
-  if (seglog >= 1) v += t;
-  t = dpp_mov<0x112, 0xF>(v);    // row_shr:2
-  if (seglog >= 2) v += t;
-  t = dpp_mov<0x114, 0xF>(v);    // row_shr:4
-  if (seglog >= 3) v += t;
-  t = dpp_mov<0x118, 0xF>(v);    // row_shr:8
-  if (seglog >= 4) v += t;
-  t = dpp_mov<0x142, 0xA>(v);    // row_bcast:15
-  if (seglog >= 5) v += t;
-  t = dpp_mov<0x143, 0xC>(v);    // row_bcast:31
-  if (seglog >= 6) v += t;
+__device__ __forceinline__ Y seg_sum_last(Y v, int seglog, int maxlog) {
+  Y t;
+  if (maxlog >= 1) {
+    t = dpp_mov<0x111, 0xF>(v);  // row_shr:1
+    if (seglog >= 1) v += t;
+  }
+  if (maxlog >= 2) {
+    t = dpp_mov<0x112, 0xF>(v);  // row_shr:2
+    if (seglog >= 2) v += t;
+  }
+  if (maxlog >= 3) {
+    t = dpp_mov<0x114, 0xF>(v);  // row_shr:4
+    if (seglog >= 3) v += t;
+  }
+  if (maxlog >= 4) {
+    t = dpp_mov<0x118, 0xF>(v);  // row_shr:8
+    if (seglog >= 4) v += t;
+  }
+  if (maxlog >= 5) {
+    t = dpp_mov<0x142, 0xA>(v);  // row_bcast:15
+    if (seglog >= 5) v += t;
+  }
+  if (maxlog >= 6) {
+    t = dpp_mov<0x143, 0xC>(v);  // row_bcast:31
+    if (seglog >= 6) v += t;
+  }
   return v;
 }
 
-template <typename S, typename Y>
-__device__ __forceinline__ void run_slot(const S& sl, int count, Y* ly) {
+// one step for lane threadIdx.x: sum of its E products, segment sum, lead add
+template <bool LDSY, int EM, typename S, typename Y>
+__device__ __forceinline__ void run_step(const S (&r)[EM], int meta, Y* ly) {
+  const int count = meta & 0xffff;
   if ((int)threadIdx.x < count) {  // count is a multiple of 64: wave-uniform
-    Y v = sl.w * ly[sl.src];
-    v = seg_sum_last(v, (int)((sl.meta >> 26) & 7u));
-    const unsigned lead = sl.meta >> 29;
-    const unsigned row = sl.meta & ((1u << 26) - 1);
-    if (lead == 1)
+    Y v = r[0].w * ly[r[0].src];
+#pragma unroll
+    for (int e = 1; e < EM; ++e) v += r[e].w * ly[r[e].src];
+    v = seg_sum_last(v, (int)((r[0].meta >> 26) & 7u), (meta >> 19) & 7);
+    // in LDS the lead lanes add without waiting for a reply: a single-
+    // segment row has exactly one add per level (the same result as a
+    // read-modify-write), the chunks of a split row add in any order.  In
+    // global memory a single-segment row is read-modified-written, so the
+    // workgroup's later loads see it through its L1.
+    const unsigned lead = r[0].meta >> 29;
+    const unsigned row = r[0].meta & ((1u << 26) - 1);
+    if (LDSY) {
+      if (lead != 0) atomicAdd(&ly[row], v);
+    } else if (lead == 1) {
       ly[row] += v;
-    else if (lead == 2)
+    } else if (lead == 2) {
       atomicAdd(&ly[row], v);
+    }
   }
 }
 
@@ -194,35 +233,57 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 // Runs nsteps steps of a slot program on ly (in LDS when LDSY, else in
-// global memory), loading slots D steps ahead.
-template <int D, bool LDSY, typename S, typename Y>
+// global memory).  Every step of the program has E = EM entries per lane.
+// Slots are double-buffered D steps at a time in two register sets used in
+// turn (no copies between them): the loads of the next D steps are issued
+// before the current D steps run, so every wait counts exactly the loads
+// still needed.  2D divides 64, so a block of the step table is consumed
+// by whole iterations.
+template <int D, bool LDSY, int EM, typename S, typename Y>
 __device__ __forceinline__ void run_program(const TailStepD* __restrict__ steps, int nsteps,
                                             const S* __restrict__ slots, Y* ly) {
-  static_assert(64 % D == 0, "D divides 64");
+  static_assert(64 % (2 * D) == 0, "2D divides 64");
   const int t = threadIdx.x;
   const int lane = t & 63;
+  // step table: blkA = steps [base, base + 64), blkB = the next 64
   int2 blkA = load_step(steps, nsteps, lane);
   int2 blkB = load_step(steps, nsteps, 64 + lane);
-  S a[D], b[D];
+  S a[D][EM], b[D][EM];
 #pragma unroll
   for (int d = 0; d < D; ++d)
-    a[d] = load_slot(slots, __builtin_amdgcn_readlane(blkA.x, d),
-                     __builtin_amdgcn_readlane(blkA.y, d) & 0xffff, t);
-  for (int c = 0; c < nsteps; c += D) {
-    const int cn = c + D;
-    const bool nb = (cn & 63) == 0;
+    load_planes(a[d], slots, __builtin_amdgcn_readlane(blkA.x, d),
+                __builtin_amdgcn_readlane(blkA.y, d), t);
+  for (int c = 0; c < nsteps; c += 2 * D) {
+    const int i0 = c & 63;  // steps c .. c+2D-1 are blkA[i0 .. i0+2D)
+    // steps c+2D .. c+3D-1: blkA, or blkB when c+2D starts a new block
+    const bool nb = i0 + 2 * D == 64;
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+      load_planes(b[d], slots, __builtin_amdgcn_readlane(blkA.x, i0 + D + d),
+                  __builtin_amdgcn_readlane(blkA.y, i0 + D + d), t);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const int idx = (cn + d) & 63;
-      const int off = nb ? __builtin_amdgcn_readlane(blkB.x, idx) : __builtin_amdgcn_readlane(blkA.x, idx);
-      const int meta = nb ? __builtin_amdgcn_readlane(blkB.y, idx) : __builtin_amdgcn_readlane(blkA.y, idx);
-      b[d] = load_slot(slots, off, meta & 0xffff, t);
+      const int meta = __builtin_amdgcn_readlane(blkA.y, i0 + d);
+      run_step<LDSY>(a[d], meta, ly);
+      if ((meta >> 16) & 1) {
+        if (LDSY)
+          lds_barrier();
+        else
+          __syncthreads();
+      }
     }
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const int meta = __builtin_amdgcn_readlane(blkA.y, (c + d) & 63);
-      run_slot(a[d], meta & 0xffff, ly);
-      if (meta >> 16) {
+      const int idx = (i0 + 2 * D + d) & 63;
+      const int off = nb ? __builtin_amdgcn_readlane(blkB.x, idx) : __builtin_amdgcn_readlane(blkA.x, idx);
+      const int meta = nb ? __builtin_amdgcn_readlane(blkB.y, idx) : __builtin_amdgcn_readlane(blkA.y, idx);
+      load_planes(a[d], slots, off, meta, t);
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int meta = __builtin_amdgcn_readlane(blkA.y, i0 + D + d);
+      run_step<LDSY>(b[d], meta, ly);
+      if ((meta >> 16) & 1) {
         if (LDSY)
           lds_barrier();
         else
@@ -231,11 +292,16 @@ __device__ __forceinline__ void run_program(const TailStepD* __restrict__ steps,
     }
     if (nb) {
       blkA = blkB;
-      blkB = load_step(steps, nsteps, cn + 64 + lane);
+      blkB = load_step(steps, nsteps, c + 2 * D + 64 + lane);
     }
-#pragma unroll
-    for (int d = 0; d < D; ++d) a[d] = b[d];
   }
+}
+
+// Steps per register set for E entries per lane (2 D E slots per lane in
+// registers).
+template <int E>
+__host__ __device__ constexpr int tail_depth() {
+  return (kTailThreads <= 256 ? 16 : 8) / E;
 }
 
 }  // namespace lapgpu

@@ -268,8 +268,9 @@ k_tail_ext(int T0, int NT, const int* __restrict__ Eptr,
   if (lane == 0 && !st->stopped) ext[row] = (T)r[T0 + row] + acc;
 }
 
-// forward + diagonal + backward on the tail; writes yF of the tail to y
-template <typename T, typename S>
+// forward + diagonal + backward on the tail; writes yF of the tail to y.
+// E: entries per lane of the tail's slot programs.
+template <typename T, typename S, int E>
 __global__ void __launch_bounds__(kTailThreads)
 k_tail(int T0, int NT, const T* __restrict__ ext, const T* __restrict__ s,
        T* __restrict__ yF, const TailStepD* __restrict__ fsteps, int nfsteps,
@@ -280,11 +281,11 @@ k_tail(int T0, int NT, const T* __restrict__ ext, const T* __restrict__ s,
   if (st->stopped) return;
   for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] = ext[i];
   __syncthreads();
-  run_program<kTailDepth, true>(fsteps, nfsteps, fslots, ly);
+  run_program<tail_depth<E>(), true, E>(fsteps, nfsteps, fslots, ly);
   __syncthreads();
   for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] *= s[T0 + i];
   __syncthreads();
-  run_program<kTailDepth, true>(bsteps, nbsteps, bslots, ly);
+  run_program<tail_depth<E>(), true, E>(bsteps, nbsteps, bslots, ly);
   __syncthreads();
   for (int i = threadIdx.x; i < NT; i += kTailThreads) yF[T0 + i] = ly[i];
 }

@@ -265,7 +265,7 @@ k_mtail_ext(int T0, int NT, const int* __restrict__ Eptr, const int* __restrict_
 // The tail for rhs j = blockIdx.x: the single-RHS slot programs on an
 // LDS-resident copy of column j (as k_tail); K workgroups run side by side
 // on K CUs.  Columns that have stopped are skipped.
-template <int K, typename T, typename S>
+template <int K, typename T, typename S, int E>
 __global__ void __launch_bounds__(kTailThreads)
 k_mtail(int T0, int NT, const T* __restrict__ ext, const T* __restrict__ s, T* __restrict__ z,
         const TailStepD* __restrict__ fsteps, int nfsteps, const S* __restrict__ fslots,
@@ -276,11 +276,11 @@ k_mtail(int T0, int NT, const T* __restrict__ ext, const T* __restrict__ s, T* _
   if (flags[0] || mst[j].stopped) return;
   for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] = ext[(int64_t)i * K + j];
   __syncthreads();
-  run_program<kTailDepth, true>(fsteps, nfsteps, fslots, ly);
+  run_program<tail_depth<E>(), true, E>(fsteps, nfsteps, fslots, ly);
   __syncthreads();
   for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] *= s[T0 + i];
   __syncthreads();
-  run_program<kTailDepth, true>(bsteps, nbsteps, bslots, ly);
+  run_program<tail_depth<E>(), true, E>(bsteps, nbsteps, bslots, ly);
   __syncthreads();
   for (int i = threadIdx.x; i < NT; i += kTailThreads) z[(int64_t)(T0 + i) * K + j] = ly[i];
 }

@@ -83,11 +83,24 @@ int log2_exact(int64_t v) {
   return l;
 }
 
-// Lanes one row takes: chunks of at most 64 entries, each in an aligned
-// power-of-two segment.
-int64_t row_lanes(int64_t dg) {
+// Lanes one row of dg entries takes at E = 2^elog entries per lane: chunks
+// of at most 64 lanes (64 E entries), each in an aligned power-of-two
+// segment.
+int64_t row_lanes(int64_t dg, int elog) {
   if (dg == 0) return 0;
-  return 64 * (dg / 64) + ((dg % 64) ? pow2_at_least(dg % 64) : 0);
+  const int64_t E = int64_t(1) << elog, per = 64 * E;
+  const int64_t rem = dg % per;
+  return 64 * (dg / per) + (rem ? pow2_at_least((rem + E - 1) / E) : 0);
+}
+
+// Entries per lane of a program from the mean length of its non-empty
+// rows: more entries per lane for longer rows, so that a long row takes
+// fewer lanes and fewer cross-lane reduction stages.
+int choose_elog(int64_t rows, int64_t ent, int max_elog) {
+  if (rows == 0) return 0;
+  const double avg = (double)ent / (double)rows;
+  int e = avg >= 16.0 ? 2 : avg >= 8.0 ? 1 : 0;
+  return std::min(e, max_elog);
 }
 
 struct RowEntries {
@@ -95,13 +108,16 @@ struct RowEntries {
   std::vector<std::pair<int32_t, double>> e;  // (local src, weight)
 };
 
-// Pack one level's rows into steps of kTailLanes lanes (see TailSlot).  A row
-// with at most 64 entries is one segment whose lead lane adds its sum to
-// y[row] (lead = 1); a longer row is split into chunks whose leads add with
-// an LDS atomic (lead = 2).  Segments are placed largest first with a bump
-// pointer per step, which keeps every segment aligned to its size.
-void pack_level(std::vector<RowEntries>& rows, std::vector<TailStep>& steps,
-                std::vector<TailSlot>& slots) {
+// Pack one level's rows into steps of `lanes` lanes (see TailSlot), E =
+// 2^elog entries per lane.  A row of at most 64 E entries is one segment
+// whose lead lane adds its sum to y[row] (lead = 1); a longer row is split
+// into chunks whose leads add with an LDS atomic (lead = 2).  Lane j of a
+// segment takes the chunk's entries j E .. j E + E - 1.  Segments are
+// placed largest first with a bump pointer per step, which keeps every
+// segment aligned to its size.
+void pack_level(std::vector<RowEntries>& rows, int elog, int64_t lanes,
+                std::vector<TailStep>& steps, std::vector<TailSlot>& slots) {
+  const int64_t E = int64_t(1) << elog;
   struct Chunk {
     int32_t row;
     int64_t size;  // segment lanes
@@ -113,107 +129,128 @@ void pack_level(std::vector<RowEntries>& rows, std::vector<TailStep>& steps,
   for (const RowEntries& R : rows) {
     const int64_t dg = (int64_t)R.e.size();
     if (dg == 0) continue;
-    const bool multi = dg > 64;
-    for (int64_t k0 = 0; k0 < dg; k0 += 64) {
-      const int64_t k1 = std::min(dg, k0 + 64);
-      chunks.push_back({R.row, pow2_at_least(k1 - k0), k0, k1, &R, multi});
+    const bool multi = dg > 64 * E;
+    for (int64_t k0 = 0; k0 < dg; k0 += 64 * E) {
+      const int64_t k1 = std::min(dg, k0 + 64 * E);
+      chunks.push_back({R.row, pow2_at_least((k1 - k0 + E - 1) / E), k0, k1, &R, multi});
     }
   }
   std::stable_sort(chunks.begin(), chunks.end(),
                    [](const Chunk& a, const Chunk& b) { return a.size > b.size; });
+  // pass p: E planes of kTailLanes slots, plane-major
   std::vector<std::vector<TailSlot>> pass;
   std::vector<int64_t> ptr;
+  std::vector<int> maxlog;
   for (const Chunk& C : chunks) {
     size_t p = 0;
-    while (p < pass.size() && ptr[p] + C.size > kTailLanes) ++p;
+    while (p < pass.size() && ptr[p] + C.size > lanes) ++p;
     if (p == pass.size()) {
-      pass.emplace_back(kTailLanes, TailSlot{0.0, 0, 0});
+      pass.emplace_back(E * lanes, TailSlot{0.0, 0, 0});
       ptr.push_back(0);
+      maxlog.push_back(0);
     }
     const int64_t off = ptr[p];
     ptr[p] += C.size;
+    const int lg = log2_exact(C.size);
+    maxlog[p] = std::max(maxlog[p], lg);
     for (int64_t t = 0; t < C.size; ++t) {
-      TailSlot& S = pass[p][off + t];
-      S.meta = slot_meta((uint32_t)C.row, (uint32_t)log2_exact(C.size),
+      TailSlot& M = pass[p][off + t];
+      M.meta = slot_meta((uint32_t)C.row, (uint32_t)lg,
                          t == C.size - 1 ? (C.atomic ? 2u : 1u) : 0u);
-      if (C.k0 + t < C.k1) {
-        S.src = (uint32_t)C.R->e[C.k0 + t].first;
-        S.w = C.R->e[C.k0 + t].second;
+      for (int64_t e = 0; e < E; ++e) {
+        const int64_t k = C.k0 + t * E + e;
+        if (k < C.k1) {
+          TailSlot& S = pass[p][e * lanes + off + t];
+          S.src = (uint32_t)C.R->e[k].first;
+          S.w = C.R->e[k].second;
+        }
       }
     }
   }
   for (size_t p = 0; p < pass.size(); ++p) {
     const int64_t cnt = (ptr[p] + 63) / 64 * 64;
-    TailStep st{(int32_t)slots.size(), (int32_t)(cnt | ((p + 1 == pass.size() ? 1 : 0) << 16))};
-    steps.push_back(st);
-    slots.insert(slots.end(), pass[p].begin(), pass[p].begin() + cnt);
+    steps.push_back(TailStep{(int32_t)slots.size(),
+                             step_meta(cnt, p + 1 == pass.size(), elog, maxlog[p])});
+    for (int64_t e = 0; e < E; ++e)
+      slots.insert(slots.end(), pass[p].begin() + e * lanes,
+                   pass[p].begin() + e * lanes + cnt);
   }
 }
 
 }  // namespace
 
-bool build_tail(Plan& P, int64_t max_steps, int64_t cap) {
+bool build_tail(Plan& P, int64_t max_steps, int64_t cap, int max_elog, int lanes) {
   const int64_t n = P.n;
-    int64_t t = P.nlev, nt = 0;
-    while (t > 0) {
-      const int64_t l = t - 1;
-      const int64_t w = P.lvl[l + 1] - P.lvl[l];
-      if (nt + w > cap) break;
-      int64_t fl = 0, bl = 0;
-      for (int64_t pos = P.lvl[l]; pos < P.lvl[l + 1]; ++pos) {
-        fl += row_lanes(P.Fptr[pos + 1] - P.Fptr[pos]);
-        bl += row_lanes(P.Bptr[pos + 1] - P.Bptr[pos]);
-      }
-      // bump packing of power-of-two segments in decreasing size fills
-      // whole steps, so lanes / kTailLanes rounded up is the step count
-      const int64_t steps_l = (std::max(fl, bl) + kTailLanes - 1) / kTailLanes;
-      if (steps_l > max_steps) break;
-      nt += w;
-      t = l;
+  // extent: the longest suffix of levels within cap whose levels need at
+  // most max_steps steps of kExtentLanes lanes at one entry per lane
+  int64_t t = P.nlev, nt = 0;
+  while (t > 0) {
+    const int64_t l = t - 1;
+    const int64_t w = P.lvl[l + 1] - P.lvl[l];
+    if (nt + w > cap) break;
+    int64_t fl = 0, bl = 0;
+    for (int64_t pos = P.lvl[l]; pos < P.lvl[l + 1]; ++pos) {
+      fl += row_lanes(P.Fptr[pos + 1] - P.Fptr[pos], 0);
+      bl += row_lanes(P.Bptr[pos + 1] - P.Bptr[pos], 0);
     }
-    TailProgram& T = P.tail;
-    T = TailProgram();
-    P.tail_level = t;
-    T.T0 = P.lvl[t];
-    T.NT = n - T.T0;
-    const int32_t T0 = (int32_t)T.T0;
-    T.Eptr.assign(T.NT + 1, 0);
-    for (int64_t i = 0; i < T.NT; ++i) {
-      const int64_t pos = T0 + i;
-      for (int32_t k = P.Fptr[pos]; k < P.Fptr[pos + 1]; ++k)
-        if (P.Fidx[k] < T0) {
-          T.Eidx.push_back(P.Fidx[k]);
-          T.Ew.push_back(P.Fw[k]);
-        }
-      T.Eptr[i + 1] = (int32_t)T.Eidx.size();
-    }
-    bool ok = true;
-    for (int64_t l = t; l < P.nlev; ++l) {
-      std::vector<RowEntries> rows;
-      for (int64_t pos = P.lvl[l]; pos < P.lvl[l + 1]; ++pos) {
-        RowEntries R;
-        R.row = (int32_t)(pos - T0);
-        for (int32_t k = P.Fptr[pos]; k < P.Fptr[pos + 1]; ++k)
-          if (P.Fidx[k] >= T0) R.e.emplace_back(P.Fidx[k] - T0, P.Fw[k]);
-        rows.push_back(std::move(R));
-      }
-      pack_level(rows, T.fsteps, T.fslots);
-    }
-    for (int64_t l = P.nlev - 1; l >= t && ok; --l) {
-      std::vector<RowEntries> rows;
-      for (int64_t pos = P.lvl[l]; pos < P.lvl[l + 1]; ++pos) {
-        RowEntries R;
-        R.row = (int32_t)(pos - T0);
-        for (int32_t k = P.Bptr[pos]; k < P.Bptr[pos + 1]; ++k) {
-          if (P.Bidx[k] < T0) ok = false;
-          R.e.emplace_back(P.Bidx[k] - T0, P.Bw[k]);
-        }
-        rows.push_back(std::move(R));
-      }
-      pack_level(rows, T.bsteps, T.bslots);
-    }
-    return ok;
+    // bump packing of power-of-two segments in decreasing size fills
+    // whole steps, so lanes / kTailLanes rounded up is the step count
+    const int64_t steps_l = (std::max(fl, bl) + kExtentLanes - 1) / kExtentLanes;
+    if (steps_l > max_steps) break;
+    nt += w;
+    t = l;
   }
+  TailProgram& T = P.tail;
+  T = TailProgram();
+  P.tail_level = t;
+  T.T0 = P.lvl[t];
+  T.NT = n - T.T0;
+  const int32_t T0 = (int32_t)T.T0;
+  T.Eptr.assign(T.NT + 1, 0);
+  int64_t rows = 0, ent = 0;
+  for (int64_t i = 0; i < T.NT; ++i) {
+    const int64_t pos = T0 + i;
+    int64_t fin = 0;
+    for (int32_t k = P.Fptr[pos]; k < P.Fptr[pos + 1]; ++k)
+      if (P.Fidx[k] < T0) {
+        T.Eidx.push_back(P.Fidx[k]);
+        T.Ew.push_back(P.Fw[k]);
+      } else {
+        ++fin;
+      }
+    T.Eptr[i + 1] = (int32_t)T.Eidx.size();
+    const int64_t bin = P.Bptr[pos + 1] - P.Bptr[pos];
+    rows += (fin > 0) + (bin > 0);
+    ent += fin + bin;
+  }
+  T.elog = choose_elog(rows, ent, max_elog);
+  bool ok = true;
+  for (int64_t l = t; l < P.nlev; ++l) {
+    std::vector<RowEntries> rws;
+    for (int64_t pos = P.lvl[l]; pos < P.lvl[l + 1]; ++pos) {
+      RowEntries R;
+      R.row = (int32_t)(pos - T0);
+      for (int32_t k = P.Fptr[pos]; k < P.Fptr[pos + 1]; ++k)
+        if (P.Fidx[k] >= T0) R.e.emplace_back(P.Fidx[k] - T0, P.Fw[k]);
+      rws.push_back(std::move(R));
+    }
+    pack_level(rws, T.elog, lanes, T.fsteps, T.fslots);
+  }
+  for (int64_t l = P.nlev - 1; l >= t && ok; --l) {
+    std::vector<RowEntries> rws;
+    for (int64_t pos = P.lvl[l]; pos < P.lvl[l + 1]; ++pos) {
+      RowEntries R;
+      R.row = (int32_t)(pos - T0);
+      for (int32_t k = P.Bptr[pos]; k < P.Bptr[pos + 1]; ++k) {
+        if (P.Bidx[k] < T0) ok = false;
+        R.e.emplace_back(P.Bidx[k] - T0, P.Bw[k]);
+      }
+      rws.push_back(std::move(R));
+    }
+    pack_level(rws, T.elog, lanes, T.bsteps, T.bslots);
+  }
+  return ok;
+}
 
 namespace {
 

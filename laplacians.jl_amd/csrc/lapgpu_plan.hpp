@@ -16,11 +16,13 @@ struct HostCsr {
   std::vector<double> val;
 };
 
-// One lane's work in a slot program: w * y[src] summed over an aligned
-// segment of 2^seglog lanes; the segment's last lane adds the sum to y[row]
-// (lead = 1: plain add; lead = 2: atomic add, for rows split in chunks).
-// meta packs row (bits 0-25), seglog (bits 26-28) and lead (bits 29-30).
-// Indices are local to the program (position - T0).
+// One entry of a slot program: w * y[src].  A lane of a step sums up to
+// E = 2^elog entries (the step's "planes"), then the lanes of an aligned
+// segment of 2^seglog lanes are summed and the segment's last lane adds the
+// sum to y[row] (lead = 1: plain add; lead = 2: atomic add, for rows split
+// in chunks).  meta packs row (bits 0-25), seglog (bits 26-28) and lead
+// (bits 29-30); it is read from plane 0 only.  Indices are local to the
+// program (position - T0).
 struct TailSlot {
   double w;
   uint32_t src;
@@ -32,19 +34,35 @@ inline uint32_t slot_meta(uint32_t row, uint32_t seglog, uint32_t lead) {
 }
 static_assert(sizeof(TailSlot) == 16, "TailSlot is 16 bytes");
 
-// A step = up to kTailLanes slots run by the tail workgroup at once.
-// meta = count (active lanes, a multiple of 64) | end << 16, where `end`
-// marks the last step of a level (a workgroup barrier follows it).
+// A step = up to kTailLanes lanes run by the tail workgroup at once.  Its
+// slots are E planes of `count` slots each: lane t's entry e is slot
+// off + e * count + t.  meta = count (active lanes, a multiple of 64)
+// | end << 16 (the last step of a level: a workgroup barrier follows)
+// | elog << 17 (E = 2^elog, at most kTailMaxElog)
+// | maxlog << 19 (the largest seglog in the step).
 struct TailStep {
   int32_t off;   // first slot
   int32_t meta;
 };
+constexpr int kTailMaxElog = 2;
+inline int32_t step_meta(int64_t count, bool end, int elog, int maxlog) {
+  return (int32_t)(count | ((end ? 1 : 0) << 16) | (elog << 17) | (maxlog << 19));
+}
 
-constexpr int kTailLanes = 1024;
+// Lanes of one step of the tail's slot programs (the tail workgroup's
+// threads).  The whole-graph programs of the batched solver use their own
+// workgroup width.
+#ifndef LAPGPU_TAIL_THREADS
+#define LAPGPU_TAIL_THREADS 512
+#endif
+constexpr int kTailLanes = LAPGPU_TAIL_THREADS;
+// Lanes per step the extent rule of build_tail counts steps in.
+constexpr int kExtentLanes = 1024;
 constexpr int64_t kTailCap = 18432;  // tail vertices held in LDS (147 KiB)
 
 struct TailProgram {
   int64_t T0 = 0, NT = 0;
+  int elog = 0;  // every step of both sweeps has 2^elog entries per lane
   std::vector<TailStep> fsteps, bsteps;
   std::vector<TailSlot> fslots, bslots;
   // forward in-edges of tail rows from outside the tail (CSR over tail rows)
@@ -108,7 +126,9 @@ bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
 // (Re)build P.tail as the longest suffix of levels with at most `cap`
 // vertices whose levels need at most `max_steps` steps each.  With both
 // limits at INT64_MAX the program covers the whole sweep (tail_level 0).
-bool build_tail(Plan& P, int64_t max_steps, int64_t cap);
+// Entries per lane are chosen for the whole program, at most 2^max_elog.
+bool build_tail(Plan& P, int64_t max_steps, int64_t cap, int max_elog = kTailMaxElog,
+                int lanes = kTailLanes);
 
 // Group the levels before the tail into launches of up to `max_depth`
 // levels while a group's composite rows stay within `growth` times the

@@ -16,6 +16,10 @@ namespace lapgpu {
 
 constexpr int kWgThreads = 1024;
 constexpr int kWgG = 8;  // lanes per SpMV row
+// entries per lane of the whole-graph slot programs (their max_elog is
+// kWgElog): one, so the solver keeps its registers for the rest
+constexpr int kWgElog = 0;
+constexpr int kWgPlanes = 1 << kWgElog;
 
 struct WgGraphD {
   int n, nf, nb, pad;
@@ -73,11 +77,11 @@ __device__ __forceinline__ void wg_ldl(const WgGraphD& G, const double* r, Y* y)
   const int n = G.n;
   for (int i = threadIdx.x; i < n; i += kWgThreads) y[i] = r[i];
   __syncthreads();
-  run_program<4, LDSY>(G.fsteps, G.nf, G.fslots, y);
+  run_program<4, LDSY, kWgPlanes>(G.fsteps, G.nf, G.fslots, y);
   __syncthreads();
   for (int i = threadIdx.x; i < n; i += kWgThreads) y[i] *= G.s[i];
   __syncthreads();
-  run_program<4, LDSY>(G.bsteps, G.nb, G.bslots, y);
+  run_program<4, LDSY, kWgPlanes>(G.bsteps, G.nb, G.bslots, y);
   __syncthreads();
 }
 
