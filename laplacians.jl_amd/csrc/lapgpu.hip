@@ -62,10 +62,14 @@ struct lapgpu_ctx {
     int lo, hi;
     int Gf, nhf, hoff_f;
     int Gb, nhb, hoff_b;
+    // multi-RHS: entry slices per row (before the 64/K bound) and the rows
+    // longer than kMHeavyLen, run one workgroup per row
+    int Sf, mnhf, mhoff_f;
+    int Sb, mnhb, mhoff_b;
   };
   std::vector<GroupLaunch> groups;
   int *gFptr = nullptr, *gFidx = nullptr, *gBptr = nullptr, *gBidx = nullptr,
-      *heavy = nullptr;
+      *heavy = nullptr, *mheavy = nullptr;
   double *gFw = nullptr, *gBw = nullptr;
   float *gFw32 = nullptr, *gBw32 = nullptr, *z32 = nullptr;
   double* zF = nullptr;  // backward output yF (double path)
@@ -365,7 +369,7 @@ int free_graph(lapgpu_ctx* c) {
   c->fsteps = c->bsteps = nullptr;
   c->fslots = c->bslots = nullptr;
   int* ip[] = {c->perm, c->Lptr, c->Lidx, c->lvl, c->gFptr, c->gFidx, c->gBptr, c->gBidx,
-               c->heavy};
+               c->heavy, c->mheavy};
   for (int* a : ip)
     if (a) (void)hipFree(a);
   double* dp[] = {c->Lval, c->s, c->b, c->r, c->x, c->bestx, c->p, c->q, c->y, c->z, c->io,
@@ -373,7 +377,7 @@ int free_graph(lapgpu_ctx* c) {
   for (double* a : dp)
     if (a) (void)hipFree(a);
   c->perm = c->Lptr = c->Lidx = c->lvl = nullptr;
-  c->gFptr = c->gFidx = c->gBptr = c->gBidx = c->heavy = nullptr;
+  c->gFptr = c->gFidx = c->gBptr = c->gBidx = c->heavy = c->mheavy = nullptr;
   c->Lval = c->s = c->b = c->r = c->x = c->bestx = c->p = c->q = c->y = c->z = c->io = nullptr;
   c->gFw = c->gBw = c->zF = nullptr;
   c->groups.clear();
@@ -422,7 +426,26 @@ int upload_groups(lapgpu_ctx* c) {
   build_blocks(P, c->block_depth, c->block_growth, c->block_extra);
   const LevelBlocks& B = P.blocks;
   c->groups.clear();
-  std::vector<int32_t> heavy;
+  std::vector<int32_t> heavy, mheavy;
+  // multi-RHS light rows: slices so that a lane sums at most ~8 entries
+  auto plan_multi = [&](const std::vector<int32_t>& ptr, int lo, int hi, int& S, int& nh,
+                        int& hoff) {
+    int64_t rows = 0, ent = 0;
+    hoff = (int)mheavy.size();
+    for (int v = lo; v < hi; ++v) {
+      const int64_t len = ptr[v + 1] - ptr[v];
+      if (len > kMHeavyLen) {
+        mheavy.push_back(v);
+      } else {
+        ++rows;
+        ent += len;
+      }
+    }
+    nh = (int)mheavy.size() - hoff;
+    const double avg = rows ? (double)ent / (double)rows : 1.0;
+    S = 1;
+    while (S < 64 && (double)S * 8.0 < avg) S *= 2;
+  };
   auto plan_sweep = [&](const std::vector<int32_t>& ptr, int lo, int hi, int& G, int& nh,
                         int& hoff) {
     int64_t light_rows = 0, light_ent = 0;
@@ -450,6 +473,8 @@ int upload_groups(lapgpu_ctx* c) {
     if (gl.hi <= gl.lo) continue;
     plan_sweep(B.fptr, gl.lo, gl.hi, gl.Gf, gl.nhf, gl.hoff_f);
     plan_sweep(B.bptr, gl.lo, gl.hi, gl.Gb, gl.nhb, gl.hoff_b);
+    plan_multi(B.fptr, gl.lo, gl.hi, gl.Sf, gl.mnhf, gl.mhoff_f);
+    plan_multi(B.bptr, gl.lo, gl.hi, gl.Sb, gl.mnhb, gl.mhoff_b);
     c->groups.push_back(gl);
   }
   int rc;
@@ -460,6 +485,7 @@ int upload_groups(lapgpu_ctx* c) {
   if ((rc = dev_upload(c, &c->gBidx, B.bidx))) return rc;
   if ((rc = dev_upload(c, &c->gBw, B.bw))) return rc;
   if ((rc = dev_upload(c, &c->heavy, heavy))) return rc;
+  if ((rc = dev_upload(c, &c->mheavy, mheavy))) return rc;
   return 0;
 }
 
@@ -793,6 +819,43 @@ int multi_prepare(lapgpu_ctx* c, int K) {
   return 0;
 }
 
+// One level group for K right-hand sides: light rows with S entry slices
+// per row (S bounded by 64/K), rows longer than kMHeavyLen one workgroup
+// each.
+template <int K, typename T, typename TB>
+void launch_mgroup(lapgpu_ctx* c, int lo, int hi, int S, const int* heavy, int nh,
+                   const int* ptr, const int* idx, const T* w, const T* A, const TB* B,
+                   T* out) {
+  auto& m = c->m;
+  const int n = (int)c->n;
+  const int s = std::min(S, kWave / K);
+  const int64_t threads = (int64_t)(hi - lo) * K * s;
+  switch (s) {
+    case 1:
+      hipLaunchKernelGGL((k_mgroup<K, 1, T, TB>), dim3(grid_for(threads, kBlock)), dim3(kBlock),
+                         0, c->stream, n, lo, hi, ptr, idx, w, A, B, out, m.flags);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_mgroup<K, (K <= 32 ? 2 : 1), T, TB>),
+                         dim3(grid_for(threads, kBlock)), dim3(kBlock), 0, c->stream, n, lo, hi,
+                         ptr, idx, w, A, B, out, m.flags);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_mgroup<K, (K <= 16 ? 4 : 1), T, TB>),
+                         dim3(grid_for(threads, kBlock)), dim3(kBlock), 0, c->stream, n, lo, hi,
+                         ptr, idx, w, A, B, out, m.flags);
+      break;
+    default:
+      hipLaunchKernelGGL((k_mgroup<K, (K <= 8 ? 8 : 1), T, TB>),
+                         dim3(grid_for(threads, kBlock)), dim3(kBlock), 0, c->stream, n, lo, hi,
+                         ptr, idx, w, A, B, out, m.flags);
+      break;
+  }
+  if (nh > 0)
+    hipLaunchKernelGGL((k_mgroup_heavy<K, T, TB>), dim3(nh), dim3(kMHeavyThreads), 0, c->stream,
+                       n, heavy, ptr, idx, w, A, B, out, m.flags);
+}
+
 // z = LDLsolver(rin) without the mean removal for K interleaved right-hand
 // sides: forward groups, the tail (one workgroup per column), backward
 // groups.  T, S: sweep precision and tail slot type.
@@ -803,9 +866,8 @@ void msweeps_t(lapgpu_ctx* c, const double* rin, T* y, T* z, T* ext, const T* gF
   const Plan& P = c->plan;
   const int n = (int)c->n;
   for (const auto& g : c->groups)
-    hipLaunchKernelGGL((k_mgroup<K, T, double>),
-                       dim3(grid_for((int64_t)(g.hi - g.lo) * K, kBlock)), dim3(kBlock), 0,
-                       c->stream, n, g.lo, g.hi, c->gFptr, c->gFidx, gFw, y, rin, y, m.flags);
+    launch_mgroup<K, T, double>(c, g.lo, g.hi, g.Sf, c->mheavy + g.mhoff_f, g.mnhf, c->gFptr,
+                                c->gFidx, gFw, y, rin, y);
   const TailProgram& TP = P.tail;
   if (TP.NT > 0) {
     hipLaunchKernelGGL((k_mtail_ext<K, T>), dim3(grid_for(TP.NT * K, kBlock)), dim3(kBlock), 0,
@@ -817,9 +879,8 @@ void msweeps_t(lapgpu_ctx* c, const double* rin, T* y, T* z, T* ext, const T* gF
                       (int)TP.bsteps.size(), bslots, m.st, m.flags);
   }
   for (auto it = c->groups.rbegin(); it != c->groups.rend(); ++it)
-    hipLaunchKernelGGL((k_mgroup<K, T, T>),
-                       dim3(grid_for((int64_t)(it->hi - it->lo) * K, kBlock)), dim3(kBlock), 0,
-                       c->stream, n, it->lo, it->hi, c->gBptr, c->gBidx, gBw, z, y, z, m.flags);
+    launch_mgroup<K, T, T>(c, it->lo, it->hi, it->Sb, c->mheavy + it->mhoff_b, it->mnhb,
+                           c->gBptr, c->gBidx, gBw, z, y, z);
 }
 
 template <int K>

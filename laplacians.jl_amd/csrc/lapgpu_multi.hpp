@@ -224,24 +224,98 @@ __global__ void k_mscalar_nb(const double* __restrict__ part, DevState* mst, int
 
 // ---- the LDLinv apply --------------------------------------------------------
 
-// One level group (composite rows, as k_group), K lanes per row:
+// One level group (composite rows, as k_group) for K right-hand sides:
 //   out[v*K+j] = sum_e w[e] * (idx[e] < n ? A[idx[e]*K+j] : B[(idx[e]-n)*K+j])
 // forward: A = y0, B = r, out = y0; backward: A = yF, B = y0, out = yF.
-template <int K, typename T, typename TB>
+// Light rows (at most kMHeavyLen entries): a wave holds 64 / (K S) rows;
+// lane (row r, slice s, rhs j) sums the entries s, s + S, ... of row r, and
+// the S slices are added with shuffles (S = 1: one lane per rhs walks the
+// whole row, indices broadcast, see mrow_dot).  Longer rows are left to
+// k_mgroup_heavy.
+constexpr int kMHeavyLen = 64;
+constexpr int kMHeavyThreads = 1024;
+
+template <int K, int S, typename T, typename TB>
 __global__ void __launch_bounds__(kBlock)
 k_mgroup(int n, int lo, int hi, const int* __restrict__ ptr, const int* __restrict__ idx,
          const T* __restrict__ w, const T* A, const TB* B, T* out,
          const int* __restrict__ flags) {
   if (flags[0]) return;
-  constexpr int R = kWave / K;  // rows per wave
-  const int lane = threadIdx.x % kWave, j = lane % K, sub = lane / K;
-  const int row = lo + ((blockIdx.x * kBlock + threadIdx.x) / kWave) * R + sub;
-  if (row >= hi) return;  // whole row groups leave together
-  const int k0 = wave_uniform<K>(ptr[row]), k1 = wave_uniform<K>(ptr[row + 1]);
-  const T acc = mrow_dot<K, T>(k0, k1, j, sub, idx, w, [&](int i) -> T {
+  constexpr int R = kWave / (K * S);  // rows per wave
+  const int lane = threadIdx.x % kWave;
+  const int j = lane % K, s = (lane / K) % S, r = lane / (K * S);
+  const int row = lo + ((blockIdx.x * kBlock + threadIdx.x) / kWave) * R + r;
+  auto src = [&](int i) -> T {
     return i < n ? A[(int64_t)i * K + j] : (T)B[(int64_t)(i - n) * K + j];
-  });
-  out[(int64_t)row * K + j] = acc;
+  };
+  if constexpr (S == 1) {
+    if (row >= hi) return;  // whole row groups leave together
+    const int k0 = wave_uniform<K>(ptr[row]), k1 = wave_uniform<K>(ptr[row + 1]);
+    if (k1 - k0 > kMHeavyLen) return;
+    out[(int64_t)row * K + j] = mrow_dot<K, T>(k0, k1, j, r, idx, w, src);
+  } else {
+    T acc = 0;
+    bool mine = false;
+    if (row < hi) {
+      const int k0 = ptr[row], k1 = ptr[row + 1];
+      if (k1 - k0 <= kMHeavyLen) {
+        mine = true;
+        int k = k0 + s;
+        for (; k + 3 * S < k1; k += 4 * S) {
+          const int i0 = idx[k], i1 = idx[k + S], i2 = idx[k + 2 * S], i3 = idx[k + 3 * S];
+          const T w0 = w[k], w1 = w[k + S], w2 = w[k + 2 * S], w3 = w[k + 3 * S];
+          const T v0 = src(i0), v1 = src(i1), v2 = src(i2), v3 = src(i3);
+          acc += w0 * v0;
+          acc += w1 * v1;
+          acc += w2 * v2;
+          acc += w3 * v3;
+        }
+        for (; k < k1; k += S) acc += w[k] * src(idx[k]);
+      }
+    }
+#pragma unroll
+    for (int o = K; o < K * S; o <<= 1) acc += __shfl_xor(acc, o, kWave);
+    if (mine && s == 0) out[(int64_t)row * K + j] = acc;
+  }
+}
+
+// The rows of a group longer than kMHeavyLen, one workgroup per row: thread
+// (slice s, rhs j) sums the entries s, s + SL, ... (SL = kMHeavyThreads / K)
+// and the slices are added in a fixed tree order in LDS.
+template <int K, typename T, typename TB>
+__global__ void __launch_bounds__(kMHeavyThreads)
+k_mgroup_heavy(int n, const int* __restrict__ heavy, const int* __restrict__ ptr,
+               const int* __restrict__ idx, const T* __restrict__ w, const T* A, const TB* B,
+               T* out, const int* __restrict__ flags) {
+  if (flags[0]) return;
+  constexpr int SL = kMHeavyThreads / K;
+  __shared__ T red[SL][K];
+  const int row = heavy[blockIdx.x];
+  const int j = threadIdx.x % K, s = threadIdx.x / K;
+  const int k0 = ptr[row], k1 = ptr[row + 1];
+  auto src = [&](int i) -> T {
+    return i < n ? A[(int64_t)i * K + j] : (T)B[(int64_t)(i - n) * K + j];
+  };
+  T acc = 0;
+  int k = k0 + s;
+  for (; k + 3 * SL < k1; k += 4 * SL) {
+    const int i0 = idx[k], i1 = idx[k + SL], i2 = idx[k + 2 * SL], i3 = idx[k + 3 * SL];
+    const T w0 = w[k], w1 = w[k + SL], w2 = w[k + 2 * SL], w3 = w[k + 3 * SL];
+    const T v0 = src(i0), v1 = src(i1), v2 = src(i2), v3 = src(i3);
+    acc += w0 * v0;
+    acc += w1 * v1;
+    acc += w2 * v2;
+    acc += w3 * v3;
+  }
+  for (; k < k1; k += SL) acc += w[k] * src(idx[k]);
+  red[s][j] = acc;
+  __syncthreads();
+#pragma unroll
+  for (int h = SL / 2; h > 0; h >>= 1) {
+    if (s < h) red[s][j] += red[s + h][j];
+    __syncthreads();
+  }
+  if (s == 0) out[(int64_t)row * K + j] = red[0][j];
 }
 
 // ext[i*K+j] = r[(T0+i)*K+j] + the forward in-edges of tail row i from
