@@ -75,6 +75,15 @@ def algorithmic_bytes(n: int, nnzL: int, nnzF: int, sv: int = 8, si: int = 4) ->
     return {"spmv": spmv, "ldl": ldl, "vec": vec}
 
 
+def group_bytes(gi: dict, sv: int = 8, si: int = 4) -> int:
+    """Algorithmic bytes of the level-group launches of one iteration (both
+    sweeps): per composite-row entry its index and weight, per row its
+    pointer, one read of its input value (r or y0) and its write, and the
+    gathered values counted once per row they produce."""
+    rows = gi["rows"]
+    return (gi["fentries"] + gi["bentries"]) * (sv + si) + 2 * rows * (si + 3 * sv)
+
+
 def tail_bytes(ti: dict, sv: int = 8) -> int:
     """Algorithmic bytes of one k_tail launch: its slot programs (16 B per
     slot, 8 B per step), the tail inputs and scales read, yF written."""
@@ -117,7 +126,8 @@ def mode_plain(args) -> int:
     ctx.upload_ldli(f)
     info = ctx.factor_info()
     ti = ctx.tail_info()
-    log(f"schedule: {info} tail {ti}")
+    gi = ctx.group_info()
+    log(f"schedule: {info} tail {ti} groups {gi}")
 
     db = DeviceArray.from_host(b)
     dx = DeviceArray(n)
@@ -152,6 +162,9 @@ def mode_plain(args) -> int:
     ldl_gbs = byts["ldl"] / (tm["ldl_ms"] * 1e-3) / 1e9 if tm["ldl_ms"] > 0 else None
     tb = tail_bytes(ti)
     tail_gbs = tb / (st["tail_ms"] * 1e-3) / 1e9 if st["tail_ms"] > 0 else None
+    gb = group_bytes(gi)
+    g_ms = st["fgroups_ms"] + st["bgroups_ms"]
+    group_gbs = gb / (g_ms * 1e-3) / 1e9 if g_ms > 0 else None
     log(f"profile solve {prof}: {tm} {st} spmv {spmv_gbs} GB/s ldl {ldl_gbs} GB/s tail {tail_gbs} GB/s")
 
     cpu = None
@@ -211,14 +224,20 @@ def mode_plain(args) -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_tail (single-workgroup tail of the LDLinv sweeps; latency-bound by its dependent steps)",
-            "achieved": tail_gbs,
+            "kernel": "k_group (the level-group launches of both LDLinv sweeps, one iteration; "
+                      "latency-bound by dependent launches)",
+            "achieved": group_gbs,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": (tail_gbs / HBM_PEAK_GBS) if tail_gbs else None,
-            "traffic": read_traffic("k_tail", args.workload),
-            "bytes_per_launch": tb,
-            "avg_launch_ms": st["tail_ms"],
+            "frac": (group_gbs / HBM_PEAK_GBS) if group_gbs else None,
+            "traffic": read_traffic("k_group", args.workload),
+            "bytes_per_iteration": gb,
+            "launches_per_iteration": 2 * gi["ngroups"],
+            "ms_per_iteration": g_ms,
+            "tail": {"kernel": "k_tail", "achieved": tail_gbs,
+                     "frac": (tail_gbs / HBM_PEAK_GBS) if tail_gbs else None,
+                     "traffic": read_traffic("k_tail", args.workload), "bytes_per_launch": tb,
+                     "avg_launch_ms": st["tail_ms"]},
             "spmv": {"kernel": "k_spmv_dot", "achieved": spmv_gbs,
                      "frac": (spmv_gbs / HBM_PEAK_GBS) if spmv_gbs else None,
                      "traffic": read_traffic("k_spmv_dot", args.workload)},

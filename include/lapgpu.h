@@ -142,6 +142,13 @@ int lapgpu_factor_info(lapgpu_ctx* ctx, int64_t* nlevels, int64_t* nwide,
 int lapgpu_last_timings(lapgpu_ctx* ctx, double* spmv_ms, double* ldl_ms,
                         double* vec_ms, double* total_ms, int64_t* its);
 
+/* Size of the level groups of the uploaded factor (the levels before the
+ * tail): number of groups (launches per sweep), rows, and composite-row
+ * entries of the forward and backward sweeps.  Measurement only
+ * (algorithmic bytes of the group kernels). */
+int lapgpu_group_info(lapgpu_ctx* ctx, int64_t* ngroups, int64_t* rows, int64_t* fentries,
+                      int64_t* bentries);
+
 /* Size of the single-workgroup tail of the uploaded factor: tail vertices,
  * program steps and slots of both sweeps, and forward in-edges from outside
  * the tail.  Measurement only (algorithmic bytes of the tail kernel). */

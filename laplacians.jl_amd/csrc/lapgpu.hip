@@ -1373,6 +1373,18 @@ int lapgpu_factor_info(lapgpu_ctx* c, int64_t* nlevels, int64_t* nwide,
   return 0;
 }
 
+int lapgpu_group_info(lapgpu_ctx* c, int64_t* ngroups, int64_t* rows, int64_t* fentries,
+                      int64_t* bentries) {
+  if (!c) return -1;
+  if (!c->has_F) return set_err(c, -2, "group_info: no LDLinv uploaded");
+  const LevelBlocks& B = c->plan.blocks;
+  if (ngroups) *ngroups = (int64_t)c->groups.size();
+  if (rows) *rows = c->plan.tail.T0;
+  if (fentries) *fentries = B.fptr.empty() ? 0 : (int64_t)B.fptr.back();
+  if (bentries) *bentries = B.bptr.empty() ? 0 : (int64_t)B.bptr.back();
+  return 0;
+}
+
 int lapgpu_tail_info(lapgpu_ctx* c, int64_t* nt, int64_t* nsteps, int64_t* nslots,
                      int64_t* nedges_in) {
   if (!c) return -1;

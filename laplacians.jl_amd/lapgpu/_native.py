@@ -49,6 +49,7 @@ EXPORTS = (
     "lapgpu_last_timings",
     "lapgpu_last_sweep_timings",
     "lapgpu_tail_info",
+    "lapgpu_group_info",
     "lapgpu_last_error",
     "lapgpu_destroy",
     "lapgpu_batch_create",
@@ -97,6 +98,7 @@ def load() -> ctypes.CDLL:
     L.lapgpu_last_timings.argtypes = [P, pdbl, pdbl, pdbl, pdbl, pi64]
     L.lapgpu_last_sweep_timings.argtypes = [P, pdbl, pdbl, pdbl, pdbl]
     L.lapgpu_tail_info.argtypes = [P, pi64, pi64, pi64, pi64]
+    L.lapgpu_group_info.argtypes = [P, pi64, pi64, pi64, pi64]
     L.lapgpu_last_error.argtypes = [P]
     L.lapgpu_last_error.restype = ctypes.c_char_p
     L.lapgpu_destroy.argtypes = [P]
@@ -363,6 +365,11 @@ class Context:
         v = [ctypes.c_int64() for _ in range(4)]
         self._check(self._lib.lapgpu_tail_info(self._h, *(ctypes.byref(t) for t in v)), "lapgpu_tail_info")
         return {"nt": v[0].value, "nsteps": v[1].value, "nslots": v[2].value, "nedges_in": v[3].value}
+
+    def group_info(self) -> dict:
+        v = [ctypes.c_int64() for _ in range(4)]
+        self._check(self._lib.lapgpu_group_info(self._h, *(ctypes.byref(t) for t in v)), "lapgpu_group_info")
+        return {"ngroups": v[0].value, "rows": v[1].value, "fentries": v[2].value, "bentries": v[3].value}
 
     def last_sweep_timings(self) -> dict:
         """Per-iteration ms of the sweep kernels of the last PROFILE solve."""
