@@ -519,7 +519,9 @@ def mode_serve(args) -> int:
         F = list(ex.map(lambda g: lapgpu.build_ldli(g, seed=1), G))
     log(f"{len(G)} resident graphs, n {min(g.shape[0] for g in G)}..{max(g.shape[0] for g in G)} "
         f"(sum {sum(g.shape[0] for g in G)}), factored in {time.time() - t0:.1f}s")
-    srv = Server(0, hbm_budget=200 * 2**30)
+    srv = Server(0, hbm_budget=200 * 2**30, plain_workers=args.serve_workers,
+                 factor_workers=args.factor_workers, batch_window=args.batch_window,
+                 max_wait=args.max_wait)
     for i, (g, f) in enumerate(zip(G, F)):
         assert srv.register(i, g, ldli=f)
     # request stream
@@ -547,6 +549,19 @@ def mode_serve(args) -> int:
     s = summary(res, wall)
     paths = {p: sum(o.path == p for o in res) for p in {o.path for o in res}}
     log(f"served: {s} paths {paths}")
+    for pth in sorted(paths):
+        sel = [(r, o) for r, o in zip(reqs, res) if o.path == pth]
+        lat = np.array([o.latency for _, o in sel])
+        done = max(r.arrival + o.latency for r, o in sel)
+        newg = sum(r.new_graph is not None for r, _ in sel)
+        log(f"  path {pth}: {len(sel)} ({newg} new graphs), latency p50 {np.percentile(lat, 50):.2f} s "
+            f"p99 {np.percentile(lat, 99):.2f} s, last done at {done:.1f} s")
+    sl = np.array(srv.solve_log[len(G):]) if len(srv.solve_log) > len(G) else np.zeros((0, 3))
+    for lo_n, hi_n in ((0, 1e5), (1e5, 1e6), (1e6, 1e8)):
+        m = (sl[:, 0] >= lo_n) & (sl[:, 0] < hi_n) if len(sl) else np.zeros(0, bool)
+        if m.any():
+            log(f"  solves n in [{lo_n:g},{hi_n:g}): {m.sum()} solves, {sl[m, 1].sum():.0f} columns, "
+                f"mean columns {sl[m, 1].mean():.1f}, {sl[m, 2].sum():.1f} s")
     # plain one-at-a-time FP64 GPU PCG on the same GPU: the service time of
     # one plain solve per resident graph, then the same arrival stream
     # through a single FIFO server (new graphs: their own plain solve time,
@@ -628,6 +643,10 @@ def main() -> int:
     ap.add_argument("--serve-max-n", type=float, default=4e6)
     ap.add_argument("--serve-new-max-n", type=float, default=2e5)
     ap.add_argument("--new-frac", type=float, default=0.2)
+    ap.add_argument("--serve-workers", type=int, default=8, help="serving: graphs solved concurrently")
+    ap.add_argument("--factor-workers", type=int, default=8, help="serving: host threads factoring new graphs")
+    ap.add_argument("--batch-window", type=float, default=2.0, help="serving: wait for more requests, x last solve time")
+    ap.add_argument("--max-wait", type=float, default=3.0, help="serving: longest wait for more requests (s)")
     ap.add_argument("--calib", action="store_true",
                     help="plain mode: end with a device-to-device copy of 512 MiB (PMC calibration)")
     args = ap.parse_args()

@@ -10,7 +10,9 @@ over time; each is answered by the same solve closure as approxchol_lap's
   worker solves everything queued for that graph in one multi-RHS solve
   (lapgpu_solve_multi, up to the graph's `max_rhs` columns sharing every
   launch; lapgpu_solve when only one is queued) on the graph's own stream;
-  all kinds overlap on the device;
+  all kinds overlap on the device.  The batch is taken when the worker
+  starts, and an idle graph waits up to half its last solve time (at most
+  `max_wait`) for more requests unless `max_rhs` are already queued;
 * keeps up to four batched launches in flight (the batch's launch slots):
   requests that arrive while others run start in the next free slot instead
   of waiting for the running launch to finish;
@@ -72,6 +74,8 @@ class _Graph:
     busy: int = 0
     pending: deque = field(default_factory=deque)
     max_rhs: int = 1  # columns per multi-RHS solve on this graph
+    last_solve_s: float = 0.0  # duration of the graph's last solve
+    ready_at: float = 0.0  # an idle graph with few requests waits until then
 
 
 def graph_bytes_estimate(n: int, nnz_l: int, nnzf: int) -> int:
@@ -93,7 +97,8 @@ def multi_bytes_estimate(n: int, k: int) -> int:
 class Server:
     def __init__(self, device: int = 0, hbm_budget: int | None = None, wg_max_n: int = 20_000,
                  max_launch: int = 512, plain_workers: int = 4, factor_workers: int = 4,
-                 keep_x: bool = False, max_rhs: int = 64):
+                 keep_x: bool = False, max_rhs: int = 64, batch_window: float = 0.5,
+                 max_wait: float = 1.0):
         self.device = device
         self.batch = _native.Batch(device)
         self.hbm_budget = hbm_budget if hbm_budget is not None else 200 * 2**30
@@ -101,6 +106,12 @@ class Server:
         self.max_launch = max_launch
         self.keep_x = keep_x
         self.max_rhs = max_rhs
+        # an idle graph whose first queued request has waited less than
+        # batch_window x its last solve time (at most max_wait seconds) and
+        # that has fewer than max_rhs requests queued collects more first
+        self.batch_window = batch_window
+        self.max_wait = max_wait
+        self.solve_log: list = []  # (n, columns, seconds) of each plain / multi-RHS solve
         self.graphs: "OrderedDict[object, _Graph]" = OrderedDict()
         self.hbm_used = 0
         self._plain_pool = ThreadPoolExecutor(max_workers=plain_workers)
@@ -172,6 +183,7 @@ class Server:
         plain_futs = {}
         factor_futs = {}
         waiting_admission: deque = deque()
+        waiting: set = set()  # idle graphs collecting a batch
         t0 = time.perf_counter()
         nxt = 0
         total = len(reqs)
@@ -186,26 +198,43 @@ class Server:
                 g.busy += 1
                 small.append((g, r))
             else:
+                if not g.pending and not g.busy:
+                    g.ready_at = time.perf_counter() + min(self.batch_window * g.last_solve_s,
+                                                           self.max_wait)
                 g.pending.append(r)
                 pump_plain(g)
 
-        def pump_plain(g: _Graph):
-            # one solve at a time per context (a context owns one stream);
-            # everything queued with the same tol and maxits goes together
-            if g.busy or not g.pending:
-                return
+        def solve_queued(g: _Graph):
+            # runs in a worker: takes everything queued for the graph NOW
+            # (not when the task was submitted, so requests that arrive while
+            # the task waits for a worker join it), with the same tol and
+            # maxits as the oldest, up to max_rhs columns
             take = [g.pending.popleft()]
             while (g.pending and len(take) < g.max_rhs and g.pending[0].tol == take[0].tol
                    and g.pending[0].maxits == take[0].maxits):
                 take.append(g.pending.popleft())
-            g.busy = 1
             r0 = take[0]
+            t_start = time.perf_counter()
             if len(take) == 1:
-                fut = self._plain_pool.submit(g.handle.solve, r0.b, r0.tol, r0.maxits, float("inf"), 5, 0)
+                x, info = g.handle.solve(r0.b, r0.tol, r0.maxits, float("inf"), 5, 0)
             else:
                 B = np.stack([r.b for r in take], axis=1)
-                fut = self._plain_pool.submit(g.handle.solve_multi, B, r0.tol, r0.maxits, float("inf"), 5, 0)
-            plain_futs[fut] = (g, take)
+                x, info = g.handle.solve_multi(B, r0.tol, r0.maxits, float("inf"), 5, 0)
+            dt = time.perf_counter() - t_start
+            self.solve_log.append((g.n, len(take), dt))
+            g.last_solve_s = dt
+            return take, x, info
+
+        def pump_plain(g: _Graph):
+            # one solve at a time per context (a context owns one stream)
+            if g.busy or not g.pending:
+                return
+            if len(g.pending) < g.max_rhs and time.perf_counter() < g.ready_at:
+                waiting.add(g.key)
+                return
+            waiting.discard(g.key)
+            g.busy = 1
+            plain_futs[self._plain_pool.submit(solve_queued, g)] = g
 
         while len(results) < total:
             now = time.perf_counter() - t0
@@ -235,6 +264,15 @@ class Server:
                     progressed = True
                 else:
                     waiting_admission.append((r, (la, ldli)))
+            # idle graphs whose batching window has ended
+            for key in list(waiting):
+                g = self.graphs.get(key)
+                if g is None:
+                    waiting.discard(key)
+                    continue
+                if len(g.pending) >= g.max_rhs or time.perf_counter() >= g.ready_at:
+                    pump_plain(g)
+                    progressed = True
             # finished batched launches
             for item in [it for it in launches if self.batch.query(it[0])]:
                 launches.remove(item)
@@ -245,9 +283,10 @@ class Server:
                 progressed = True
             # finished plain solves
             for fut in [f for f in plain_futs if f.done()]:
-                g, take = plain_futs.pop(fut)
-                x, info = fut.result()
+                g = plain_futs.pop(fut)
+                take, x, info = fut.result()
                 g.busy = 0
+                g.ready_at = 0.0  # requests queued meanwhile go at once
                 if len(take) == 1:
                     finish(take[0], info, x, "plain")
                 else:
