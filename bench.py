@@ -506,6 +506,13 @@ def _serve_graph(rng, nv: float):
 def mode_serve(args) -> int:
     from concurrent.futures import ThreadPoolExecutor
 
+    # every resident graph owns a stream; HIP maps streams round-robin onto
+    # its hardware queues (4 unless GPU_MAX_HW_QUEUES says otherwise), and two
+    # busy streams on one queue run one after the other.  Read at HIP start-up,
+    # so it is set before the library loads.
+    if args.hw_queues:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
+
     import lapgpu
     from lapgpu import _native
     from lapgpu.serve import Request, Server, summary
@@ -556,6 +563,11 @@ def mode_serve(args) -> int:
         newg = sum(r.new_graph is not None for r, _ in sel)
         log(f"  path {pth}: {len(sel)} ({newg} new graphs), latency p50 {np.percentile(lat, 50):.2f} s "
             f"p99 {np.percentile(lat, 99):.2f} s, last done at {done:.1f} s")
+    log(f"  dispatch loop: {srv.stats}")
+    for name, lg in (("factor", srv.factor_log), ("context upload", srv.upload_log)):
+        if lg:
+            log(f"  new graphs, {name}: {len(lg)} x, median {np.median(lg) * 1e3:.1f} ms, "
+                f"max {max(lg) * 1e3:.1f} ms, sum {sum(lg):.1f} s")
     sl = np.array(srv.solve_log[len(G):]) if len(srv.solve_log) > len(G) else np.zeros((0, 3))
     for lo_n, hi_n in ((0, 1e5), (1e5, 1e6), (1e6, 1e8)):
         m = (sl[:, 0] >= lo_n) & (sl[:, 0] < hi_n) if len(sl) else np.zeros(0, bool)
@@ -647,6 +659,7 @@ def main() -> int:
     ap.add_argument("--factor-workers", type=int, default=8, help="serving: host threads factoring new graphs")
     ap.add_argument("--batch-window", type=float, default=2.0, help="serving: wait for more requests, x last solve time")
     ap.add_argument("--max-wait", type=float, default=3.0, help="serving: longest wait for more requests (s)")
+    ap.add_argument("--hw-queues", type=int, default=0, help="serving: HIP hardware queues (0: HIP's default)")
     ap.add_argument("--calib", action="store_true",
                     help="plain mode: end with a device-to-device copy of 512 MiB (PMC calibration)")
     args = ap.parse_args()

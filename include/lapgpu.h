@@ -163,6 +163,12 @@ int lapgpu_last_sweep_timings(lapgpu_ctx* ctx, double* fgroups_ms, double* tail_
                               double* tail_ms, double* bgroups_ms);
 
 const char* lapgpu_last_error(lapgpu_ctx* ctx);
+
+/* Release a context.  Device memory is stream-ordered (hipFreeAsync on the
+ * context's stream, from the device's default pool, which keeps freed memory
+ * for reuse) and pinned buffers return to a process-wide free list, so
+ * neither creating nor destroying a context waits for other contexts' work
+ * on the device. */
 void lapgpu_destroy(lapgpu_ctx* ctx);
 
 /* ---- batched solves: many graphs and many right-hand sides ------------- */
@@ -185,7 +191,9 @@ int lapgpu_batch_add_graph(lapgpu_batch* B, int64_t n, const int64_t* colptr,
                            const double* fval, const double* d, int index_base,
                            int32_t* graph_id);
 
-/* Free a graph's device memory; its id is not reused. */
+/* Free a graph's device memory.  Returns at once: the frees are ordered
+ * after the uncollected launches that read the graph, and no other launch
+ * is waited for.  A later lapgpu_batch_add_graph may reuse the id. */
 int lapgpu_batch_remove_graph(lapgpu_batch* B, int32_t graph_id);
 
 /* n and the device bytes held by a graph (for admission by HBM footprint). */

@@ -13,11 +13,14 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "lapgpu.h"
 #include "lapgpu_lock.hpp"
+#include "lapgpu_mem.hpp"
 #include "lapgpu_multi.hpp"
 #include "lapgpu_plan.hpp"
 
@@ -25,9 +28,63 @@ using namespace lapgpu;
 
 static_assert(kTailThreads == kTailLanes, "tail workgroup width = lanes of a tail step");
 
+namespace lapgpu {
+
+hipError_t pool_setup(int device) {
+  static std::mutex mu;
+  static std::vector<bool> done;
+  std::lock_guard<std::mutex> g(mu);
+  if ((size_t)device < done.size() && done[device]) return hipSuccess;
+  hipMemPool_t pool;
+  hipError_t e = hipDeviceGetDefaultMemPool(&pool, device);
+  if (e != hipSuccess) return e;
+  uint64_t keep = UINT64_MAX;
+  if ((e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep)) != hipSuccess)
+    return e;
+  if ((size_t)device >= done.size()) done.resize(device + 1, false);
+  done[device] = true;
+  return hipSuccess;
+}
+
+namespace {
+std::mutex g_pinned_mu;
+std::map<size_t, std::vector<void*>> g_pinned;  // size class -> free blocks
+
+size_t pinned_class(size_t bytes) {
+  size_t c = 256;
+  while (c < bytes) c <<= 1;
+  return c;
+}
+}  // namespace
+
+hipError_t pinned_get(void** p, size_t bytes, size_t* cap) {
+  const size_t c = pinned_class(bytes);
+  if (cap) *cap = c;
+  {
+    std::lock_guard<std::mutex> g(g_pinned_mu);
+    auto it = g_pinned.find(c);
+    if (it != g_pinned.end() && !it->second.empty()) {
+      *p = it->second.back();
+      it->second.pop_back();
+      return hipSuccess;
+    }
+  }
+  LAPGPU_LOCK();
+  return hipHostMalloc(p, c, hipHostMallocDefault);
+}
+
+void pinned_put(void* p, size_t bytes) {
+  if (!p) return;
+  std::lock_guard<std::mutex> g(g_pinned_mu);
+  g_pinned[pinned_class(bytes)].push_back(p);
+}
+
+}  // namespace lapgpu
+
 namespace {
 
 std::string g_create_err;
+constexpr int kMultiMax = 64;  // columns of one multi-RHS chunk
 
 template <typename T>
 struct DevArr {
@@ -149,29 +206,31 @@ int set_err(lapgpu_ctx* c, int code, const std::string& msg) {
                                    hipGetErrorString(e_));                   \
   } while (0)
 
+// Device buffers are stream-ordered on the context's stream (a free waits
+// for the context's own work only, never for the whole device).
+void dev_free(lapgpu_ctx* c, void* p) {
+  if (p) (void)hipFreeAsync(p, c->stream);
+}
+
 template <typename T>
 int dev_upload(lapgpu_ctx* c, T** dst, const std::vector<T>& src) {
-  LAPGPU_LOCK();
-  if (*dst) {
-    HIPC(c, hipFree(*dst));
-    *dst = nullptr;
-  }
+  dev_free(c, *dst);
+  *dst = nullptr;
   size_t bytes = sizeof(T) * (src.empty() ? 1 : src.size());
-  HIPC(c, hipMalloc((void**)dst, bytes));
-  if (!src.empty())
-    HIPC(c, hipMemcpy(*dst, src.data(), sizeof(T) * src.size(),
-                      hipMemcpyHostToDevice));
+  HIPC(c, hipMallocAsync((void**)dst, bytes, c->stream));
+  if (!src.empty()) {
+    HIPC(c, hipMemcpyAsync(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice,
+                           c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));  // src is the caller's host buffer
+  }
   return 0;
 }
 
 template <typename T>
 int dev_alloc(lapgpu_ctx* c, T** dst, size_t count) {
-  LAPGPU_LOCK();
-  if (*dst) {
-    HIPC(c, hipFree(*dst));
-    *dst = nullptr;
-  }
-  HIPC(c, hipMalloc((void**)dst, sizeof(T) * (count ? count : 1)));
+  dev_free(c, *dst);
+  *dst = nullptr;
+  HIPC(c, hipMallocAsync((void**)dst, sizeof(T) * (count ? count : 1), c->stream));
   return 0;
 }
 
@@ -330,26 +389,21 @@ void drop_graphs(lapgpu_ctx* c) {
 
 // the multi-RHS workspace and tail CSRs (rebuilt on the next multi solve)
 void free_multi(lapgpu_ctx* c) {
-  LAPGPU_LOCK();
   auto& m = c->m;
   for (auto& e : m.exec)
     if (e) (void)hipGraphExecDestroy(e);
   void* dp[] = {m.r, m.x, m.bestx, m.p, m.q, m.y, m.z, m.io, m.part, m.ext,
-                m.y32, m.z32, m.pf, m.ext32, m.r0};
-  for (void* a : dp)
-    if (a) (void)hipFree(a);
-  if (m.flags) (void)hipFree(m.flags);
-  if (m.st) (void)hipFree(m.st);
-  if (m.hst) (void)hipHostFree(m.hst);
+                m.y32, m.z32, m.pf, m.ext32, m.r0, m.flags, m.st};
+  for (void* a : dp) dev_free(c, a);
+  // the context's stream is idle whenever the multi workspace is dropped
+  lapgpu::pinned_put(m.hst, sizeof(DevState) * kMultiMax);
   m = lapgpu_ctx::Multi();
 }
 
 void free_f32(lapgpu_ctx* c) {
-  LAPGPU_LOCK();
   void* fp[] = {c->Lval32, c->gFw32, c->gBw32, c->s32, c->Ew32, c->y32, c->z32, c->ext32, c->pf,
                 c->fslots32, c->bslots32, c->r0};
-  for (void* a : fp)
-    if (a) (void)hipFree(a);
+  for (void* a : fp) dev_free(c, a);
   c->Lval32 = c->gFw32 = c->gBw32 = c->s32 = c->Ew32 = c->y32 = c->z32 = c->ext32 = c->pf = nullptr;
   c->fslots32 = c->bslots32 = nullptr;
   c->r0 = nullptr;
@@ -357,25 +411,21 @@ void free_f32(lapgpu_ctx* c) {
 }
 
 int free_graph(lapgpu_ctx* c) {
-  LAPGPU_LOCK();
   drop_graphs(c);
   free_f32(c);
   free_multi(c);
   void* tp[] = {c->Eptr, c->Eidx, c->Ew, c->ext, c->fsteps, c->bsteps, c->fslots, c->bslots};
-  for (void* a : tp)
-    if (a) (void)hipFree(a);
+  for (void* a : tp) dev_free(c, a);
   c->Eptr = c->Eidx = nullptr;
   c->Ew = c->ext = nullptr;
   c->fsteps = c->bsteps = nullptr;
   c->fslots = c->bslots = nullptr;
   int* ip[] = {c->perm, c->Lptr, c->Lidx, c->lvl, c->gFptr, c->gFidx, c->gBptr, c->gBidx,
                c->heavy, c->mheavy};
-  for (int* a : ip)
-    if (a) (void)hipFree(a);
+  for (int* a : ip) dev_free(c, a);
   double* dp[] = {c->Lval, c->s, c->b, c->r, c->x, c->bestx, c->p, c->q, c->y, c->z, c->io,
                   c->gFw, c->gBw, c->zF};
-  for (double* a : dp)
-    if (a) (void)hipFree(a);
+  for (double* a : dp) dev_free(c, a);
   c->perm = c->Lptr = c->Lidx = c->lvl = nullptr;
   c->gFptr = c->gFidx = c->gBptr = c->gBidx = c->heavy = c->mheavy = nullptr;
   c->Lval = c->s = c->b = c->r = c->x = c->bestx = c->p = c->q = c->y = c->z = c->io = nullptr;
@@ -782,12 +832,9 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
     default: { constexpr int K_ = 64; F(__VA_ARGS__); break; }      \
   }
 
-constexpr int kMultiMax = 64;
-
 // Workspace for n * K elements per vector (kept while it is large enough);
 // FP32 buffers only in the FP32-inner mode.
 int multi_prepare(lapgpu_ctx* c, int K) {
-  LAPGPU_LOCK();
   auto& m = c->m;
   const int64_t n = c->n;
   const int64_t NT = std::max<int64_t>(c->plan.tail.NT, 1);
@@ -796,7 +843,7 @@ int multi_prepare(lapgpu_ctx* c, int K) {
     if ((rc = dev_alloc(c, &m.part, (size_t)kNumSlots * kMRed * kMultiMax))) return rc;
     if ((rc = dev_alloc(c, &m.st, (size_t)kMultiMax))) return rc;
     if ((rc = dev_alloc(c, &m.flags, 4))) return rc;
-    HIPC(c, hipHostMalloc((void**)&m.hst, sizeof(DevState) * kMultiMax, hipHostMallocDefault));
+    HIPC(c, lapgpu::pinned_get((void**)&m.hst, sizeof(DevState) * kMultiMax));
   }
   const int64_t need = n * K;
   if (need > m.cap) {
@@ -1139,18 +1186,21 @@ int lapgpu_create(lapgpu_ctx** out, int device) {
   };
   hipError_t e;
   if ((e = hipSetDevice(device)) != hipSuccess) return fail("hipSetDevice", e);
+  if ((e = lapgpu::pool_setup(device)) != hipSuccess) return fail("hipMemPoolSetAttribute", e);
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
     return fail("hipStreamCreate", e);
   int khz = 0;
   if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
     c->wallclock_khz = (double)khz;
-  if ((e = hipMalloc((void**)&c->part, sizeof(double) * kNumSlots * kRedBlocks)) != hipSuccess)
-    return fail("hipMalloc", e);
-  if ((e = hipMalloc((void**)&c->st, sizeof(DevState))) != hipSuccess) return fail("hipMalloc", e);
-  if ((e = hipHostMalloc((void**)&c->hst, sizeof(DevState), hipHostMallocDefault)) != hipSuccess)
+  if ((e = hipMallocAsync((void**)&c->part, sizeof(double) * kNumSlots * kRedBlocks, c->stream)) !=
+      hipSuccess)
+    return fail("hipMallocAsync", e);
+  if ((e = hipMallocAsync((void**)&c->st, sizeof(DevState), c->stream)) != hipSuccess)
+    return fail("hipMallocAsync", e);
+  if ((e = lapgpu::pinned_get((void**)&c->hst, sizeof(DevState))) != hipSuccess)
     return fail("hipHostMalloc", e);
-  if ((e = hipHostMalloc((void**)&c->hflags, sizeof(int32_t) * lapgpu_ctx::kRing,
-                         hipHostMallocDefault)) != hipSuccess)
+  if ((e = lapgpu::pinned_get((void**)&c->hflags, sizeof(int32_t) * lapgpu_ctx::kRing)) !=
+      hipSuccess)
     return fail("hipHostMalloc", e);
   for (auto& ev : c->ring_ev)
     if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess)
@@ -1170,12 +1220,15 @@ void lapgpu_destroy(lapgpu_ctx* c) {
   if (!c) return;
   LAPGPU_LOCK();
   (void)hipSetDevice(c->device);
-  if (c->stream) (void)hipStreamSynchronize(c->stream);
-  free_graph(c);
-  if (c->part) (void)hipFree(c->part);
-  if (c->st) (void)hipFree(c->st);
-  if (c->hst) (void)hipHostFree(c->hst);
-  if (c->hflags) (void)hipHostFree(c->hflags);
+  if (c->stream) {
+    (void)hipStreamSynchronize(c->stream);
+    free_graph(c);
+    dev_free(c, c->part);
+    dev_free(c, c->st);
+    (void)hipStreamSynchronize(c->stream);
+  }
+  lapgpu::pinned_put(c->hst, sizeof(DevState));
+  lapgpu::pinned_put(c->hflags, sizeof(int32_t) * lapgpu_ctx::kRing);
   for (auto& ev : c->ring_ev)
     if (ev) (void)hipEventDestroy(ev);
   for (auto& ev : c->ev)

@@ -18,6 +18,7 @@
 
 #include "lapgpu.h"
 #include "lapgpu_lock.hpp"
+#include "lapgpu_mem.hpp"
 #include "lapgpu_plan.hpp"
 #include "lapgpu_wg.hpp"
 
@@ -40,11 +41,17 @@ constexpr int kBatchSlots = 4;
 
 }  // namespace
 
-// One launch slot: its own stream, request table and workspace, so that
-// launches on different slots overlap on the device.
+// One launch slot: its own stream, request table, workspace and copy of the
+// graph descriptor table, so that launches on different slots overlap on the
+// device and adding or removing a graph never waits for a running launch.
 struct BatchSlot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
+  WgGraphD* d_graphs = nullptr;
+  size_t d_graphs_cap = 0;
+  std::vector<WgGraphD> h_graphs;  // source of the last table upload
+  uint64_t version = 0;            // store version of the table on the device
+  std::vector<int32_t> gids;       // graphs the running launch reads
   WgReqD* d_reqs = nullptr;
   size_t reqs_cap = 0;
   WgReqD* h_reqs = nullptr;  // pinned
@@ -68,9 +75,7 @@ struct lapgpu_batch {
   double wallclock_khz = 100000.0;
   std::string err;
   std::vector<GraphStore> graphs;
-  WgGraphD* d_graphs = nullptr;
-  size_t d_graphs_cap = 0;
-  bool graphs_dirty = true;
+  uint64_t version = 1;  // bumped by every add and remove
   std::vector<BatchSlot> slots;
   double* d_io = nullptr;
   size_t d_io_cap = 0;
@@ -93,33 +98,50 @@ int berr(lapgpu_batch* B, int code, const std::string& m) {
       return berr(B, -10, std::string(#call) + ": " + hipGetErrorString(e_));  \
   } while (0)
 
+// Device memory is stream-ordered (hipMallocAsync / hipFreeAsync): a free
+// waits for the work on its stream, never for the whole device.  Graph
+// uploads go through the batch's own stream, which add_graph synchronises
+// before the graph can be named by a launch.
 template <typename T>
 int up(lapgpu_batch* B, T** dst, const T* src, size_t count, int64_t& bytes) {
-  LAPGPU_LOCK();
   const size_t nb = sizeof(T) * (count ? count : 1);
-  BHIP(B, hipMalloc((void**)dst, nb));
-  if (count) BHIP(B, hipMemcpy(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice));
+  BHIP(B, hipMallocAsync((void**)dst, nb, B->stream));
+  if (count)
+    BHIP(B, hipMemcpyAsync(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice, B->stream));
   bytes += (int64_t)nb;
   return 0;
 }
 
 template <typename T>
-int grow(lapgpu_batch* B, T** p, size_t* cap, size_t want) {
+int grow(lapgpu_batch* B, hipStream_t stream, T** p, size_t* cap, size_t want) {
   if (want <= *cap) return 0;
-  LAPGPU_LOCK();
-  if (*p) BHIP(B, hipFree(*p));
+  if (*p) BHIP(B, hipFreeAsync(*p, stream));
   *p = nullptr;
   size_t c = std::max(want, *cap * 2);
-  BHIP(B, hipMalloc((void**)p, sizeof(T) * c));
+  BHIP(B, hipMallocAsync((void**)p, sizeof(T) * c, stream));
   *cap = c;
   return 0;
 }
 
-void free_store(GraphStore& G) {
-  LAPGPU_LOCK();
+// Pinned staging of at least `count` elements (kept while large enough).
+template <typename T>
+int grow_pinned(lapgpu_batch* B, T** p, size_t* cap, size_t count) {
+  if (count <= *cap) return 0;
+  lapgpu::pinned_put(*p, sizeof(T) * *cap);
+  *p = nullptr;
+  *cap = 0;
+  size_t got = 0;
+  BHIP(B, lapgpu::pinned_get((void**)p, sizeof(T) * count, &got));
+  *cap = got / sizeof(T);
+  return 0;
+}
+
+// Frees are ordered on `stream` (which must already wait for every launch
+// that reads the graph).
+void free_store(GraphStore& G, hipStream_t stream) {
   void* ps[] = {G.perm, G.Lptr, G.Lidx, G.Lval, G.s, G.fsteps, G.bsteps, G.fslots, G.bslots};
   for (void* p : ps)
-    if (p) (void)hipFree(p);
+    if (p) (void)hipFreeAsync(p, stream);
   G = GraphStore();
 }
 
@@ -139,6 +161,7 @@ int lapgpu_batch_create(lapgpu_batch** out, int device) {
   lapgpu_batch* B = new lapgpu_batch();
   B->device = device;
   hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = lapgpu::pool_setup(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     g_batch_err = std::string("lapgpu_batch_create: ") + hipGetErrorString(e);
@@ -168,20 +191,24 @@ void lapgpu_batch_destroy(lapgpu_batch* B) {
   (void)hipSetDevice(B->device);
   if (B->stream) (void)hipStreamSynchronize(B->stream);
   for (BatchSlot& S : B->slots) {
-    if (S.stream) (void)hipStreamSynchronize(S.stream);
-    if (S.d_reqs) (void)hipFree(S.d_reqs);
-    if (S.d_work) (void)hipFree(S.d_work);
-    if (S.h_reqs) (void)hipHostFree(S.h_reqs);
-    if (S.d_io) (void)hipFree(S.d_io);
-    if (S.h_io) (void)hipHostFree(S.h_io);
+    if (S.stream) {
+      (void)hipStreamSynchronize(S.stream);
+      void* ps[] = {S.d_graphs, S.d_reqs, S.d_work, S.d_io};
+      for (void* p : ps)
+        if (p) (void)hipFreeAsync(p, S.stream);
+      (void)hipStreamSynchronize(S.stream);
+    }
+    lapgpu::pinned_put(S.h_reqs, sizeof(WgReqD) * S.h_cap);
+    lapgpu::pinned_put(S.h_io, sizeof(double) * S.hio_cap);
     if (S.done) (void)hipEventDestroy(S.done);
     if (S.stream) (void)hipStreamDestroy(S.stream);
   }
-  for (GraphStore& G : B->graphs) free_store(G);
-  void* ps[] = {B->d_graphs, B->d_io};
-  for (void* p : ps)
-    if (p) (void)hipFree(p);
-  if (B->stream) (void)hipStreamDestroy(B->stream);
+  if (B->stream) {
+    for (GraphStore& G : B->graphs) free_store(G, B->stream);
+    if (B->d_io) (void)hipFreeAsync(B->d_io, B->stream);
+    (void)hipStreamSynchronize(B->stream);
+    (void)hipStreamDestroy(B->stream);
+  }
   delete B;
 }
 
@@ -221,13 +248,23 @@ int lapgpu_batch_add_graph(lapgpu_batch* B, int64_t n, const int64_t* colptr,
       (rc = up(B, &G.bsteps, (const TailStepD*)P.tail.bsteps.data(), P.tail.bsteps.size(), G.bytes)) ||
       (rc = up(B, &G.fslots, (const TailSlotD*)P.tail.fslots.data(), P.tail.fslots.size(), G.bytes)) ||
       (rc = up(B, &G.bslots, (const TailSlotD*)P.tail.bslots.data(), P.tail.bslots.size(), G.bytes))) {
-    free_store(G);
+    (void)hipStreamSynchronize(B->stream);
+    free_store(G, B->stream);
     return rc;
   }
+  // the plan's host vectors are the copies' sources
+  BHIP(B, hipStreamSynchronize(B->stream));
   G.live = true;
-  B->graphs.push_back(G);
-  B->graphs_dirty = true;
-  *graph_id = (int32_t)(B->graphs.size() - 1);
+  // reuse the id of a removed graph (launches that read it were ordered
+  // before its frees; new launches get a new descriptor table)
+  size_t id = 0;
+  while (id < B->graphs.size() && B->graphs[id].live) ++id;
+  if (id == B->graphs.size())
+    B->graphs.push_back(G);
+  else
+    B->graphs[id] = G;
+  ++B->version;
+  *graph_id = (int32_t)id;
   B->err.clear();
   return 0;
 }
@@ -236,11 +273,12 @@ int lapgpu_batch_remove_graph(lapgpu_batch* B, int32_t graph_id) {
   if (!B || graph_id < 0 || graph_id >= (int32_t)B->graphs.size() || !B->graphs[graph_id].live)
     return berr(B, -1, "lapgpu_batch_remove_graph: unknown graph");
   BHIP(B, hipSetDevice(B->device));
-  BHIP(B, hipStreamSynchronize(B->stream));
+  // the frees wait (on the device) for the launches that read this graph
   for (BatchSlot& S : B->slots)
-    if (S.busy) BHIP(B, hipStreamSynchronize(S.stream));
-  free_store(B->graphs[graph_id]);
-  B->graphs_dirty = true;
+    if (S.busy && std::find(S.gids.begin(), S.gids.end(), graph_id) != S.gids.end())
+      BHIP(B, hipStreamWaitEvent(B->stream, S.done, 0));
+  free_store(B->graphs[graph_id], B->stream);
+  ++B->version;
   return 0;
 }
 
@@ -271,21 +309,21 @@ int launch_on(lapgpu_batch* B, int slot, int64_t nreq, const int32_t* graph_ids,
   BatchSlot& S = B->slots[slot];
   const int ng = (int)B->graphs.size();
   int rc;
-  if (B->graphs_dirty) {
-    LAPGPU_LOCK();
-    // other slots may still read the descriptor table: let them finish
-    for (BatchSlot& o : B->slots)
-      if (o.busy) BHIP(B, hipStreamSynchronize(o.stream));
-    std::vector<WgGraphD> hg(ng);
+  if (S.version != B->version) {
+    // this slot's own table (the slot is idle; other slots keep theirs);
+    // h_graphs stays untouched until the slot's next launch, after collect
+    S.h_graphs.resize(ng);
     for (int i = 0; i < ng; ++i) {
       const GraphStore& G = B->graphs[i];
-      hg[i] = WgGraphD{G.n, G.nf, G.nb, 0, G.perm, G.Lptr, G.Lidx, G.Lval, G.s,
-                       G.fsteps, G.fslots, G.bsteps, G.bslots};
+      S.h_graphs[i] = WgGraphD{G.n, G.nf, G.nb, 0, G.perm, G.Lptr, G.Lidx, G.Lval, G.s,
+                               G.fsteps, G.fslots, G.bsteps, G.bslots};
     }
-    if ((rc = grow(B, &B->d_graphs, &B->d_graphs_cap, (size_t)ng))) return rc;
-    BHIP(B, hipMemcpy(B->d_graphs, hg.data(), sizeof(WgGraphD) * ng, hipMemcpyHostToDevice));
-    B->graphs_dirty = false;
+    if ((rc = grow(B, S.stream, &S.d_graphs, &S.d_graphs_cap, (size_t)ng))) return rc;
+    BHIP(B, hipMemcpyAsync(S.d_graphs, S.h_graphs.data(), sizeof(WgGraphD) * ng,
+                           hipMemcpyHostToDevice, S.stream));
+    S.version = B->version;
   }
+  S.gids.assign(graph_ids, graph_ids + nreq);
   // largest first
   S.order.resize(nreq);
   std::iota(S.order.begin(), S.order.end(), 0);
@@ -294,15 +332,9 @@ int launch_on(lapgpu_batch* B, int slot, int64_t nreq, const int32_t* graph_ids,
   });
   size_t work = 0;
   for (int64_t i = 0; i < nreq; ++i) work += 8 * (size_t)B->graphs[graph_ids[i]].n;
-  if ((rc = grow(B, &S.d_work, &S.work_cap, work))) return rc;
-  if ((rc = grow(B, &S.d_reqs, &S.reqs_cap, (size_t)nreq))) return rc;
-  if ((size_t)nreq > S.h_cap) {
-    LAPGPU_LOCK();
-    if (S.h_reqs) BHIP(B, hipHostFree(S.h_reqs));
-    S.h_reqs = nullptr;
-    BHIP(B, hipHostMalloc((void**)&S.h_reqs, sizeof(WgReqD) * nreq, hipHostMallocDefault));
-    S.h_cap = (size_t)nreq;
-  }
+  if ((rc = grow(B, S.stream, &S.d_work, &S.work_cap, work))) return rc;
+  if ((rc = grow(B, S.stream, &S.d_reqs, &S.reqs_cap, (size_t)nreq))) return rc;
+  if ((rc = grow_pinned(B, &S.h_reqs, &S.h_cap, (size_t)nreq))) return rc;
   const bool has_mt = maxtime > 0 && std::isfinite(maxtime);
   size_t off = 0;
   for (int64_t k = 0; k < nreq; ++k) {
@@ -324,7 +356,7 @@ int launch_on(lapgpu_batch* B, int slot, int64_t nreq, const int32_t* graph_ids,
   BHIP(B, hipMemcpyAsync(S.d_reqs, S.h_reqs, sizeof(WgReqD) * nreq, hipMemcpyHostToDevice,
                          S.stream));
   hipLaunchKernelGGL(k_wg_pcg, dim3((unsigned)nreq), dim3(kWgThreads), 0, S.stream,
-                     B->d_graphs, S.d_reqs);
+                     S.d_graphs, S.d_reqs);
   BHIP(B, hipGetLastError());
   BHIP(B, hipMemcpyAsync(S.h_reqs, S.d_reqs, sizeof(WgReqD) * nreq, hipMemcpyDeviceToHost,
                          S.stream));
@@ -379,14 +411,8 @@ int lapgpu_batch_submit_host(lapgpu_batch* B, int64_t nreq, const int32_t* graph
   BatchSlot& S = B->slots[slot];
   size_t total = 0;
   for (int64_t i = 0; i < nreq; ++i) total += (size_t)B->graphs[graph_ids[i]].n;
-  if ((rc = grow(B, &S.d_io, &S.io_cap, 2 * total))) return rc;
-  if (total > S.hio_cap) {
-    LAPGPU_LOCK();
-    if (S.h_io) BHIP(B, hipHostFree(S.h_io));
-    S.h_io = nullptr;
-    BHIP(B, hipHostMalloc((void**)&S.h_io, sizeof(double) * total, hipHostMallocDefault));
-    S.hio_cap = total;
-  }
+  if ((rc = grow(B, S.stream, &S.d_io, &S.io_cap, 2 * total))) return rc;
+  if ((rc = grow_pinned(B, &S.h_io, &S.hio_cap, total))) return rc;
   std::memcpy(S.h_io, b_concat, sizeof(double) * total);
   BHIP(B, hipMemcpyAsync(S.d_io, S.h_io, sizeof(double) * total, hipMemcpyHostToDevice, S.stream));
   std::vector<const double*> pb(nreq);
@@ -479,7 +505,7 @@ int lapgpu_batch_solve(lapgpu_batch* B, int64_t nreq, const int32_t* graph_ids,
     total += (size_t)B->graphs[graph_ids[i]].n;
   }
   int rc;
-  if ((rc = grow(B, &B->d_io, &B->d_io_cap, 2 * total))) return rc;
+  if ((rc = grow(B, B->stream, &B->d_io, &B->d_io_cap, 2 * total))) return rc;
   BHIP(B, hipMemcpyAsync(B->d_io, b_concat, sizeof(double) * total, hipMemcpyHostToDevice,
                          B->stream));
   BHIP(B, hipStreamSynchronize(B->stream));

@@ -1,8 +1,10 @@
-// lapgpu_lock.hpp — one process-wide lock around the HIP calls that must
-// not overlap a stream capture running in another thread (allocations,
-// frees, synchronous copies, which synchronise with the null stream) and
-// around the captures themselves.  Solves on different contexts otherwise
-// run concurrently from different host threads (the serving loop does).
+// lapgpu_lock.hpp — one process-wide lock around the stream captures and
+// the few HIP calls that may touch the null stream or device-wide state
+// (stream and event creation and destruction, pinned host allocation), so
+// that none of them overlaps a capture running in another thread.  Device
+// memory is stream-ordered on each owner's own non-blocking stream and takes
+// no lock; solves on different contexts run concurrently from different
+// host threads (the serving loop does).
 #pragma once
 
 #include <mutex>

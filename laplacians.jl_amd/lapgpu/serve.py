@@ -20,7 +20,9 @@ over time; each is answered by the same solve closure as approxchol_lap's
   idle non-resident graphs are evicted (least recently used first), and a
   request waits while its graph cannot be admitted;
 * factors new graphs on host threads (liblapfactor, re-entrant) while the
-  device keeps solving.
+  device keeps solving; the same threads create a large graph's context and
+  upload it (its HBM is reserved first), and one more thread closes dropped
+  contexts, so the dispatch loop itself never waits on an upload or a free.
 
 Every result carries its latency (arrival to completion); `summary` gives
 solves/s and p50/p99 latency.
@@ -112,10 +114,16 @@ class Server:
         self.batch_window = batch_window
         self.max_wait = max_wait
         self.solve_log: list = []  # (n, columns, seconds) of each plain / multi-RHS solve
+        # seconds the dispatch loop spent admitting and dropping graphs
+        self.stats = {"admit_s": 0.0, "drop_s": 0.0, "admits": 0, "drops": 0}
+        # worker seconds per new graph: factoring, context upload
+        self.factor_log: list = []
+        self.upload_log: list = []
         self.graphs: "OrderedDict[object, _Graph]" = OrderedDict()
         self.hbm_used = 0
         self._plain_pool = ThreadPoolExecutor(max_workers=plain_workers)
         self._factor_pool = ThreadPoolExecutor(max_workers=factor_workers)
+        self._close_pool = ThreadPoolExecutor(max_workers=1)
         self._anon = 0
 
     # ------------------------------------------------------------ graphs ---
@@ -129,29 +137,53 @@ class Server:
             ldli = build_ldli(a, seed=seed)
         return self._admit(key, la, ldli, resident)
 
-    def _admit(self, key, la, ldli, resident) -> bool:
+    def _reserve(self, la, ldli):
+        """HBM footprint and multi-RHS width of a graph, evicting idle
+        non-resident graphs when needed; None when it does not fit."""
         n = la.shape[0]
         need = graph_bytes_estimate(n, la.nnz, ldli.nnz)
         if self.hbm_used + need > self.hbm_budget:
             self._evict(need)
             if self.hbm_used + need > self.hbm_budget:
-                return False
-        if n <= self.wg_max_n:
-            gid = self.batch.add_graph(la, ldli)
-            g = _Graph(key, n, "wg", self.batch.graph_info(gid)["device_bytes"] + 64 * n, resident, gid)
-        else:
+                return None
+        k = 1
+        if n > self.wg_max_n:
             # the widest multi-RHS workspace that still fits the budget
             k = self.max_rhs
             while k > 1 and self.hbm_used + need + multi_bytes_estimate(n, k) > self.hbm_budget:
                 k //= 2
             if k > 1:
                 need += multi_bytes_estimate(n, k)
-            ctx = _native.Context(self.device)
-            ctx.upload_lap(la)
-            ctx.upload_ldli(ldli)
-            g = _Graph(key, n, "plain", need, resident, ctx, max_rhs=k)
+        return need, k
+
+    def _upload_context(self, la, ldli):
+        t = time.perf_counter()
+        ctx = _native.Context(self.device)
+        ctx.upload_lap(la)
+        ctx.upload_ldli(ldli)
+        self.upload_log.append(time.perf_counter() - t)
+        return ctx
+
+    def _admit_wg(self, key, la, ldli, resident) -> _Graph:
+        n = la.shape[0]
+        gid = self.batch.add_graph(la, ldli)
+        g = _Graph(key, n, "wg", self.batch.graph_info(gid)["device_bytes"] + 64 * n, resident, gid)
         self.graphs[key] = g
         self.hbm_used += g.bytes
+        return g
+
+    def _admit(self, key, la, ldli, resident) -> bool:
+        res = self._reserve(la, ldli)
+        if res is None:
+            return False
+        need, k = res
+        n = la.shape[0]
+        if n <= self.wg_max_n:
+            self._admit_wg(key, la, ldli, resident)
+        else:
+            g = _Graph(key, n, "plain", need, resident, self._upload_context(la, ldli), max_rhs=k)
+            self.graphs[key] = g
+            self.hbm_used += g.bytes
         return True
 
     def _evict(self, need: int) -> None:
@@ -167,7 +199,8 @@ class Server:
         if g.kind == "wg":
             self.batch.remove_graph(g.handle)
         else:
-            g.handle.close()
+            # closing waits for the device's frees; the context is idle
+            self._close_pool.submit(g.handle.close)
         self.hbm_used -= g.bytes
         del self.graphs[g.key]
 
@@ -182,6 +215,7 @@ class Server:
         launches = []  # (handle, [(graph, request)])
         plain_futs = {}
         factor_futs = {}
+        upload_futs = {}  # context uploads of admitted large graphs
         waiting_admission: deque = deque()
         waiting: set = set()  # idle graphs collecting a batch
         t0 = time.perf_counter()
@@ -257,13 +291,30 @@ class Server:
                 progressed = True
             for _ in range(len(waiting_admission)):
                 r, (la, ldli) = waiting_admission.popleft()
+                ta = time.perf_counter()
+                res = self._reserve(la, ldli)
+                if res is None:
+                    waiting_admission.append((r, (la, ldli)))
+                    continue
                 self._anon += 1
                 key = ("new", self._anon)
-                if self._admit(key, la, ldli, resident=False):
-                    route(self.graphs[key], r)
-                    progressed = True
+                need, k = res
+                if la.shape[0] <= self.wg_max_n:
+                    route(self._admit_wg(key, la, ldli, resident=False), r)
                 else:
-                    waiting_admission.append((r, (la, ldli)))
+                    # reserved now, uploaded on a worker thread
+                    self.hbm_used += need
+                    upload_futs[self._factor_pool.submit(self._upload_context, la, ldli)] = (
+                        key, r, la.shape[0], need, k)
+                self.stats["admit_s"] += time.perf_counter() - ta
+                self.stats["admits"] += 1
+                progressed = True
+            for fut in [f for f in upload_futs if f.done()]:
+                key, r, n, need, k = upload_futs.pop(fut)
+                g = _Graph(key, n, "plain", need, False, fut.result(), max_rhs=k)
+                self.graphs[key] = g
+                route(g, r)
+                progressed = True
             # idle graphs whose batching window has ended
             for key in list(waiting):
                 g = self.graphs.get(key)
@@ -308,19 +359,25 @@ class Server:
                 progressed = True
             # drop finished one-off graphs
             for key in [k for k, g in self.graphs.items() if not g.resident and not g.busy and not g.pending]:
+                td = time.perf_counter()
                 self._drop(self.graphs[key])
+                self.stats["drop_s"] += time.perf_counter() - td
+                self.stats["drops"] += 1
             if not progressed:
                 time.sleep(50e-6)
         return [results[r.rid] for r in requests]
 
-    @staticmethod
-    def _factor(a):
+    def _factor(self, a):
+        t = time.perf_counter()
         a = sp.csc_matrix(a)
-        return lap(a), build_ldli(a, seed=1)
+        out = lap(a), build_ldli(a, seed=1)
+        self.factor_log.append(time.perf_counter() - t)
+        return out
 
     def close(self):
         self._plain_pool.shutdown(wait=True)
         self._factor_pool.shutdown(wait=True)
+        self._close_pool.shutdown(wait=True)
         for g in list(self.graphs.values()):
             if g.kind == "plain":
                 g.handle.close()

@@ -103,3 +103,31 @@ def test_graph_info_and_remove(store):
     B.remove_graph(gid)
     with pytest.raises(lapgpu.LapGPUError):
         B.solve([gid], [_rhs(25, 0)])
+
+
+def test_remove_during_launch_and_id_reuse(store):
+    # a graph removed while a launch that reads it is in flight: the frees
+    # are ordered after that launch, whose result is unchanged; the id is
+    # then reused by the next graph, which solves correctly
+    B, items = store
+    a = graphs.weighted_grid2(40, 30, np.random.default_rng(7), "expU2")
+    la, f = lapgpu.lap(a), lapgpu.build_ldli(a, seed=1)
+    gid = B.add_graph(la, f)
+    other = items[3][0]
+    bs = [_rhs(la.shape[0], 11), _rhs(B.sizes[other], 12)]
+    ref, _ = B.solve([gid, other], bs, tol=1e-8)
+    h = B.submit_host([gid, other], bs, tol=1e-8)
+    assert h is not None
+    B.remove_graph(gid)
+    a2 = graphs.grid2(25)
+    la2, f2 = lapgpu.lap(a2), lapgpu.build_ldli(a2, seed=1)
+    gid2 = B.add_graph(la2, f2)
+    assert gid2 == gid
+    xs, infos = B.collect(h)
+    for x, xr in zip(xs, ref):
+        assert np.linalg.norm(x - xr) <= 1e-12 * np.linalg.norm(xr)
+    b2 = _rhs(la2.shape[0], 13)
+    x2, i2 = B.solve([gid2], [b2], tol=1e-8)
+    assert i2[0]["stop"] == "converged"
+    assert np.linalg.norm(la2 @ x2[0] - b2) <= 1e-7 * np.linalg.norm(b2)
+    B.remove_graph(gid2)
