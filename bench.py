@@ -564,16 +564,20 @@ def mode_serve(args) -> int:
         log(f"  path {pth}: {len(sel)} ({newg} new graphs), latency p50 {np.percentile(lat, 50):.2f} s "
             f"p99 {np.percentile(lat, 99):.2f} s, last done at {done:.1f} s")
     log(f"  dispatch loop: {srv.stats}")
-    for name, lg in (("factor", srv.factor_log), ("context upload", srv.upload_log)):
-        if lg:
+    up = np.array(srv.upload_log).reshape(-1, 4)
+    for name, lg in (("factor", np.array(srv.factor_log)), ("context upload", up[:, 0]),
+                     ("  create", up[:, 1]), ("  upload la", up[:, 2]), ("  upload ldli", up[:, 3])):
+        if len(lg):
             log(f"  new graphs, {name}: {len(lg)} x, median {np.median(lg) * 1e3:.1f} ms, "
-                f"max {max(lg) * 1e3:.1f} ms, sum {sum(lg):.1f} s")
-    sl = np.array(srv.solve_log[len(G):]) if len(srv.solve_log) > len(G) else np.zeros((0, 3))
+                f"max {lg.max() * 1e3:.1f} ms, sum {lg.sum():.1f} s")
+    sl = np.array(srv.solve_log[len(G):], dtype=float) if len(srv.solve_log) > len(G) else np.zeros((0, 4))
     for lo_n, hi_n in ((0, 1e5), (1e5, 1e6), (1e6, 1e8)):
         m = (sl[:, 0] >= lo_n) & (sl[:, 0] < hi_n) if len(sl) else np.zeros(0, bool)
         if m.any():
+            f1 = m & (sl[:, 3] > 0)
             log(f"  solves n in [{lo_n:g},{hi_n:g}): {m.sum()} solves, {sl[m, 1].sum():.0f} columns, "
-                f"mean columns {sl[m, 1].mean():.1f}, {sl[m, 2].sum():.1f} s")
+                f"mean columns {sl[m, 1].mean():.1f}, {sl[m, 2].sum():.1f} s; first on a context: "
+                f"{f1.sum()}, {sl[f1, 2].sum():.1f} s")
     # plain one-at-a-time FP64 GPU PCG on the same GPU: the service time of
     # one plain solve per resident graph, then the same arrival stream
     # through a single FIFO server (new graphs: their own plain solve time,
@@ -659,7 +663,7 @@ def main() -> int:
     ap.add_argument("--factor-workers", type=int, default=8, help="serving: host threads factoring new graphs")
     ap.add_argument("--batch-window", type=float, default=2.0, help="serving: wait for more requests, x last solve time")
     ap.add_argument("--max-wait", type=float, default=3.0, help="serving: longest wait for more requests (s)")
-    ap.add_argument("--hw-queues", type=int, default=0, help="serving: HIP hardware queues (0: HIP's default)")
+    ap.add_argument("--hw-queues", type=int, default=16, help="serving: HIP hardware queues (0: HIP's default)")
     ap.add_argument("--calib", action="store_true",
                     help="plain mode: end with a device-to-device copy of 512 MiB (PMC calibration)")
     args = ap.parse_args()

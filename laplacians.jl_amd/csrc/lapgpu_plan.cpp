@@ -18,10 +18,123 @@
 #include "lapgpu_lock.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <cstdlib>
 #include <cstring>
+#include <functional>
+#include <mutex>
 #include <numeric>
+#include <thread>
 
 namespace lapgpu {
+
+namespace {
+
+// Host threads for the planner's parallel loops.  for_each(n, grain, fn)
+// calls fn(lo, hi) over [0, n) in chunks of `grain` indices, the calling
+// thread taking part; every chunk has finished when it returns.  Each chunk
+// writes outputs of its own, so results do not depend on the thread count.
+class Team {
+ public:
+  explicit Team(int threads) {
+    for (int t = 1; t < threads; ++t) pool_.emplace_back([this] { work(); });
+  }
+  ~Team() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      stop_ = true;
+      ++gen_;
+    }
+    cv_.notify_all();
+    for (auto& th : pool_) th.join();
+  }
+  int size() const { return (int)pool_.size() + 1; }
+  template <class F>
+  void for_each(int64_t n, int64_t grain, F&& fn) {
+    if (n <= 0) return;
+    if (pool_.empty() || n <= grain) {
+      fn(int64_t(0), n);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      job_ = [&fn](int64_t a, int64_t b) { fn(a, b); };
+      n_ = n;
+      grain_ = grain;
+      next_.store(0);
+      busy_ = (int)pool_.size();
+      ++gen_;
+    }
+    cv_.notify_all();
+    chunks();
+    std::unique_lock<std::mutex> lk(mu_);
+    done_.wait(lk, [&] { return busy_ == 0; });
+    job_ = nullptr;
+  }
+
+ private:
+  void chunks() {
+    for (;;) {
+      const int64_t lo = next_.fetch_add(grain_);
+      if (lo >= n_) return;
+      job_(lo, std::min(n_, lo + grain_));
+    }
+  }
+  void work() {
+    uint64_t seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return gen_ != seen; });
+        seen = gen_;
+        if (stop_) return;
+      }
+      chunks();
+      std::lock_guard<std::mutex> g(mu_);
+      if (--busy_ == 0) done_.notify_one();
+    }
+  }
+  std::vector<std::thread> pool_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+  std::function<void(int64_t, int64_t)> job_;
+  int64_t n_ = 0, grain_ = 1;
+  std::atomic<int64_t> next_{0};
+  int busy_ = 0;
+};
+
+// Threads for one planning call: LAPGPU_PLAN_THREADS (default: the
+// hardware threads, at most 16) shared among the calls running at once.
+// A nested call (build_tail inside build_plan) keeps its caller's share.
+std::atomic<int> g_active_plans{0};
+thread_local int tl_plan_threads = 0;
+
+struct PlanThreads {
+  int n;
+  bool outer;
+  PlanThreads() : outer(tl_plan_threads == 0) {
+    if (!outer) {
+      n = tl_plan_threads;
+      return;
+    }
+    const int active = ++g_active_plans;
+    int base = (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+    if (const char* e = getenv("LAPGPU_PLAN_THREADS")) base = std::max(1, std::min(64, atoi(e)));
+    n = std::max(1, base / active);
+    tl_plan_threads = n;
+  }
+  ~PlanThreads() {
+    if (outer) {
+      tl_plan_threads = 0;
+      --g_active_plans;
+    }
+  }
+};
+
+}  // namespace
 
 static constexpr int64_t kMaxI32 = 2147483647LL;
 
@@ -224,64 +337,108 @@ bool build_tail(Plan& P, int64_t max_steps, int64_t cap, int max_elog, int lanes
     ent += fin + bin;
   }
   T.elog = choose_elog(rows, ent, max_elog);
-  bool ok = true;
-  for (int64_t l = t; l < P.nlev; ++l) {
+  // levels pack independently (each into its own steps and slots, offsets
+  // local), then concatenate in sweep order
+  struct LevelProg {
+    std::vector<TailStep> steps;
+    std::vector<TailSlot> slots;
+  };
+  const int64_t nl = P.nlev - t;
+  std::vector<LevelProg> fp(nl), bp(nl);
+  std::atomic<bool> ok{true};
+  PlanThreads threads;
+  Team team(threads.n);
+  team.for_each(2 * nl, 4, [&](int64_t lo, int64_t hi) {
     std::vector<RowEntries> rws;
-    for (int64_t pos = P.lvl[l]; pos < P.lvl[l + 1]; ++pos) {
-      RowEntries R;
-      R.row = (int32_t)(pos - T0);
-      for (int32_t k = P.Fptr[pos]; k < P.Fptr[pos + 1]; ++k)
-        if (P.Fidx[k] >= T0) R.e.emplace_back(P.Fidx[k] - T0, P.Fw[k]);
-      rws.push_back(std::move(R));
-    }
-    pack_level(rws, T.elog, lanes, T.fsteps, T.fslots);
-  }
-  for (int64_t l = P.nlev - 1; l >= t && ok; --l) {
-    std::vector<RowEntries> rws;
-    for (int64_t pos = P.lvl[l]; pos < P.lvl[l + 1]; ++pos) {
-      RowEntries R;
-      R.row = (int32_t)(pos - T0);
-      for (int32_t k = P.Bptr[pos]; k < P.Bptr[pos + 1]; ++k) {
-        if (P.Bidx[k] < T0) ok = false;
-        R.e.emplace_back(P.Bidx[k] - T0, P.Bw[k]);
+    for (int64_t q = lo; q < hi; ++q) {
+      const bool fwd = q < nl;
+      const int64_t l = t + (fwd ? q : q - nl);
+      rws.clear();
+      for (int64_t pos = P.lvl[l]; pos < P.lvl[l + 1]; ++pos) {
+        RowEntries R;
+        R.row = (int32_t)(pos - T0);
+        if (fwd) {
+          for (int32_t k = P.Fptr[pos]; k < P.Fptr[pos + 1]; ++k)
+            if (P.Fidx[k] >= T0) R.e.emplace_back(P.Fidx[k] - T0, P.Fw[k]);
+        } else {
+          for (int32_t k = P.Bptr[pos]; k < P.Bptr[pos + 1]; ++k) {
+            if (P.Bidx[k] < T0) ok = false;
+            R.e.emplace_back(P.Bidx[k] - T0, P.Bw[k]);
+          }
+        }
+        rws.push_back(std::move(R));
       }
-      rws.push_back(std::move(R));
+      LevelProg& G = fwd ? fp[l - t] : bp[l - t];
+      pack_level(rws, T.elog, lanes, G.steps, G.slots);
     }
-    pack_level(rws, T.elog, lanes, T.bsteps, T.bslots);
+  });
+  // forward sweep: levels ascending; backward: descending
+  auto append = [](const LevelProg& G, std::vector<TailStep>& steps,
+                   std::vector<TailSlot>& slots) {
+    const int32_t base = (int32_t)slots.size();
+    for (TailStep st : G.steps) {
+      st.off += base;
+      steps.push_back(st);
+    }
+    slots.insert(slots.end(), G.slots.begin(), G.slots.end());
+  };
+  size_t fs = 0, fl = 0, bs = 0, bl = 0;
+  for (int64_t q = 0; q < nl; ++q) {
+    fs += fp[q].steps.size();
+    fl += fp[q].slots.size();
+    bs += bp[q].steps.size();
+    bl += bp[q].slots.size();
   }
-  return ok;
+  T.fsteps.reserve(fs);
+  T.fslots.reserve(fl);
+  T.bsteps.reserve(bs);
+  T.bslots.reserve(bl);
+  for (int64_t q = 0; q < nl; ++q) append(fp[q], T.fsteps, T.fslots);
+  for (int64_t q = nl - 1; q >= 0; --q) append(bp[q], T.bsteps, T.bslots);
+  return ok.load();
 }
 
 namespace {
 
-// Sparse accumulator over [0, 2n) for composite rows.
-struct Accum {
-  std::vector<double> val;
-  std::vector<int32_t> mark, touched;
-  int32_t stamp = 0;
-  explicit Accum(int64_t size) : val(size, 0.0), mark(size, -1) {}
+// One composite row: its terms in insertion order, merged by index.  The
+// sort is stable in effect (the key holds the insertion number), so equal
+// indices are summed in insertion order starting from the first term: the
+// same sums, bit for bit, as a dense accumulator filled in that order.
+struct RowMerge {
+  struct Term {
+    int64_t key;  // index << 32 | insertion number
+    double w;
+  };
+  std::vector<Term> t;
+  uint32_t seq = 0;
   void begin() {
-    ++stamp;
-    touched.clear();
+    t.clear();
+    seq = 0;
   }
-  void add(int32_t i, double w) {
-    if (mark[i] != stamp) {
-      mark[i] = stamp;
-      val[i] = 0.0;
-      touched.push_back(i);
+  void add(int32_t i, double w) { t.push_back({((int64_t)i << 32) | seq++, w}); }
+  // append the merged row; returns its length
+  int32_t take(std::vector<int32_t>& idx, std::vector<double>& w) {
+    std::sort(t.begin(), t.end(), [](const Term& a, const Term& b) { return a.key < b.key; });
+    int32_t len = 0;
+    for (size_t k = 0; k < t.size();) {
+      const int32_t i = (int32_t)(t[k].key >> 32);
+      double sum = t[k].w;
+      for (++k; k < t.size() && (int32_t)(t[k].key >> 32) == i; ++k) sum += t[k].w;
+      idx.push_back(i);
+      w.push_back(sum);
+      ++len;
     }
-    val[i] += w;
+    return len;
   }
 };
 
-using Rep = std::vector<std::pair<int32_t, double>>;
-
-void take(Accum& A, Rep& out) {
-  std::sort(A.touched.begin(), A.touched.end());
-  out.clear();
-  out.reserve(A.touched.size());
-  for (int32_t i : A.touched) out.emplace_back(i, A.val[i]);
-}
+// Rows of one level computed in chunks of consecutive positions; the
+// chunks' outputs are concatenated in position order.
+struct RowChunk {
+  std::vector<int32_t> idx, len;
+  std::vector<double> w;
+  int64_t orig = 0;
+};
 
 }  // namespace
 
@@ -297,32 +454,90 @@ void build_blocks(Plan& P, int max_depth, double growth, int64_t max_extra) {
     B.lev = {0};
     return;
   }
-  Accum acc(2 * n);
-  std::vector<Rep> frep(T0), brep(T0);
-  // forward composites and the grouping (forward reps of a group's earlier
-  // levels do not change when the group grows at its end)
+  PlanThreads threads;
+  Team team(threads.n);
+  constexpr int64_t kRows = 256;
+  std::vector<RowChunk> outs;
+  // grow v to n elements, doubling its capacity when it must grow
+  auto grow_to = [](auto& v, size_t n) {
+    if (n > v.capacity()) v.reserve(std::max(n, 2 * v.capacity()));
+    v.resize(n);
+  };
+  // append the chunks' rows [lo, lo + rows) to (ptr, idx, w): ptr[v + 1]
+  // (relative to `origin`) gets each row's end, the copies run in parallel
+  std::vector<int64_t> at;
+  auto append_rows = [&](int64_t nc, int64_t lo, std::vector<int64_t>& ends,
+                         int64_t origin, std::vector<int32_t>& idx, std::vector<double>& w) {
+    at.assign(nc + 1, (int64_t)idx.size());
+    for (int64_t c = 0; c < nc; ++c) at[c + 1] = at[c] + (int64_t)outs[c].idx.size();
+    grow_to(idx, (size_t)at[nc]);
+    grow_to(w, (size_t)at[nc]);
+    team.for_each(nc, 1, [&](int64_t c0, int64_t c1) {
+      for (int64_t c = c0; c < c1; ++c) {
+        const RowChunk& O = outs[c];
+        std::copy(O.idx.begin(), O.idx.end(), idx.begin() + at[c]);
+        std::copy(O.w.begin(), O.w.end(), w.begin() + at[c]);
+        int64_t e = at[c], v = lo + c * kRows - origin;
+        for (int32_t len : O.len) {
+          e += len;
+          ends[v++] = e;
+        }
+      }
+    });
+  };
+  // rows [lo, hi): row(v, rm) adds v's terms and returns its original
+  // entries; returns the number of chunks
+  auto run_rows = [&](int64_t lo, int64_t hi, const auto& row) {
+    const int64_t nc = (hi - lo + kRows - 1) / kRows;
+    if ((int64_t)outs.size() < nc) outs.resize(nc);
+    team.for_each(nc, 1, [&](int64_t c0, int64_t c1) {
+      RowMerge rm;
+      for (int64_t c = c0; c < c1; ++c) {
+        RowChunk& O = outs[c];
+        O.idx.clear();
+        O.w.clear();
+        O.len.clear();
+        O.orig = 0;
+        const int64_t a = lo + c * kRows, b = std::min(hi, a + kRows);
+        for (int64_t v = a; v < b; ++v) {
+          rm.begin();
+          O.orig += row(v, rm);
+          O.len.push_back(rm.take(O.idx, O.w));
+        }
+      }
+    });
+    return nc;
+  };
+
+  // forward composites and the grouping (forward rows of a group's earlier
+  // levels do not change when the group grows at its end); rows are stored
+  // in position order as each level is settled
   int64_t L0 = 0;
   B.lev.push_back(0);
   int64_t orig_block = 0, comp_block = 0;
+  std::vector<int64_t> fend(T0 + 1, 0);  // row ends, position v at v + 1
   for (int64_t l = 0; l < L; ++l) {
     const int64_t lo = P.lvl[l], hi = P.lvl[l + 1];
-    int64_t orig = 0, comp = 0;
+    int64_t nc = 0, orig = 0, comp = 0;
     auto compose = [&](int64_t start) {
-      orig = comp = 0;
-      for (int64_t v = lo; v < hi; ++v) {
-        acc.begin();
-        acc.add((int32_t)(n + v), 1.0);
+      const int32_t s0 = P.lvl[start];
+      nc = run_rows(lo, hi, [&](int64_t v, RowMerge& rm) -> int64_t {
+        rm.add((int32_t)(n + v), 1.0);
         for (int32_t k = P.Fptr[v]; k < P.Fptr[v + 1]; ++k) {
           const int32_t c = P.Fidx[k];
-          if (c < P.lvl[start]) {
-            acc.add(c, P.Fw[k]);
+          if (c < s0) {
+            rm.add(c, P.Fw[k]);
           } else {
-            for (const auto& e : frep[c]) acc.add(e.first, P.Fw[k] * e.second);
+            for (int32_t e = B.fptr[c]; e < B.fptr[c + 1]; ++e)
+              rm.add(B.fidx[e], P.Fw[k] * B.fw[e]);
           }
         }
-        take(acc, frep[v]);
-        orig += P.Fptr[v + 1] - P.Fptr[v] + 1;
-        comp += (int64_t)frep[v].size();
+        return P.Fptr[v + 1] - P.Fptr[v] + 1;
+      });
+      orig = comp = 0;
+      for (int64_t c = 0; c < nc; ++c) {
+        orig += outs[c].orig;
+        comp += (int64_t)outs[c].idx.size();
       }
     };
     bool joined = false;
@@ -342,45 +557,56 @@ void build_blocks(Plan& P, int max_depth, double growth, int64_t max_extra) {
       orig_block = orig;
       comp_block = comp;
     }
+    append_rows(nc, lo, fend, -1, B.fidx, B.fw);
+    for (int64_t v = lo; v < hi; ++v) B.fptr[v + 1] = (int32_t)fend[v + 1];
   }
   B.lev.push_back((int32_t)L);
-  // backward composites per group (top level first)
+  // backward composites per group (top level first), kept per group and
+  // then stored in position order
+  std::vector<int64_t> gend;  // group rows: end of position base + i at i
+  std::vector<int32_t> glen, gidx;
+  std::vector<double> gw;
   for (size_t g = 0; g + 1 < B.lev.size(); ++g) {
     const int64_t g0 = B.lev[g], g1 = B.lev[g + 1];
-    const int64_t top = P.lvl[g1];
+    const int32_t top = P.lvl[g1];
+    const int64_t base = P.lvl[g0];
+    gend.assign(top - base, 0);
+    glen.assign(top - base, 0);
+    gidx.clear();
+    gw.clear();
     for (int64_t l = g1 - 1; l >= g0; --l) {
-      for (int64_t v = P.lvl[l]; v < P.lvl[l + 1]; ++v) {
-        acc.begin();
-        acc.add((int32_t)(n + v), P.s[v]);
+      const int64_t lo = P.lvl[l], hi = P.lvl[l + 1];
+      const int64_t nc = run_rows(lo, hi, [&](int64_t v, RowMerge& rm) -> int64_t {
+        rm.add((int32_t)(n + v), P.s[v]);
         for (int32_t k = P.Bptr[v]; k < P.Bptr[v + 1]; ++k) {
           const int32_t j = P.Bidx[k];
           if (j >= top) {
-            acc.add(j, P.Bw[k]);
+            rm.add(j, P.Bw[k]);
           } else {
-            for (const auto& e : brep[j]) acc.add(e.first, P.Bw[k] * e.second);
+            const int64_t o = gend[j - base] - glen[j - base];
+            for (int32_t e = 0; e < glen[j - base]; ++e)
+              rm.add(gidx[o + e], P.Bw[k] * gw[o + e]);
           }
         }
-        take(acc, brep[v]);
+        return 0;
+      });
+      append_rows(nc, lo, gend, base, gidx, gw);
+      for (int64_t c = 0; c < nc; ++c) {
+        int64_t v = lo + c * kRows - base;
+        for (int32_t len : outs[c].len) glen[v++] = len;
       }
     }
-  }
-  for (int64_t v = 0; v < T0; ++v) {
-    B.fptr[v + 1] = B.fptr[v] + (int32_t)frep[v].size();
-    B.bptr[v + 1] = B.bptr[v] + (int32_t)brep[v].size();
-  }
-  B.fidx.reserve(B.fptr[T0]);
-  B.fw.reserve(B.fptr[T0]);
-  B.bidx.reserve(B.bptr[T0]);
-  B.bw.reserve(B.bptr[T0]);
-  for (int64_t v = 0; v < T0; ++v) {
-    for (const auto& e : frep[v]) {
-      B.fidx.push_back(e.first);
-      B.fw.push_back(e.second);
-    }
-    for (const auto& e : brep[v]) {
-      B.bidx.push_back(e.first);
-      B.bw.push_back(e.second);
-    }
+    // the group's rows in position order
+    for (int64_t v = base; v < top; ++v) B.bptr[v + 1] = B.bptr[v] + glen[v - base];
+    grow_to(B.bidx, (size_t)B.bptr[top]);
+    grow_to(B.bw, (size_t)B.bptr[top]);
+    team.for_each(top - base, 1024, [&](int64_t a, int64_t b) {
+      for (int64_t i = a; i < b; ++i) {
+        const int64_t o = gend[i] - glen[i];
+        std::copy(gidx.begin() + o, gidx.begin() + o + glen[i], B.bidx.begin() + B.bptr[base + i]);
+        std::copy(gw.begin() + o, gw.begin() + o + glen[i], B.bw.begin() + B.bptr[base + i]);
+      }
+    });
   }
 }
 
@@ -417,23 +643,50 @@ bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
       root = v;
       elim[v] = (int32_t)(n - 1);
     }
+  PlanThreads threads;
+  Team team(threads.n);
+  constexpr int64_t kGrain = 4096;
   if (n > 1) {
     if (colptr[0] != base || colptr[n - 1] - base != nnzF) {
       err = "upload_ldli: colptr does not span nnzF entries";
       return false;
     }
-    for (int64_t ii = 0; ii < n - 1; ++ii) {
-      if (colptr[ii + 1] <= colptr[ii]) {
-        err = "upload_ldli: empty or non-monotone column";
-        return false;
-      }
-      for (int64_t k = colptr[ii] - base; k < colptr[ii + 1] - base; ++k) {
-        const int64_t j = rowval[k] - base;
-        if (j < 0 || j >= n || elim[j] <= ii) {
-          err = "upload_ldli: a row of a column is not eliminated later";
-          return false;
+    // the first bad column (in column order) names the error, as a
+    // sequential scan would
+    int64_t bad_col = INT64_MAX;
+    int bad_kind = 0;
+    std::mutex bad_mu;
+    team.for_each(n - 1, kGrain, [&](int64_t lo, int64_t hi) {
+      for (int64_t ii = lo; ii < hi; ++ii) {
+        int kind = 0;
+        if (colptr[ii + 1] <= colptr[ii]) {
+          kind = 1;
+        } else {
+          for (int64_t k = colptr[ii] - base; k < colptr[ii + 1] - base; ++k) {
+            const int64_t j = rowval[k] - base;
+            if (j < 0 || j >= n || elim[j] <= ii) {
+              kind = 2;
+              break;
+            }
+          }
+        }
+        if (kind) {
+          std::lock_guard<std::mutex> g(bad_mu);
+          if (ii < bad_col) {
+            bad_col = ii;
+            bad_kind = kind;
+          }
+          return;
         }
       }
+    });
+    if (bad_kind == 1) {
+      err = "upload_ldli: empty or non-monotone column";
+      return false;
+    }
+    if (bad_kind == 2) {
+      err = "upload_ldli: a row of a column is not eliminated later";
+      return false;
     }
   }
   (void)root;
@@ -443,16 +696,18 @@ bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
   // ---- per-entry weights and per-column scale -----------------------------
   std::vector<double> w(nnzF);
   std::vector<double> Pc(n, 1.0);
-  for (int64_t ii = 0; ii < n - 1; ++ii) {
-    const int64_t k0 = colptr[ii] - base, k1 = colptr[ii + 1] - base - 1;
-    double prod = 1.0;
-    for (int64_t k = k0; k < k1; ++k) {
-      w[k] = fval[k] * prod;
-      prod *= (1.0 - fval[k]);
+  team.for_each(n - 1, kGrain, [&](int64_t lo, int64_t hi) {
+    for (int64_t ii = lo; ii < hi; ++ii) {
+      const int64_t k0 = colptr[ii] - base, k1 = colptr[ii + 1] - base - 1;
+      double prod = 1.0;
+      for (int64_t k = k0; k < k1; ++k) {
+        w[k] = fval[k] * prod;
+        prod *= (1.0 - fval[k]);
+      }
+      w[k1] = prod;
+      Pc[col[ii] - base] = prod;
     }
-    w[k1] = prod;
-    Pc[col[ii] - base] = prod;
-  }
+  });
 
   // ---- forward levels (longest path from a source) ------------------------
   std::vector<int32_t> lev(n, 0);
@@ -492,9 +747,9 @@ bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
     const int32_t v = P.perm[pos];
     P.Lptr[pos + 1] = P.Lptr[pos] + (L.ptr[v + 1] - L.ptr[v]);
   }
-  {
+  team.for_each(n, kGrain, [&](int64_t lo, int64_t hi) {
     std::vector<std::pair<int32_t, double>> row;
-    for (int64_t pos = 0; pos < n; ++pos) {
+    for (int64_t pos = lo; pos < hi; ++pos) {
       const int32_t v = P.perm[pos];
       row.clear();
       for (int32_t k = L.ptr[v]; k < L.ptr[v + 1]; ++k)
@@ -508,7 +763,7 @@ bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
         ++o;
       }
     }
-  }
+  });
 
   // ---- backward rows: column of v, in column order ------------------------
   P.Bptr.assign(n + 1, 0);
@@ -520,34 +775,70 @@ bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
     P.Bptr[P.iperm[v] + 1] = (int32_t)(colptr[ii + 1] - colptr[ii]);
   }
   for (int64_t pos = 0; pos < n; ++pos) P.Bptr[pos + 1] += P.Bptr[pos];
-  for (int64_t ii = 0; ii < n - 1; ++ii) {
-    const int64_t v = col[ii] - base;
-    int32_t o = P.Bptr[P.iperm[v]];
-    for (int64_t k = colptr[ii] - base; k < colptr[ii + 1] - base; ++k) {
-      P.Bidx[o] = P.iperm[rowval[k] - base];
-      P.Bw[o] = w[k];
-      ++o;
+  // destination row (position) of every factor entry, for both directions
+  std::vector<int32_t> dest(nnzF);
+  team.for_each(n - 1, kGrain, [&](int64_t lo, int64_t hi) {
+    for (int64_t ii = lo; ii < hi; ++ii) {
+      const int64_t v = col[ii] - base;
+      int32_t o = P.Bptr[P.iperm[v]];
+      for (int64_t k = colptr[ii] - base; k < colptr[ii + 1] - base; ++k) {
+        const int32_t j = P.iperm[rowval[k] - base];
+        dest[k] = j;
+        P.Bidx[o] = j;
+        P.Bw[o] = w[k];
+        ++o;
+      }
+      const double dv = d[v];
+      P.s[P.iperm[v]] = dv != 0.0 ? Pc[v] * Pc[v] / dv : Pc[v] * Pc[v];
     }
-    const double dv = d[v];
-    P.s[P.iperm[v]] = dv != 0.0 ? Pc[v] * Pc[v] / dv : Pc[v] * Pc[v];
-  }
+  });
 
   // ---- forward rows: in-edges of v, in elimination order of the source ----
   P.Fptr.assign(n + 1, 0);
   P.Fidx.resize(nnzF);
   P.Fw.resize(nnzF);
-  for (int64_t k = 0; k < nnzF; ++k) P.Fptr[P.iperm[rowval[k] - base] + 1]++;
-  for (int64_t pos = 0; pos < n; ++pos) P.Fptr[pos + 1] += P.Fptr[pos];
+  // a transpose: the columns are split into contiguous ranges, one per
+  // part; each part counts its entries per row, the counts give every
+  // (part, row) its own cursor, and each part fills its columns in order,
+  // so a row holds its entries in column (elimination) order
   {
-    std::vector<int32_t> cur(P.Fptr.begin(), P.Fptr.end() - 1);
-    for (int64_t ii = 0; ii < n - 1; ++ii) {
-      const int32_t src = P.iperm[col[ii] - base];
-      for (int64_t k = colptr[ii] - base; k < colptr[ii + 1] - base; ++k) {
-        const int32_t o = cur[P.iperm[rowval[k] - base]]++;
-        P.Fidx[o] = src;
-        P.Fw[o] = w[k];
+    const int64_t parts = std::min<int64_t>(team.size(), 8);
+    std::vector<int32_t> cur((size_t)parts * n, 0);
+    auto range = [&](int64_t q) { return (n - 1) * q / parts; };
+    team.for_each(parts, 1, [&](int64_t q0, int64_t q1) {
+      for (int64_t q = q0; q < q1; ++q) {
+        int32_t* c = cur.data() + q * n;
+        for (int64_t k = colptr[range(q)] - base; k < colptr[range(q + 1)] - base; ++k) ++c[dest[k]];
       }
+    });
+    for (int64_t pos = 0; pos < n; ++pos) {
+      int32_t tot = 0;
+      for (int64_t q = 0; q < parts; ++q) tot += cur[q * n + pos];
+      P.Fptr[pos + 1] = P.Fptr[pos] + tot;
     }
+    team.for_each(n, kGrain, [&](int64_t lo, int64_t hi) {
+      for (int64_t pos = lo; pos < hi; ++pos) {
+        int32_t at = P.Fptr[pos];
+        for (int64_t q = 0; q < parts; ++q) {
+          const int32_t c = cur[q * n + pos];
+          cur[q * n + pos] = at;
+          at += c;
+        }
+      }
+    });
+    team.for_each(parts, 1, [&](int64_t q0, int64_t q1) {
+      for (int64_t q = q0; q < q1; ++q) {
+        int32_t* c = cur.data() + q * n;
+        for (int64_t ii = range(q); ii < range(q + 1); ++ii) {
+          const int32_t src = P.iperm[col[ii] - base];
+          for (int64_t k = colptr[ii] - base; k < colptr[ii + 1] - base; ++k) {
+            const int32_t o = c[dest[k]]++;
+            P.Fidx[o] = src;
+            P.Fw[o] = w[k];
+          }
+        }
+      }
+    });
   }
   if (!build_tail(P, max_tail_steps, kTailCap)) {
     err = "upload_ldli: internal error packing the tail program";

@@ -26,6 +26,12 @@ over time; each is answered by the same solve closure as approxchol_lap's
 
 Every result carries its latency (arrival to completion); `summary` gives
 solves/s and p50/p99 latency.
+
+Every graph context owns a stream, and HIP maps streams round-robin onto its
+hardware queues (4 unless GPU_MAX_HW_QUEUES says otherwise): a small solve on
+a stream that shares a queue with a long multi-RHS solve waits for it.  A
+process that serves many graphs should set GPU_MAX_HW_QUEUES=16 before its
+first HIP call (more than 16 queues slowed plain solves on MI355X).
 """
 
 from __future__ import annotations
@@ -78,6 +84,7 @@ class _Graph:
     max_rhs: int = 1  # columns per multi-RHS solve on this graph
     last_solve_s: float = 0.0  # duration of the graph's last solve
     ready_at: float = 0.0  # an idle graph with few requests waits until then
+    solves: int = 0  # plain / multi-RHS solves run on the graph's context
 
 
 def graph_bytes_estimate(n: int, nnz_l: int, nnzf: int) -> int:
@@ -113,10 +120,12 @@ class Server:
         # that has fewer than max_rhs requests queued collects more first
         self.batch_window = batch_window
         self.max_wait = max_wait
-        self.solve_log: list = []  # (n, columns, seconds) of each plain / multi-RHS solve
+        # (n, columns, seconds, first on its context) of each plain / multi-RHS solve
+        self.solve_log: list = []
         # seconds the dispatch loop spent admitting and dropping graphs
         self.stats = {"admit_s": 0.0, "drop_s": 0.0, "admits": 0, "drops": 0}
-        # worker seconds per new graph: factoring, context upload
+        # worker seconds per new graph: factoring; context upload (total,
+        # create, Laplacian, factor)
         self.factor_log: list = []
         self.upload_log: list = []
         self.graphs: "OrderedDict[object, _Graph]" = OrderedDict()
@@ -157,11 +166,14 @@ class Server:
         return need, k
 
     def _upload_context(self, la, ldli):
-        t = time.perf_counter()
+        t0 = time.perf_counter()
         ctx = _native.Context(self.device)
+        t1 = time.perf_counter()
         ctx.upload_lap(la)
+        t2 = time.perf_counter()
         ctx.upload_ldli(ldli)
-        self.upload_log.append(time.perf_counter() - t)
+        t3 = time.perf_counter()
+        self.upload_log.append((t3 - t0, t1 - t0, t2 - t1, t3 - t2))
         return ctx
 
     def _admit_wg(self, key, la, ldli, resident) -> _Graph:
@@ -255,7 +267,8 @@ class Server:
                 B = np.stack([r.b for r in take], axis=1)
                 x, info = g.handle.solve_multi(B, r0.tol, r0.maxits, float("inf"), 5, 0)
             dt = time.perf_counter() - t_start
-            self.solve_log.append((g.n, len(take), dt))
+            self.solve_log.append((g.n, len(take), dt, g.solves == 0))
+            g.solves += 1
             g.last_solve_s = dt
             return take, x, info
 
