@@ -1,0 +1,56 @@
+"""The host planner (laplacians.jl_amd/csrc/lapgpu_plan.cpp) gives the same
+plan, bit for bit, for any number of planner threads: every plan array
+(level order, permuted la, both sweep layouts, level-group composite rows,
+tail slot programs) is hashed by a small driver built from the planner's
+source with g++ and compared between 1 and 4 threads."""
+
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import lapgpu
+from lapgpu import graphs
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "laplacians.jl_amd", "csrc")
+
+
+@pytest.fixture(scope="module")
+def driver(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("plan") / "plan_hash")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", f"-I{ROOT}/include", f"-I{CSRC}", "-o", exe,
+                    os.path.join(ROOT, "tests", "native", "plan_hash.cpp"), os.path.join(CSRC, "lapgpu_plan.cpp")],
+                   check=True)
+    return exe
+
+
+def _dump(a, d):
+    la = lapgpu.lap(a)
+    f = lapgpu.build_ldli(a, seed=1)
+    os.makedirs(d, exist_ok=True)
+    for k, v in (("cp", la.indptr), ("rv", la.indices), ("nz", la.data), ("col", f.col), ("fcp", f.colptr),
+                 ("frv", f.rowval), ("fv", f.fval), ("d", f.d)):
+        np.ascontiguousarray(v, dtype=np.float64 if k in ("nz", "fv", "d") else np.int64).tofile(os.path.join(d, k))
+
+
+@pytest.mark.parametrize("name", ["grid2", "grid3", "path_matchings"])
+def test_plan_same_for_any_thread_count(driver, tmp_path, name):
+    rng = np.random.default_rng(11)
+    a = {"grid2": lambda: graphs.weighted_grid2(300, 200, rng, "expU2"),
+         "grid3": lambda: graphs.weighted_grid3(40, 40, 40, rng, "expU2"),
+         "path_matchings": lambda: graphs.path_matchings(60000, 4, rng, "logU3")}[name]()
+    d = str(tmp_path / name)
+    _dump(a, d)
+    outs = []
+    for t in (1, 4):
+        r = subprocess.run([driver, d], env={**os.environ, "LAPGPU_PLAN_THREADS": str(t)},
+                           capture_output=True, text=True, check=True)
+        outs.append(r.stdout)
+    assert outs[0] == outs[1]
+    # the grid cases need level groups before the tail to cover the composite rows
+    last = outs[0].strip().splitlines()[-1].split()
+    assert int(last[1]) > 1 and int(last[3]) > 0
+    if name != "path_matchings":
+        assert int(last[3]) > 1, "expected several level groups"
