@@ -35,6 +35,32 @@ int lapfactor_build(int64_t n, const int64_t* colptr, const int64_t* rowval,
                     const double* nzval, int index_base, uint64_t seed,
                     lapfactor_ldli** out);
 
+/* The fixed-order elimination approxChol(LLMatOrd(a, p))
+ * (src/approxChol.jl:203-231, 265-296, 758-862), which ApproxCholParams
+ * orders :given (p = identity), :wdeg (p = sortperm of weighted degree times
+ * 1 + U(0,1)) and :random (a random permutation) use
+ * (src/approxChol.jl:1628-1790).  perm[t] is the vertex eliminated t-th
+ * (NULL: the given order 0..n-1); `index_base` applies to perm too.  The
+ * factor is returned in the graph's own vertex labels (col, rowval and d
+ * relabelled as the reference's ldlip), so it uploads like any other.  Same
+ * RNG stream and error codes as lapfactor_build. */
+int lapfactor_build_ordered(int64_t n, const int64_t* colptr, const int64_t* rowval,
+                            const double* nzval, int index_base, const int64_t* perm,
+                            uint64_t seed, lapfactor_ldli** out);
+
+/* The split-edge elimination approxChol(LLmatp(a, split), split, merge)
+ * (src/approxChol.jl:138-186, 541-603, 979-1157, 2121-2144) that
+ * approxchol_lapGreedy runs when ApproxCholParams has split >= 1 and
+ * merge >= 1, and approxchol_lap2 requests (src/approxChol.jl:1559-1579).
+ * Every edge starts as `split` copies of weight w/split; a column's copies
+ * of one row share their mean weight, at most `merge` of them are kept
+ * (merge < 1: all of them, the split-only elimination of
+ * src/approxChol.jl:1162-1339), and the factor holds one entry per row.
+ * Same RNG stream and error codes as lapfactor_build. */
+int lapfactor_build_split(int64_t n, const int64_t* colptr, const int64_t* rowval,
+                          const double* nzval, int index_base, int64_t split, int64_t merge,
+                          uint64_t seed, lapfactor_ldli** out);
+
 /* Number of vertices n and of factor entries nnzF = len(fval). */
 int64_t lapfactor_n(const lapfactor_ldli* f);
 int64_t lapfactor_nnz(const lapfactor_ldli* f);

@@ -1,8 +1,8 @@
 # LapGPU.jl — Julia host side of the MI355X solve phase of approxchol_lap.
 #
 # The randomized elimination that builds LDLinv stays in Laplacians.jl's own
-# code (approxChol(LLmatp(a)), src/approxChol.jl:864-977) with the caller's
-# seed; only the solve f(b) moves to the GPU, through ccall into liblapgpu.so
+# code (approxChol(LLmatp(a)), src/approxChol.jl:864-977, or the builder its
+# ApproxCholParams names) with the caller's seed; only the solve f(b) moves to the GPU, through ccall into liblapgpu.so
 # (include/lapgpu.h).  The returned closure has the signature and keyword
 # semantics of the reference's (src/approxChol.jl:1624), and the component
 # handling is the reference's own lapWrapComponents
@@ -18,7 +18,7 @@ using Laplacians
 using SparseArrays
 using Random
 
-export approxchol_lap_gpu, approxchol_sddm_gpu
+export approxchol_lap_gpu, approxchol_lap2_gpu, approxchol_sddm_gpu, harmonic_interp_gpu
 
 const libpath = joinpath(@__DIR__, "..", "lapgpu", "liblapgpu.so")
 
@@ -76,17 +76,39 @@ function gpu_solve(ctx::Ctx, b::Vector{Float64}, tol, maxits, maxtime, verbose,
     return x
 end
 
+# The reference's own factor builders for each ApproxCholParams
+# (approxchol_lapGreedy / Wdeg / Random / Given, src/approxChol.jl:1581-1790),
+# fixed-order factors relabelled to the graph's vertices as the reference's
+# ldlip is
+function build_ldli(a::SparseMatrixCSC, params)
+    if params.order == :deg
+        if params.split >= 1 && params.merge < 1
+            return Laplacians.approxChol(Laplacians.LLmatp(a, params.split), params.split)
+        elseif params.split >= 1 && params.merge >= 1
+            return Laplacians.approxChol(Laplacians.LLmatp(a, params.split), params.split,
+                                         params.merge)
+        end
+        return Laplacians.approxChol(Laplacians.LLmatp(a))
+    elseif params.order == :wdeg || params.order == :random
+        n = size(a, 1)
+        p = params.order == :wdeg ? sortperm(vec(sum(a, dims = 1)) .* (1 .+ rand(n))) :
+            randperm(n)
+        ldli = Laplacians.approxChol(Laplacians.LLMatOrd(a, p))
+        return Laplacians.LDLinv(p[ldli.col], ldli.colptr, p[ldli.rowval], ldli.fval,
+                                 ldli.d[invperm(p)])
+    end
+    return Laplacians.approxChol(Laplacians.LLMatOrd(a))
+end
+
 # approxchol_lap1 for a connected graph: the reference's factor on the host,
-# the solve on the GPU (src/approxChol.jl:1581-1626)
+# the solve on the GPU (src/approxChol.jl:1791-1851)
 function approxchol_lap1_gpu(a::SparseMatrixCSC; tol::Real = 1e-6, maxits = 1000,
                              maxtime = Inf, verbose = false, pcgIts = Int[],
                              params = ApproxCholParams(), seed::Integer = 1,
                              device::Integer = 0)
-    params.order == :deg && params.split < 1 ||
-        error("approxchol_lap_gpu: only ApproxCholParams(:deg) is wired here")
     la = lap(a)
     Random.seed!(seed)
-    ldli = Laplacians.approxChol(Laplacians.LLmatp(a))
+    ldli = build_ldli(a, params)
     ctx = create(device)
     upload_lap!(ctx, la)
     upload_ldli!(ctx, ldli)
@@ -113,6 +135,33 @@ function approxchol_lap_gpu(a::SparseMatrixCSC{Tv,Ti}; tol::Real = 1e-6, maxits 
                                         maxtime = maxtime, pcgIts = pcgIts, params = params)
 end
 
+# approxchol_lap2 (src/approxChol.jl:1559-1579), with the same parameters
+function approxchol_lap2_gpu(a::SparseMatrixCSC{Tv,Ti}; tol::Real = 1e-6, maxits = 1000,
+                             maxtime = Inf, verbose = false, pcgIts = Int[],
+                             params = ApproxCholParams(), kwargs...) where {Tv,Ti}
+    if minimum(a.nzval) < 0
+        error("Adjacency matrix can not have negative edge weights")
+    end
+    return Laplacians.lapWrapComponents((a; kw...) -> approxchol_lap1_gpu(a; kw..., kwargs...), a,
+                                        verbose = verbose, tol = tol, maxits = maxits,
+                                        maxtime = maxtime, pcgIts = pcgIts,
+                                        params = ApproxCholParams(:deg, params.stag_test, 2))
+end
+
 approxchol_sddm_gpu = Laplacians.sddmWrapLap(approxchol_lap_gpu)
+
+# harmonic_interp (src/harmonic.jl:16-33) with the SDDM solve on the GPU
+function harmonic_interp_gpu(adj_mat, S::Vector, vals::Vector; tol = 1e-6)
+    n = size(adj_mat, 1)
+    b = zeros(n)
+    b[S] = vals
+    inds = ones(Bool, n)
+    inds[S] .= false
+    la = lap(adj_mat)
+    f = approxchol_sddm_gpu(la[inds, inds]; tol = tol)
+    x = copy(b)
+    x[inds] = f((-la * b)[inds])
+    return x
+end
 
 end # module

@@ -29,11 +29,15 @@ __all__ = [
     "ApproxCholParams",
     "approxchol_lap_gpu",
     "approxchol_lap1_gpu",
+    "approxchol_lap2_gpu",
     "lap_wrap_components",
     "block_solver",
     "null_solver",
     "approxchol_sddm_gpu",
     "sddm_wrap_lap",
+    "adj_val_and_excess",
+    "extend_matrix",
+    "harmonic_interp",
 ]
 
 
@@ -87,26 +91,30 @@ class GpuLapSolver:
         return x
 
 
+_ORDER_NAMES = {"deg": "greedy degree", "wdeg": "wted degree", "random": "random"}
+
+
 def approxchol_lap1_gpu(a, tol=1e-6, maxits=1000, maxtime=math.inf, verbose=False, pcgIts=None,
                         params: ApproxCholParams | None = None, seed: int = 1, device: int = 0,
                         ldli: LDLinv | None = None):
-    """approxchol_lap1 -> approxchol_lapGreedy (src/approxChol.jl:1581-1626,
-    1791-1851) for a connected graph: la = lap(a), the LDLinv on the host,
+    """approxchol_lap1 (src/approxChol.jl:1791-1851) for a connected graph:
+    la = lap(a) and the LDLinv of params.order on the host (approxchol_lapGreedy
+    :deg with its split/merge forms, approxchol_lapWdeg, approxchol_lapRandom,
+    and approxchol_lapGiven for any other order, src/approxChol.jl:1581-1790),
     both uploaded once; returns the GPU solve closure."""
     params = params or ApproxCholParams()
-    if params.order != "deg" or params.split >= 1:
-        raise NotImplementedError("only ApproxCholParams(:deg) factors are built on this host; "
-                                  "pass ldli= to use a factor built elsewhere")
     t1 = time.time()
     a = sp.csc_matrix(a)
     la = lap(a)
+    order = params.order if params.order in ("deg", "wdeg", "random") else "given"
     if ldli is None:
-        ldli = build_ldli(a, seed=seed)
+        ldli = build_ldli(a, seed=seed, order=order, split=params.split if order == "deg" else 0,
+                          merge=params.merge if order == "deg" else 0)
     ctx = _native.Context(device)
     ctx.upload_lap(la)
     ctx.upload_ldli(ldli)
     if verbose:
-        print(f"Using greedy degree ordering. Factorization time: {time.time() - t1}")
+        print(f"Using {_ORDER_NAMES.get(order, 'given')} ordering. Factorization time: {time.time() - t1}")
         print(f"Factorization size: {2 * ldli.nnz}")
         print(f"Ratio of operator edges to original edges: {2 * ldli.nnz / a.nnz}")
         dg = la.diagonal()
@@ -191,20 +199,67 @@ def approxchol_lap_gpu(a, tol=1e-6, maxits=1000, maxtime=math.inf, verbose=False
                                verbose=verbose, pcgIts=pcgIts, params=params or ApproxCholParams(), **kw)
 
 
-def sddm_wrap_lap(lap_solver: Callable, sddm, tol=1e-6, maxits=1000, maxtime=math.inf, verbose=False,
-                  pcgIts=None, **params):
-    """sddmWrapLap (src/solverInterface.jl:324-352): solve an SDDM system by
-    adding a ground vertex and solving [b; -sum(b)] on the extended graph."""
-    M = sp.csc_matrix(sddm)
-    n = M.shape[0]
-    a = sp.csc_matrix(-(M - sp.diags(M.diagonal())))
+def approxchol_lap2_gpu(a, tol=1e-6, maxits=1000, maxtime=math.inf, verbose=False, pcgIts=None,
+                        params: ApproxCholParams | None = None, **kw):
+    """approxchol_lap2 (src/approxChol.jl:1559-1579): the split-edge
+    elimination under lapWrapComponents.  The reference passes
+    ApproxCholParams(:deg, params.stag_test, 2), which its three-argument
+    constructor (src/approxChol.jl:90) reads as stag_test 5, split =
+    params.stag_test (5 by default) and merge 2; this does the same."""
+    a = sp.csc_matrix(a)
+    if a.nnz and a.data.min() < 0:
+        raise ValueError("Adjacency matrix can not have negative edge weights")
+    params = params or ApproxCholParams()
+    p2 = ApproxCholParams("deg", 5, int(params.stag_test), 2)
+    return lap_wrap_components(approxchol_lap1_gpu, a, tol=tol, maxits=maxits, maxtime=maxtime,
+                               verbose=verbose, pcgIts=pcgIts, params=p2, **kw)
+
+
+def adj_val_and_excess(sddm):
+    """adjValAndExcess (src/graphOps.jl:43-48): a = Diagonal(diag(M)) - M,
+    dVal = diag(M), dExcess = M * ones."""
+    M = sp.csc_matrix(sddm, dtype=np.float64)
+    dval = M.diagonal()
+    a = sp.csc_matrix(sp.diags(dval) - M)
     a.eliminate_zeros()
-    excess = np.asarray(M.sum(axis=1)).ravel()
-    excess[np.abs(excess) < 1e-15 * np.abs(M.diagonal()).max()] = 0.0
-    if np.all(excess == 0):
+    dexcess = np.asarray(M @ np.ones(M.shape[0])).ravel()
+    return a, dval, dexcess
+
+
+def extend_matrix(a, dval, dexcess):
+    """extendMatrix (src/graphOps.jl:56-78): [a d; d' 0] with d the diagonal
+    excess of the vertices whose excess is above 100 eps times their
+    diagonal, or `a` itself when there is no excess at all."""
+    a = sp.csc_matrix(a)
+    n = a.shape[0]
+    assert n == len(dexcess)
+    if np.abs(dexcess).sum() == 0:
+        return a
+    pos = np.flatnonzero(dexcess > 100 * np.finfo(np.float64).eps * dval)
+    if len(pos) == 0:
+        return a
+    ai, aj = a.nonzero()
+    av = np.asarray(a[ai, aj]).ravel()
+    dpos = dexcess[pos]
+    i2 = np.concatenate([ai, pos, np.full(len(pos), n)])
+    j2 = np.concatenate([aj, np.full(len(pos), n), pos])
+    v2 = np.concatenate([av, dpos, dpos])
+    return sp.csc_matrix((v2, (i2, j2)), shape=(n + 1, n + 1))
+
+
+def sddm_wrap_lap(lap_solver: Callable, sddm, tol=1e-6, maxits=math.inf, maxtime=math.inf, verbose=False,
+                  pcgIts=None, **params):
+    """sddmWrapLap (src/solverInterface.jl:324-352): solve an SDDM system M x =
+    b with a Laplacian solver on the graph extended by one ground vertex
+    joined to every vertex with diagonal excess, on [b; -sum(b)], returning
+    (x_aug - x_aug[end])[:n].  Without excess, M is a Laplacian and its
+    adjacency is solved directly."""
+    M = sp.csc_matrix(sddm, dtype=np.float64)
+    n = M.shape[0]
+    a, dval, dexcess = adj_val_and_excess(M)
+    a1 = extend_matrix(a, dval, dexcess)
+    if a1.shape == M.shape:
         return lap_solver(a, tol=tol, maxits=maxits, maxtime=maxtime, verbose=verbose, pcgIts=pcgIts, **params)
-    ex = sp.csc_matrix(excess.reshape(-1, 1))
-    a1 = sp.bmat([[a, ex], [ex.T, None]], format="csc")
     F = lap_solver(a1, tol=tol, maxits=maxits, maxtime=maxtime, verbose=verbose, pcgIts=pcgIts, **params)
 
     def f(b, tol=tol, maxits=maxits, maxtime=maxtime, verbose=verbose, pcgIts=pcgIts):
@@ -220,3 +275,26 @@ def sddm_wrap_lap(lap_solver: Callable, sddm, tol=1e-6, maxits=1000, maxtime=mat
 def approxchol_sddm_gpu(sddm, **kw):
     """approxchol_sddm = sddmWrapLap(approxchol_lap) (src/approxChol.jl:1862)."""
     return sddm_wrap_lap(approxchol_lap_gpu, sddm, **kw)
+
+
+def harmonic_interp(adj_mat, S, vals, tol=1e-6):
+    """harmonic_interp (src/harmonic.jl:16-33): the x that minimises the
+    Laplacian quadratic form of `adj_mat` subject to x[S[i]] = vals[i]
+    (indices 0-based here), by an SDDM solve on the vertices outside S with
+    approxchol_sddm_gpu.  As in the reference, it may fail for disconnected
+    graphs."""
+    a = sp.csc_matrix(adj_mat, dtype=np.float64)
+    n = a.shape[0]
+    S = np.asarray(S, dtype=np.int64)
+    b = np.zeros(n)
+    b[S] = np.asarray(vals, dtype=np.float64)
+    inds = np.ones(n, dtype=bool)
+    inds[S] = False
+    la = lap(a)
+    la_sub = la[inds][:, inds]
+    b_sub = (-(la @ b))[inds]
+    f = approxchol_sddm_gpu(la_sub, tol=tol)
+    x_sub = f(b_sub)
+    x = b.copy()
+    x[inds] = x_sub
+    return x

@@ -160,8 +160,3 @@ def test_params_defaults_match_reference():
     p = lapgpu.ApproxCholParams()
     assert (p.order, p.stag_test, p.split, p.merge) == ("deg", 5, 0, 0)
 
-
-def test_unsupported_orders_raise():
-    a = graphs.grid2(4)
-    with pytest.raises(NotImplementedError):
-        lapgpu.approxchol_lap1_gpu(a, params=lapgpu.ApproxCholParams(order="wdeg"))
