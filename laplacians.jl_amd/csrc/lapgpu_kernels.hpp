@@ -214,6 +214,36 @@ __global__ void k_scalar_nb(const double* __restrict__ part, DevState* st) {
 template <int G>
 __host__ __device__ constexpr int heavy_limit() { return G >= 64 ? (1 << 30) : 8 * G; }
 
+// sum over k = k0 + lane, k0 + lane + G, ... < k1 of w[k] * x(idx[k]), added
+// in that order.  The index and weight loads of U consecutive terms are
+// issued together, then their U gathers, so a lane waits about two memory
+// latencies per U terms instead of two per term (the sums are the same as
+// the one-term-at-a-time loop, bit for bit).
+template <int G, int U, typename T, typename X>
+__device__ __forceinline__ T strided_dot(int k0, int k1, int lane, const int* __restrict__ idx,
+                                         const T* __restrict__ w, X x) {
+  T acc = 0;
+  for (int kb = k0 + lane; kb < k1; kb += U * G) {
+    int ii[U];
+    T ww[U], xv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = kb + u * G;
+      if (k < k1) {
+        ii[u] = idx[k];
+        ww[u] = w[k];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (kb + u * G < k1) xv[u] = x(ii[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (kb + u * G < k1) acc += ww[u] * xv[u];
+  }
+  return acc;
+}
+
 template <int G, typename T, typename TB>
 __global__ void __launch_bounds__(kBlock)
 k_group(int n, int lo, int hi, const int* __restrict__ heavy, int nh,
@@ -225,11 +255,8 @@ k_group(int n, int lo, int hi, const int* __restrict__ heavy, int nh,
     const int row = heavy[gid / kWave];
     const int lane = gid % kWave;
     const int k0 = ptr[row], k1 = ptr[row + 1];
-    T acc = 0;
-    for (int k = k0 + lane; k < k1; k += kWave) {
-      const int i = idx[k];
-      acc += w[k] * (i < n ? A[i] : (T)B[i - n]);
-    }
+    T acc = strided_dot<kWave, 8>(k0, k1, lane, idx, w,
+                                  [&](int i) { return i < n ? A[i] : (T)B[i - n]; });
     acc = group_sum<kWave>(acc);
     if (lane == 0 && !st->stopped) out[row] = acc;
     return;
@@ -240,11 +267,9 @@ k_group(int n, int lo, int hi, const int* __restrict__ heavy, int nh,
   if (row >= hi) return;
   const int k0 = ptr[row], k1 = ptr[row + 1];
   if (k1 - k0 > heavy_limit<G>()) return;
-  T acc = 0;
-  for (int k = k0 + lane; k < k1; k += G) {
-    const int i = idx[k];
-    acc += w[k] * (i < n ? A[i] : (T)B[i - n]);
-  }
+  // at most 8 terms per lane: one round of loads
+  T acc = strided_dot<G, 8>(k0, k1, lane, idx, w,
+                            [&](int i) { return i < n ? A[i] : (T)B[i - n]; });
   acc = group_sum<G>(acc);
   if (lane == 0 && !st->stopped) out[row] = acc;
 }
@@ -263,7 +288,7 @@ k_tail_ext(int T0, int NT, const int* __restrict__ Eptr,
   if (row >= NT) return;
   const int k0 = Eptr[row], k1 = Eptr[row + 1];
   T acc = 0;
-  for (int k = k0 + lane; k < k1; k += G) acc += Ew[k] * y[Eidx[k]];
+  acc = strided_dot<G, 8>(k0, k1, lane, Eidx, Ew, [&](int i) { return y[i]; });
   acc = group_sum<G>(acc);
   if (lane == 0 && !st->stopped) ext[row] = (T)r[T0 + row] + acc;
 }
