@@ -296,7 +296,7 @@ void launch_group(lapgpu_ctx* c, int lo, int hi, const int* heavy, int nh, const
                   const int* idx, const T* w, const T* A, const TB* B, T* out) {
   const int64_t threads = (int64_t)nh * kWave + (int64_t)(hi - lo) * G;
   hipLaunchKernelGGL((k_group<G, T, TB>), dim3(grid_for(threads, kBlock)), dim3(kBlock), 0,
-                     c->stream, (int)c->n, lo, hi, heavy, nh, ptr, idx, w, A, B, out, c->st);
+                     c->stream, (int)c->n, lo, hi, heavy, nh, ptr, idx, w, A, B, out);
 }
 
 template <typename T, typename S>
@@ -314,12 +314,10 @@ void launch_sweeps(lapgpu_ctx* c, const double* rin, T* y, T* z, const T* gFw, c
     const int G = c->G_ldl;
     if (G <= 8)
       hipLaunchKernelGGL((k_tail_ext<8, T>), dim3(grid_for(TP.NT * 8, kBlock)), dim3(kBlock), 0,
-                         c->stream, (int)TP.T0, (int)TP.NT, c->Eptr, c->Eidx, Ew, rin, y, ext,
-                         c->st);
+                         c->stream, (int)TP.T0, (int)TP.NT, c->Eptr, c->Eidx, Ew, rin, y, ext);
     else
       hipLaunchKernelGGL((k_tail_ext<32, T>), dim3(grid_for(TP.NT * 32, kBlock)), dim3(kBlock),
-                         0, c->stream, (int)TP.T0, (int)TP.NT, c->Eptr, c->Eidx, Ew, rin, y, ext,
-                         c->st);
+                         0, c->stream, (int)TP.T0, (int)TP.NT, c->Eptr, c->Eidx, Ew, rin, y, ext);
     if (c->prof_sweeps) (void)hipEventRecord(c->ev[6], c->stream);
     LAPGPU_DISPATCH_E(TP.elog, hipLaunchKernelGGL, (k_tail<T, S, E_>), dim3(1),
                       dim3(kTailThreads), 0, c->stream, (int)TP.T0, (int)TP.NT, ext, sv, z,

@@ -211,6 +211,9 @@ __global__ void k_scalar_nb(const double* __restrict__ part, DevState* st) {
 // Rows with more than kHeavy(G) entries are listed in `heavy` and handled
 // by a whole wave each (threads [0, 64 nh)); the other rows get G lanes each.
 // T is the sweep precision (double, or float in the FP32-inner mode).
+// The sweeps (k_group, k_tail_ext) do not read the stop flag: they write
+// only the sweep vectors, whose readers check it, and a flag read would
+// put one more memory latency on each launch's critical path.
 template <int G>
 __host__ __device__ constexpr int heavy_limit() { return G >= 64 ? (1 << 30) : 8 * G; }
 
@@ -248,8 +251,7 @@ template <int G, typename T, typename TB>
 __global__ void __launch_bounds__(kBlock)
 k_group(int n, int lo, int hi, const int* __restrict__ heavy, int nh,
         const int* __restrict__ ptr, const int* __restrict__ idx,
-        const T* __restrict__ w, const T* A, const TB* B, T* out,
-        const DevState* __restrict__ st) {
+        const T* __restrict__ w, const T* A, const TB* B, T* out) {
   const int gid = blockIdx.x * kBlock + threadIdx.x;
   if (gid < nh * kWave) {
     const int row = heavy[gid / kWave];
@@ -258,7 +260,7 @@ k_group(int n, int lo, int hi, const int* __restrict__ heavy, int nh,
     T acc = strided_dot<kWave, 8>(k0, k1, lane, idx, w,
                                   [&](int i) { return i < n ? A[i] : (T)B[i - n]; });
     acc = group_sum<kWave>(acc);
-    if (lane == 0 && !st->stopped) out[row] = acc;
+    if (lane == 0) out[row] = acc;
     return;
   }
   const int g2 = gid - nh * kWave;
@@ -271,7 +273,7 @@ k_group(int n, int lo, int hi, const int* __restrict__ heavy, int nh,
   T acc = strided_dot<G, 8>(k0, k1, lane, idx, w,
                             [&](int i) { return i < n ? A[i] : (T)B[i - n]; });
   acc = group_sum<G>(acc);
-  if (lane == 0 && !st->stopped) out[row] = acc;
+  if (lane == 0) out[row] = acc;
 }
 
 // ext[i] = r[T0+i] + sum of the forward in-edges of tail row i that come
@@ -281,16 +283,15 @@ __global__ void __launch_bounds__(kBlock)
 k_tail_ext(int T0, int NT, const int* __restrict__ Eptr,
            const int* __restrict__ Eidx, const T* __restrict__ Ew,
            const double* __restrict__ r, const T* __restrict__ y,
-           T* __restrict__ ext, const DevState* __restrict__ st) {
+           T* __restrict__ ext) {
   const int gid = blockIdx.x * kBlock + threadIdx.x;
   const int row = gid / G;
   const int lane = gid % G;
   if (row >= NT) return;
   const int k0 = Eptr[row], k1 = Eptr[row + 1];
-  T acc = 0;
-  acc = strided_dot<G, 8>(k0, k1, lane, Eidx, Ew, [&](int i) { return y[i]; });
+  T acc = strided_dot<G, 8>(k0, k1, lane, Eidx, Ew, [&](int i) { return y[i]; });
   acc = group_sum<G>(acc);
-  if (lane == 0 && !st->stopped) ext[row] = (T)r[T0 + row] + acc;
+  if (lane == 0) ext[row] = (T)r[T0 + row] + acc;
 }
 
 // forward + diagonal + backward on the tail; writes yF of the tail to y.
