@@ -386,7 +386,8 @@ def mode_multi(args) -> int:
         c1.close()
     plain_wall = k * t_big + len(smalls) * float(np.mean(t_small))
     plain_sps = (k + len(smalls)) / plain_wall
-    best = out_modes["fp32_inner"]
+    best_name = max(out_modes, key=lambda m: out_modes[m]["solves_per_s"])
+    best = out_modes[best_name]
     emit({
         "metric": "solves/s (64-RHS block + 256 small grids, relres<=1e-8) vs plain FP64 GPU PCG",
         "value": best["solves_per_s"],
@@ -398,12 +399,13 @@ def mode_multi(args) -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 outer / f32 inner",
+        "dtype": "f64" if best_name == "fp64" else "f64 outer / f32 inner",
         "data": "synthetic",
         "config": {
             "workload": "grid2 1024x1024 exp(N(0,1)) weights, 64 RHS N(0,1) sum-zero + 256 unit 2-D grids "
                         "(n log-uniform [1e3,1e5], aspect [1,8]), seed 4, tol 1e-8",
-            "fp32_inner": best,
+            "reported_mode": best_name,
+            "fp32_inner": out_modes["fp32_inner"],
             "fp64": out_modes["fp64"],
             "fp32_vs_fp64_max_rel_err": err32,
             "plain_one_at_a_time_solves_per_s": plain_sps,

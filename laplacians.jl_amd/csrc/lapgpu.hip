@@ -130,9 +130,14 @@ struct lapgpu_ctx {
   double *gFw = nullptr, *gBw = nullptr;
   float *gFw32 = nullptr, *gBw32 = nullptr, *z32 = nullptr;
   double* zF = nullptr;  // backward output yF (double path)
-  int block_depth = 16;
-  double block_growth = 4.0;
-  int64_t block_extra = 1 << 22;
+  // level-group limits: a level joins the group before it while the group
+  // spans at most block_depth levels, its composite rows stay within
+  // block_growth x its entries and add at most block_extra entries (each
+  // launch costs ~4 us, each entry ~8 ps on MI355X: narrow levels join,
+  // wide ones get launches of their own)
+  int block_depth = 32;
+  double block_growth = 16.0;
+  int64_t block_extra = 196608;
   // tail program
   int *Eptr = nullptr, *Eidx = nullptr;
   double *Ew = nullptr, *ext = nullptr;
@@ -1210,6 +1215,7 @@ int lapgpu_create(lapgpu_ctx** out, int device) {
   if (const char* sw = getenv("LAPGPU_SWITCH_TOL")) c->switch_tol = atof(sw);
   if (const char* bd = getenv("LAPGPU_BLOCK_DEPTH")) c->block_depth = atoi(bd);
   if (const char* bg = getenv("LAPGPU_BLOCK_GROWTH")) c->block_growth = atof(bg);
+  if (const char* be = getenv("LAPGPU_BLOCK_EXTRA")) c->block_extra = atoll(be);
   *out = c;
   return 0;
 }
