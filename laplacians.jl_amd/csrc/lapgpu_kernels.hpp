@@ -247,6 +247,53 @@ __device__ __forceinline__ T strided_dot(int k0, int k1, int lane, const int* __
   return acc;
 }
 
+// A grid-stride loop over rows (this lane group's rows first_row,
+// first_row + stride, ...), G lanes per row, with RU rows in flight: the
+// row pointers of RU rows are loaded together, then their first terms
+// (entry k0 + lane), then those terms' gathers, so a row costs about three
+// memory latencies per RU rows instead of three per row.  Each row's sum is
+// the same as the one-row-at-a-time loop's (terms k0 + lane, k0 + lane + G,
+// ... added in that order, then group_sum); emit(row, sum) is called for
+// the rows in order.
+template <int G, int RU, typename T, typename X, typename Emit>
+__device__ __forceinline__ void rows_in_flight(int first_row, int stride, int n,
+                                               const int* __restrict__ ptr,
+                                               const int* __restrict__ idx,
+                                               const T* __restrict__ w, X x, Emit emit) {
+  const int lane = threadIdx.x % G;
+  for (int r0 = first_row; r0 < n; r0 += RU * stride) {
+    int k0[RU], k1[RU], ii[RU];
+    T ww[RU], xv[RU];
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+      const int row = r0 + u * stride;
+      k0[u] = row < n ? ptr[row] : 0;
+      k1[u] = row < n ? ptr[row + 1] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+      const int k = k0[u] + lane;
+      if (k < k1[u]) {
+        ii[u] = idx[k];
+        ww[u] = w[k];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < RU; ++u)
+      if (k0[u] + lane < k1[u]) xv[u] = x(ii[u]);
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+      const int row = r0 + u * stride;
+      if (row >= n) break;  // uniform over the lane group
+      T acc = 0;
+      if (k0[u] + lane < k1[u]) acc += ww[u] * xv[u];
+      for (int k = k0[u] + lane + G; k < k1[u]; k += G) acc += w[k] * x(idx[k]);
+      acc = group_sum<G>(acc);
+      emit(row, acc);
+    }
+  }
+}
+
 template <int G, typename T, typename TB>
 __global__ void __launch_bounds__(kBlock)
 k_group(int n, int lo, int hi, const int* __restrict__ heavy, int nh,
@@ -376,25 +423,21 @@ k_spmv_dot_mixed(int n, const int* __restrict__ Lptr, const int* __restrict__ Li
   const bool f64 = st->spmv64 != 0;
   const int lane = threadIdx.x % G;
   const int rows_per_grid = gridDim.x * (kBlock / G);
+  const int first = blockIdx.x * (kBlock / G) + threadIdx.x / G;
   double v[1] = {0.0};
-  for (int row = blockIdx.x * (kBlock / G) + threadIdx.x / G; row < n;
-       row += rows_per_grid) {
-    const int k0 = Lptr[row], k1 = Lptr[row + 1];
-    double acc;
-    if (f64) {
-      double a = 0.0;
-      for (int k = k0 + lane; k < k1; k += G) a += Lval[k] * p[Lidx[k]];
-      acc = group_sum<G>(a);
-    } else {
-      float a = 0.0f;
-      for (int k = k0 + lane; k < k1; k += G) a += Lval32[k] * p32[Lidx[k]];
-      acc = (double)group_sum<G>(a);
-    }
+  auto emit = [&](int row, double acc) {
     if (lane == 0) {
       q[row] = acc;
       v[0] += p[row] * acc;
     }
-  }
+  };
+  if (f64)
+    rows_in_flight<G, 4>(first, rows_per_grid, n, Lptr, Lidx, Lval,
+                         [&](int i) { return p[i]; }, emit);
+  else
+    rows_in_flight<G, 4>(first, rows_per_grid, n, Lptr, Lidx, Lval32,
+                         [&](int i) { return p32[i]; },
+                         [&](int row, float acc) { emit(row, (double)acc); });
   block_sum<1>(v);
   if (threadIdx.x == 0) part[S_PQ * kRedBlocks + blockIdx.x] = v[0];
 }
@@ -430,17 +473,13 @@ k_spmv_dot(int n, const int* __restrict__ Lptr, const int* __restrict__ Lidx,
   const int lane = threadIdx.x % G;
   const int rows_per_grid = gridDim.x * (kBlock / G);
   double v[1] = {0.0};
-  for (int row = blockIdx.x * (kBlock / G) + threadIdx.x / G; row < n;
-       row += rows_per_grid) {
-    const int k0 = Lptr[row], k1 = Lptr[row + 1];
-    T acc = 0;
-    for (int k = k0 + lane; k < k1; k += G) acc += Lval[k] * p[Lidx[k]];
-    acc = group_sum<G>(acc);
-    if (lane == 0) {
-      q[row] = (double)acc;
-      v[0] += pd[row] * (double)acc;
-    }
-  }
+  rows_in_flight<G, 4>(blockIdx.x * (kBlock / G) + threadIdx.x / G, rows_per_grid, n, Lptr, Lidx,
+                       Lval, [&](int i) { return p[i]; }, [&](int row, T acc) {
+                         if (lane == 0) {
+                           q[row] = (double)acc;
+                           v[0] += pd[row] * (double)acc;
+                         }
+                       });
   block_sum<1>(v);
   if (threadIdx.x == 0) part[S_PQ * kRedBlocks + blockIdx.x] = v[0];
 }

@@ -96,7 +96,7 @@ __device__ __forceinline__ void mbcast(int myi, T myw, int src, int& i, T& w) {
   }
 }
 
-// sum over the entries [k0, k1) of a row of w[e] * src(idx[e]) for the K
+// acc + the sum over the entries [k0, k1) of a row of w[e] * src(idx[e]) for the K
 // lanes (one per rhs) of row group `sub` of a wave, in entry order, in T.
 // Indices and weights are loaded K at a time, one per lane, and broadcast
 // within the group, so the gathers of one row are independent loads rather
@@ -105,9 +105,8 @@ __device__ __forceinline__ void mbcast(int myi, T myw, int src, int& i, T& w) {
 // registers.
 template <int K, typename T, typename F>
 __device__ __forceinline__ T mrow_dot(int k0, int k1, int j, int sub, const int* __restrict__ idx,
-                                      const T* __restrict__ w, F&& src) {
+                                      const T* __restrict__ w, F&& src, T acc = 0) {
   constexpr int U = K == kWave ? 8 : 4;  // gathers in flight per lane
-  T acc = 0;
   for (int base = k0; base < k1; base += K) {
     const int kk = base + j;
     int myi = 0;
@@ -143,6 +142,70 @@ template <int K>
 __device__ __forceinline__ int wave_uniform(int v) {
   if constexpr (K == kWave) return __builtin_amdgcn_readfirstlane(v);
   return v;
+}
+
+// A grid-stride loop over rows for the K lanes of row group `sub` (rows
+// first_row, first_row + stride, ...) with RU rows in flight: the row
+// pointers of RU rows are loaded together, then each row's first K
+// indices/weights, then the gathers of up to 8 entries of every row, before
+// any of them is used.  Each row's sum is mrow_dot's (entries in order);
+// emit(row, sum) is called for the rows in order.
+template <int K, int RU, typename T, typename F, typename Emit>
+__device__ __forceinline__ void mrows_in_flight(int first_row, int stride, int n, int j, int sub,
+                                                const int* __restrict__ ptr,
+                                                const int* __restrict__ idx,
+                                                const T* __restrict__ w, F src, Emit emit) {
+  constexpr int U = 8;
+  const int g0 = sub * K;
+  for (int r0 = first_row; r0 < n; r0 += RU * stride) {
+    int k0[RU], k1[RU], myi[RU];
+    T myw[RU];
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+      const int row = r0 + u * stride;
+      k0[u] = row < n ? wave_uniform<K>(ptr[row]) : 0;
+      k1[u] = row < n ? wave_uniform<K>(ptr[row + 1]) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+      const int kk = k0[u] + j;
+      myi[u] = 0;
+      myw[u] = 0;
+      if (kk < k1[u]) {
+        myi[u] = idx[kk];
+        myw[u] = w[kk];
+      }
+    }
+    int ii[RU][U];
+    T we[RU][U], v[RU][U];
+#pragma unroll
+    for (int u = 0; u < RU; ++u)
+#pragma unroll
+      for (int e = 0; e < U; ++e) mbcast<K>(myi[u], myw[u], g0 + e, ii[u][e], we[u][e]);
+#pragma unroll
+    for (int u = 0; u < RU; ++u)
+#pragma unroll
+      for (int e = 0; e < U; ++e)
+        if (e < k1[u] - k0[u]) v[u][e] = src(ii[u][e]);
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+      const int row = r0 + u * stride;
+      const int cnt = k1[u] - k0[u];
+      T acc = 0;
+#pragma unroll
+      for (int e = 0; e < U; ++e)
+        if (e < cnt) acc += we[u][e] * v[u][e];
+      // entries U .. K-1 of the first chunk, then whole chunks (rare)
+      for (int e = U; e < min(K, cnt); ++e) {
+        int i;
+        T wv;
+        mbcast<K>(myi[u], myw[u], g0 + e, i, wv);
+        acc += wv * src(i);
+      }
+      if (cnt > K) acc = mrow_dot<K, T>(k0[u] + K, k1[u], j, sub, idx, w, src, acc);
+      if (row < n) emit(row, acc);
+    }
+  }
 }
 
 // after the per-rhs steps: flags[0] = every rhs stopped
@@ -446,13 +509,11 @@ k_mspmv(int n, const int* __restrict__ Lptr, const int* __restrict__ Lidx,
   const int wave = (blockIdx.x * kBlock + threadIdx.x) / kWave;
   const int nwaves = gridDim.x * (kBlock / kWave);
   double a[1] = {0.0};
-  for (int row = wave * R + sub; row < n; row += nwaves * R) {
-    const int k0 = wave_uniform<K>(Lptr[row]), k1 = wave_uniform<K>(Lptr[row + 1]);
-    const double acc =
-        mrow_dot<K>(k0, k1, j, sub, Lidx, Lval, [&](int i) { return p[(int64_t)i * K + j]; });
-    q[(int64_t)row * K + j] = acc;
-    a[0] += p[(int64_t)row * K + j] * acc;
-  }
+  mrows_in_flight<K, 2>(wave * R + sub, nwaves * R, n, j, sub, Lptr, Lidx, Lval,
+                        [&](int i) { return p[(int64_t)i * K + j]; }, [&](int row, double acc) {
+                          q[(int64_t)row * K + j] = acc;
+                          a[0] += p[(int64_t)row * K + j] * acc;
+                        });
   const int slots[1] = {S_PQ};
   mblock_sum<K, 1>(a, part, slots);
 }
