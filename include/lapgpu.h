@@ -155,10 +155,10 @@ int lapgpu_group_info(lapgpu_ctx* ctx, int64_t* ngroups, int64_t* rows, int64_t*
 int lapgpu_tail_info(lapgpu_ctx* ctx, int64_t* nt, int64_t* nsteps, int64_t* nslots,
                      int64_t* nedges_in);
 
-/* The LDLinv apply of the last LAPGPU_PROFILE single-RHS solve split by
- * kernel (ms per iteration): forward level groups, the tail's inputs
- * (k_tail_ext), the tail (k_tail) and backward level groups.  Measurement
- * only; no reference counterpart. */
+/* The LDLinv apply of the last LAPGPU_PROFILE solve (single- or multi-RHS)
+ * split by kernel (ms per iteration): forward level groups, the tail's
+ * inputs (k_tail_ext / k_mtail_ext), the tail (k_tail / k_mtail) and
+ * backward level groups.  Measurement only; no reference counterpart. */
 int lapgpu_last_sweep_timings(lapgpu_ctx* ctx, double* fgroups_ms, double* tail_ext_ms,
                               double* tail_ms, double* bgroups_ms);
 

@@ -919,15 +919,20 @@ void msweeps_t(lapgpu_ctx* c, const double* rin, T* y, T* z, T* ext, const T* gF
     launch_mgroup<K, T, double>(c, g.lo, g.hi, g.Sf, c->mheavy + g.mhoff_f, g.mnhf, c->gFptr,
                                 c->gFidx, gFw, y, rin, y);
   const TailProgram& TP = P.tail;
+  if (c->prof_sweeps) (void)hipEventRecord(c->ev[5], c->stream);
   if (TP.NT > 0) {
     hipLaunchKernelGGL((k_mtail_ext<K, T>), dim3(grid_for(TP.NT * K, kBlock)), dim3(kBlock), 0,
                        c->stream, (int)TP.T0, (int)TP.NT, c->Eptr, c->Eidx, Ew, rin, y, ext,
                        m.flags);
+    if (c->prof_sweeps) (void)hipEventRecord(c->ev[6], c->stream);
     LAPGPU_DISPATCH_E(TP.elog, hipLaunchKernelGGL, (k_mtail<K, T, S, E_>), dim3(K),
                       dim3(kTailThreads), 0, c->stream, (int)TP.T0, (int)TP.NT, ext, sv, z,
                       c->fsteps, (int)TP.fsteps.size(), fslots, c->bsteps,
                       (int)TP.bsteps.size(), bslots, m.st, m.flags);
+  } else if (c->prof_sweeps) {
+    (void)hipEventRecord(c->ev[6], c->stream);
   }
+  if (c->prof_sweeps) (void)hipEventRecord(c->ev[7], c->stream);
   for (auto it = c->groups.rbegin(); it != c->groups.rend(); ++it)
     launch_mgroup<K, T, T>(c, it->lo, it->hi, it->Sb, c->mheavy + it->mhoff_b, it->mnhb,
                            c->gBptr, c->gBidx, gBw, z, y, z);
@@ -1018,7 +1023,9 @@ void enqueue_miter(lapgpu_ctx* c, bool prof, bool replace) {
                        replace ? 1 : 0, m.part, m.st, m.flags);
   }
   if (prof) (void)hipEventRecord(c->ev[2], c->stream);
+  c->prof_sweeps = prof;
   msweeps<K>(c, m.r);
+  c->prof_sweeps = false;
   if (prof) (void)hipEventRecord(c->ev[3], c->stream);
   mfinish_precond<K>(c, m.r);
   hipLaunchKernelGGL(k_mscalar_rho<K>, dim3(1), dim3(kScalarThreads), 0, c->stream, m.part,
@@ -1108,6 +1115,15 @@ int solve_multi_chunk(lapgpu_ctx* c, const double* d_B, double* d_X, int k, doub
       acc[1] += b2;
       acc[2] += d + e;
       acc[3] += 1;
+      float f1 = 0, f2 = 0, f3 = 0, f4 = 0;
+      (void)hipEventElapsedTime(&f1, c->ev[2], c->ev[5]);
+      (void)hipEventElapsedTime(&f2, c->ev[5], c->ev[6]);
+      (void)hipEventElapsedTime(&f3, c->ev[6], c->ev[7]);
+      (void)hipEventElapsedTime(&f4, c->ev[7], c->ev[3]);
+      acc[4] += f1;
+      acc[5] += f2;
+      acc[6] += f3;
+      acc[7] += f4;
     }
     while (!done && polled < enq - 1) {
       const int ps = (int)(polled % lapgpu_ctx::kRing);
@@ -1141,7 +1157,8 @@ int solve_multi_device(lapgpu_ctx* c, const double* d_B, double* d_X, int64_t k,
   if (!c->has_F) return set_err(c, -2, "solve_multi: upload the Laplacian and LDLinv first");
   const auto t_start = std::chrono::steady_clock::now();
   const int64_t n = c->n;
-  double acc[4] = {0, 0, 0, 0};
+  // spmv, ldl, vec, iterations; forward groups, tail inputs, tail, backward groups
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int64_t maxit = 0;
   for (int64_t c0 = 0; c0 < k; c0 += kMultiMax) {
     const int kc = (int)std::min<int64_t>(kMultiMax, k - c0);
@@ -1159,6 +1176,10 @@ int solve_multi_device(lapgpu_ctx* c, const double* d_B, double* d_X, int64_t k,
     c->t_spmv = acc[0] / acc[3];
     c->t_ldl = acc[1] / acc[3];
     c->t_vec = acc[2] / acc[3];
+    c->t_fgroups = acc[4] / acc[3];
+    c->t_tail_ext = acc[5] / acc[3];
+    c->t_tail = acc[6] / acc[3];
+    c->t_bgroups = acc[7] / acc[3];
   }
   c->t_total =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count() * 1e3;
