@@ -54,6 +54,84 @@ __device__ __forceinline__ V group_sum(V v) {
   return v;
 }
 
+// sum over k = k0 + lane, k0 + lane + G, ... < k1 of w[k] * x(idx[k]), added
+// in that order.  The index and weight loads of U consecutive terms are
+// issued together, then their U gathers, so a lane waits about two memory
+// latencies per U terms instead of two per term (the sums are the same as
+// the one-term-at-a-time loop, bit for bit).
+template <int G, int U, typename T, typename X>
+__device__ __forceinline__ T strided_dot(int k0, int k1, int lane, const int* __restrict__ idx,
+                                         const T* __restrict__ w, X x) {
+  T acc = 0;
+  for (int kb = k0 + lane; kb < k1; kb += U * G) {
+    int ii[U];
+    T ww[U], xv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = kb + u * G;
+      if (k < k1) {
+        ii[u] = idx[k];
+        ww[u] = w[k];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (kb + u * G < k1) xv[u] = x(ii[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (kb + u * G < k1) acc += ww[u] * xv[u];
+  }
+  return acc;
+}
+
+// A grid-stride loop over rows (this lane group's rows first_row,
+// first_row + stride, ...), G lanes per row, with RU rows in flight: the
+// row pointers of RU rows are loaded together, then their first terms
+// (entry k0 + lane), then those terms' gathers, so a row costs about three
+// memory latencies per RU rows instead of three per row.  Each row's sum is
+// the same as the one-row-at-a-time loop's (terms k0 + lane, k0 + lane + G,
+// ... added in that order, then group_sum); emit(row, sum) is called for
+// the rows in order.
+template <int G, int RU, typename T, typename X, typename Emit>
+__device__ __forceinline__ void rows_in_flight(int first_row, int stride, int n,
+                                               const int* __restrict__ ptr,
+                                               const int* __restrict__ idx,
+                                               const T* __restrict__ w, X x, Emit emit) {
+  const int lane = threadIdx.x % G;
+  for (int r0 = first_row; r0 < n; r0 += RU * stride) {
+    int k0[RU], k1[RU], ii[RU];
+    T ww[RU], xv[RU];
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+      const int row = r0 + u * stride;
+      k0[u] = row < n ? ptr[row] : 0;
+      k1[u] = row < n ? ptr[row + 1] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+      const int k = k0[u] + lane;
+      if (k < k1[u]) {
+        ii[u] = idx[k];
+        ww[u] = w[k];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < RU; ++u)
+      if (k0[u] + lane < k1[u]) xv[u] = x(ii[u]);
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+      const int row = r0 + u * stride;
+      if (row >= n) break;  // uniform over the lane group
+      T acc = 0;
+      if (k0[u] + lane < k1[u]) acc += ww[u] * xv[u];
+      for (int k = k0[u] + lane + G; k < k1[u]; k += G) acc += w[k] * x(idx[k]);
+      acc = group_sum<G>(acc);
+      emit(row, acc);
+    }
+  }
+}
+
+
 // Sum NV values over the block; result valid in thread 0.
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV]) {

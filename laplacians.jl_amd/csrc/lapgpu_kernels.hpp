@@ -217,83 +217,6 @@ __global__ void k_scalar_nb(const double* __restrict__ part, DevState* st) {
 template <int G>
 __host__ __device__ constexpr int heavy_limit() { return G >= 64 ? (1 << 30) : 8 * G; }
 
-// sum over k = k0 + lane, k0 + lane + G, ... < k1 of w[k] * x(idx[k]), added
-// in that order.  The index and weight loads of U consecutive terms are
-// issued together, then their U gathers, so a lane waits about two memory
-// latencies per U terms instead of two per term (the sums are the same as
-// the one-term-at-a-time loop, bit for bit).
-template <int G, int U, typename T, typename X>
-__device__ __forceinline__ T strided_dot(int k0, int k1, int lane, const int* __restrict__ idx,
-                                         const T* __restrict__ w, X x) {
-  T acc = 0;
-  for (int kb = k0 + lane; kb < k1; kb += U * G) {
-    int ii[U];
-    T ww[U], xv[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int k = kb + u * G;
-      if (k < k1) {
-        ii[u] = idx[k];
-        ww[u] = w[k];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      if (kb + u * G < k1) xv[u] = x(ii[u]);
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      if (kb + u * G < k1) acc += ww[u] * xv[u];
-  }
-  return acc;
-}
-
-// A grid-stride loop over rows (this lane group's rows first_row,
-// first_row + stride, ...), G lanes per row, with RU rows in flight: the
-// row pointers of RU rows are loaded together, then their first terms
-// (entry k0 + lane), then those terms' gathers, so a row costs about three
-// memory latencies per RU rows instead of three per row.  Each row's sum is
-// the same as the one-row-at-a-time loop's (terms k0 + lane, k0 + lane + G,
-// ... added in that order, then group_sum); emit(row, sum) is called for
-// the rows in order.
-template <int G, int RU, typename T, typename X, typename Emit>
-__device__ __forceinline__ void rows_in_flight(int first_row, int stride, int n,
-                                               const int* __restrict__ ptr,
-                                               const int* __restrict__ idx,
-                                               const T* __restrict__ w, X x, Emit emit) {
-  const int lane = threadIdx.x % G;
-  for (int r0 = first_row; r0 < n; r0 += RU * stride) {
-    int k0[RU], k1[RU], ii[RU];
-    T ww[RU], xv[RU];
-#pragma unroll
-    for (int u = 0; u < RU; ++u) {
-      const int row = r0 + u * stride;
-      k0[u] = row < n ? ptr[row] : 0;
-      k1[u] = row < n ? ptr[row + 1] : 0;
-    }
-#pragma unroll
-    for (int u = 0; u < RU; ++u) {
-      const int k = k0[u] + lane;
-      if (k < k1[u]) {
-        ii[u] = idx[k];
-        ww[u] = w[k];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < RU; ++u)
-      if (k0[u] + lane < k1[u]) xv[u] = x(ii[u]);
-#pragma unroll
-    for (int u = 0; u < RU; ++u) {
-      const int row = r0 + u * stride;
-      if (row >= n) break;  // uniform over the lane group
-      T acc = 0;
-      if (k0[u] + lane < k1[u]) acc += ww[u] * xv[u];
-      for (int k = k0[u] + lane + G; k < k1[u]; k += G) acc += w[k] * x(idx[k]);
-      acc = group_sum<G>(acc);
-      emit(row, acc);
-    }
-  }
-}
-
 template <int G, typename T, typename TB>
 __global__ void __launch_bounds__(kBlock)
 k_group(int n, int lo, int hi, const int* __restrict__ heavy, int nh,
@@ -555,17 +478,14 @@ k_resid(int n, int always, const int* __restrict__ Lptr,
   const int lane = threadIdx.x % G;
   const int rows_per_grid = gridDim.x * (kBlock / G);
   double v[1] = {0.0};
-  for (int row = blockIdx.x * (kBlock / G) + threadIdx.x / G; row < n;
-       row += rows_per_grid) {
-    double acc = 0.0;
-    for (int k = Lptr[row] + lane; k < Lptr[row + 1]; k += G) acc += Lval[k] * x[Lidx[k]];
-    acc = group_sum<G>(acc);
-    if (lane == 0) {
-      const double t = r0[row] - acc;
-      r[row] = t;
-      v[0] += t * t;
-    }
-  }
+  rows_in_flight<G, 4>(blockIdx.x * (kBlock / G) + threadIdx.x / G, rows_per_grid, n, Lptr, Lidx,
+                       Lval, [&](int i) { return x[i]; }, [&](int row, double acc) {
+                         if (lane == 0) {
+                           const double t = r0[row] - acc;
+                           r[row] = t;
+                           v[0] += t * t;
+                         }
+                       });
   block_sum<1>(v);
   if (threadIdx.x == 0) part[S_RR * kRedBlocks + blockIdx.x] = v[0];
 }

@@ -661,16 +661,14 @@ k_mresid(int n, int always, const int* __restrict__ Lptr, const int* __restrict_
   const int wave = (blockIdx.x * kBlock + threadIdx.x) / kWave;
   const int nwaves = gridDim.x * (kBlock / kWave);
   double a[1] = {0.0};
-  for (int row = wave * R + sub; row < n; row += nwaves * R) {
-    const int k0 = wave_uniform<K>(Lptr[row]), k1 = wave_uniform<K>(Lptr[row + 1]);
-    const double acc = mrow_dot<K, double>(k0, k1, j, sub, Lidx, Lval,
-                                           [&](int i) { return x[(int64_t)i * K + j]; });
-    if (run) {
-      const double t = r0[(int64_t)row * K + j] - acc;
-      r[(int64_t)row * K + j] = t;
-      a[0] += t * t;
-    }
-  }
+  mrows_in_flight<K, 2>(wave * R + sub, nwaves * R, n, j, sub, Lptr, Lidx, Lval,
+                        [&](int i) { return x[(int64_t)i * K + j]; }, [&](int row, double acc) {
+                          if (run) {
+                            const double t = r0[(int64_t)row * K + j] - acc;
+                            r[(int64_t)row * K + j] = t;
+                            a[0] += t * t;
+                          }
+                        });
   const int slots[1] = {S_RR};
   mblock_sum<K, 1>(a, part, slots);
 }

@@ -151,16 +151,13 @@ __device__ __forceinline__ void wg_pcg(const WgGraphD& G, WgReqD& R, Y* y, doubl
       double pq[1] = {0.0};
       {
         const int grp = t / kWgG, lane = t % kWgG;
-        for (int row = grp; row < n; row += kWgThreads / kWgG) {
-          double acc = 0.0;
-          for (int k = G.Lptr[row] + lane; k < G.Lptr[row + 1]; k += kWgG)
-            acc += G.Lval[k] * p[G.Lidx[k]];
-          acc = group_sum<kWgG>(acc);
-          if (lane == 0) {
-            q[row] = acc;
-            pq[0] += p[row] * acc;
-          }
-        }
+        rows_in_flight<kWgG, 2>(grp, kWgThreads / kWgG, n, G.Lptr, G.Lidx, G.Lval,
+                                [&](int i) { return p[i]; }, [&](int row, double acc) {
+                                  if (lane == 0) {
+                                    q[row] = acc;
+                                    pq[0] += p[row] * acc;
+                                  }
+                                });
       }
       wg_sum<1>(pq, red);
       const double al = rho / pq[0];
