@@ -124,6 +124,12 @@ def mode_plain(args) -> int:
     ctx = _native.Context(0)
     ctx.upload_lap(la)
     ctx.upload_ldli(f)
+    tune = None
+    if args.autotune:
+        t0 = time.time()
+        tune = ctx.autotune(k=0, groups=True)
+        tune["seconds"] = time.time() - t0
+        log(f"autotune: {tune}")
     info = ctx.factor_info()
     ti = ctx.tail_info()
     gi = ctx.group_info()
@@ -204,6 +210,7 @@ def mode_plain(args) -> int:
         "data": "synthetic",
         "config": {
             "workload": f"{args.workload}: exp(U(-2,2)) weights seed 2, 1 RHS N(0,1) sum-zero, tol {args.tol}",
+            "autotune": tune,
             "n": n,
             "nnz_L": int(la.nnz),
             "nnzF": int(f.nnz),
@@ -297,6 +304,10 @@ def mode_multi(args) -> int:
     ctx = _native.Context(0)
     ctx.upload_lap(la)
     ctx.upload_ldli(f)
+    tune = None
+    if not args.no_autotune:
+        tune = ctx.autotune(k=k, groups=True)
+        log(f"autotune (k={k}): {tune}")
     batch = _native.Batch(0)
     gids = [batch.add_graph(lapgpu.lap(g), ff) for g, ff in zip(smalls, sf)]
     dB = DeviceArray.from_host(B)
@@ -405,6 +416,7 @@ def mode_multi(args) -> int:
             "workload": "grid2 1024x1024 exp(N(0,1)) weights, 64 RHS N(0,1) sum-zero + 256 unit 2-D grids "
                         "(n log-uniform [1e3,1e5], aspect [1,8]), seed 4, tol 1e-8",
             "reported_mode": best_name,
+            "autotune": tune,
             "fp32_inner": out_modes["fp32_inner"],
             "fp64": out_modes["fp64"],
             "fp32_vs_fp64_max_rel_err": err32,
@@ -666,6 +678,10 @@ def main() -> int:
     ap.add_argument("--batch-window", type=float, default=2.0, help="serving: wait for more requests, x last solve time")
     ap.add_argument("--max-wait", type=float, default=3.0, help="serving: longest wait for more requests (s)")
     ap.add_argument("--hw-queues", type=int, default=16, help="serving: HIP hardware queues (0: HIP's default)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="plain: tune the schedule on the graph first (lapgpu_autotune)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="multi: skip the schedule autotuning (done by default)")
     ap.add_argument("--calib", action="store_true",
                     help="plain mode: end with a device-to-device copy of 512 MiB (PMC calibration)")
     args = ap.parse_args()

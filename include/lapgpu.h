@@ -155,6 +155,23 @@ int lapgpu_group_info(lapgpu_ctx* ctx, int64_t* ngroups, int64_t* rows, int64_t*
 int lapgpu_tail_info(lapgpu_ctx* ctx, int64_t* nt, int64_t* nsteps, int64_t* nslots,
                      int64_t* nedges_in);
 
+/* Autotuner (no reference counterpart; the reference has no device
+ * schedule).  Times, on this graph, every level group's kernel with each
+ * candidate lanes-per-row G in {4, 8, 16, 32} (rows longer than 8 G entries
+ * get a wave each) and, when k > 1, each candidate slices-per-row S of the
+ * k-column (rounded up to 8..64) multi-RHS kernels, and keeps the fastest
+ * per group and sweep direction; with LAPGPU_TUNE_GROUPS it first tries
+ * three level-group budgets (a regrouping each, seconds on large graphs)
+ * and keeps the one with the fastest LDLinv apply.  Later solves use the
+ * tuned schedule.  The lane split changes the order in which a row's terms
+ * are summed, so tuned results may differ from untuned ones in the last
+ * bits (and, through timing noise, between tuning runs).  times_ms (4
+ * doubles, may be NULL): the device time of one single-RHS LDLinv apply
+ * before and after, and of one k-column apply before and after (0 when
+ * k <= 1). */
+#define LAPGPU_TUNE_GROUPS 1
+int lapgpu_autotune(lapgpu_ctx* ctx, int64_t k, int flags, double* times_ms);
+
 /* The LDLinv apply of the last LAPGPU_PROFILE solve (single- or multi-RHS)
  * split by kernel (ms per iteration): forward level groups, the tail's
  * inputs (k_tail_ext / k_mtail_ext), the tail (k_tail / k_mtail) and

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -1188,6 +1189,180 @@ int solve_multi_device(lapgpu_ctx* c, const double* d_B, double* d_X, int64_t k,
   return 0;
 }
 
+// ---- autotuner: lanes per row (single-RHS) and slices per row (multi-RHS)
+// of every level group and sweep direction, and optionally the level-group
+// budget, chosen by timing the candidates on this graph.  The sweep inputs
+// are whatever the workspace holds (only timings are used).
+
+constexpr int kTuneReps = 4;
+
+// mean device time (ms) of one call of `launch` over kTuneReps calls, after
+// one untimed call
+template <typename F>
+float tune_time(lapgpu_ctx* c, F&& launch) {
+  launch();
+  (void)hipEventRecord(c->ev[0], c->stream);
+  for (int r = 0; r < kTuneReps; ++r) launch();
+  (void)hipEventRecord(c->ev[1], c->stream);
+  (void)hipEventSynchronize(c->ev[1]);
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+  return ms / (float)kTuneReps;
+}
+
+// rows of [lo, hi) longer than 8 G entries (the wave-per-row rows at G)
+void heavy_rows(const std::vector<int32_t>& ptr, int lo, int hi, int G, std::vector<int32_t>& out) {
+  for (int v = lo; v < hi; ++v)
+    if (ptr[v + 1] - ptr[v] > 8 * G) out.push_back(v);
+}
+
+template <typename T>
+int tune_lanes_t(lapgpu_ctx* c, const double* rin, T* y, T* z, const T* gFw, const T* gBw,
+                 double* before_ms, double* after_ms) {
+  const LevelBlocks& B = c->plan.blocks;
+  constexpr int kCand[4] = {4, 8, 16, 32};
+  // the heavy rows of every candidate, per group and direction
+  std::vector<int32_t> hv[4];
+  std::vector<std::array<int, 4>> off(c->groups.size());  // f, nf, b, nb
+  std::vector<std::array<int, 4>> offc[4];
+  for (int ci = 0; ci < 4; ++ci) {
+    offc[ci].resize(c->groups.size());
+    for (size_t g = 0; g < c->groups.size(); ++g) {
+      const auto& gl = c->groups[g];
+      const int f0 = (int)hv[ci].size();
+      heavy_rows(B.fptr, gl.lo, gl.hi, kCand[ci], hv[ci]);
+      const int b0 = (int)hv[ci].size();
+      heavy_rows(B.bptr, gl.lo, gl.hi, kCand[ci], hv[ci]);
+      offc[ci][g] = {f0, b0 - f0, b0, (int)hv[ci].size() - b0};
+    }
+  }
+  int* dh[4] = {nullptr, nullptr, nullptr, nullptr};
+  int rc = 0;
+  for (int ci = 0; ci < 4 && !rc; ++ci) rc = dev_upload(c, &dh[ci], hv[ci]);
+  double t_before = 0.0, t_after = 0.0;
+  for (size_t g = 0; g < c->groups.size() && !rc; ++g) {
+    auto& gl = c->groups[g];
+    for (int dir = 0; dir < 2; ++dir) {
+      const int cur = dir == 0 ? gl.Gf : gl.Gb;
+      float best = 0.0f;
+      int bestG = cur;
+      for (int ci = 0; ci < 4; ++ci) {
+        const int G = kCand[ci];
+        const int* h = dh[ci] + offc[ci][g][dir * 2];
+        const int nh = offc[ci][g][dir * 2 + 1];
+        const float t = tune_time(c, [&] {
+          if (dir == 0) {
+            LAPGPU_DISPATCH_G(G, (launch_group<G_, T, double>), c, gl.lo, gl.hi, h, nh,
+                              c->gFptr, c->gFidx, gFw, y, rin, y);
+          } else {
+            LAPGPU_DISPATCH_G(G, (launch_group<G_, T, T>), c, gl.lo, gl.hi, h, nh, c->gBptr,
+                              c->gBidx, gBw, z, y, z);
+          }
+        });
+        if (G == cur) t_before += t;
+        if (ci == 0 || t < best) {
+          best = t;
+          bestG = G;
+        }
+      }
+      t_after += best;
+      (dir == 0 ? gl.Gf : gl.Gb) = bestG;
+    }
+  }
+  for (int ci = 0; ci < 4; ++ci) dev_free(c, dh[ci]);
+  if (rc) return rc;
+  // the final heavy list, in the layout upload_groups makes
+  std::vector<int32_t> heavy;
+  for (auto& gl : c->groups) {
+    gl.hoff_f = (int)heavy.size();
+    heavy_rows(B.fptr, gl.lo, gl.hi, gl.Gf, heavy);
+    gl.nhf = (int)heavy.size() - gl.hoff_f;
+    gl.hoff_b = (int)heavy.size();
+    heavy_rows(B.bptr, gl.lo, gl.hi, gl.Gb, heavy);
+    gl.nhb = (int)heavy.size() - gl.hoff_b;
+  }
+  if ((rc = dev_upload(c, &c->heavy, heavy))) return rc;
+  *before_ms += t_before;
+  *after_ms += t_after;
+  return 0;
+}
+
+template <int K, typename T>
+int tune_slices_t(lapgpu_ctx* c, T* y, T* z, const T* gFw, const T* gBw, double* before_ms,
+                  double* after_ms) {
+  auto& m = c->m;
+  HIPC(c, hipMemsetAsync(m.flags, 0, 4 * sizeof(int), c->stream));  // no column stopped
+  double t_before = 0.0, t_after = 0.0;
+  for (auto& gl : c->groups) {
+    for (int dir = 0; dir < 2; ++dir) {
+      const int cur = dir == 0 ? gl.Sf : gl.Sb;
+      float best = 0.0f;
+      int bestS = cur;
+      for (int S = 1; S <= 8 && S * K <= kWave; S *= 2) {
+        const float t = tune_time(c, [&] {
+          if (dir == 0)
+            launch_mgroup<K, T, double>(c, gl.lo, gl.hi, S, c->mheavy + gl.mhoff_f, gl.mnhf,
+                                        c->gFptr, c->gFidx, gFw, y, m.r, y);
+          else
+            launch_mgroup<K, T, T>(c, gl.lo, gl.hi, S, c->mheavy + gl.mhoff_b, gl.mnhb,
+                                   c->gBptr, c->gBidx, gBw, z, y, z);
+        });
+        if (S == std::min(cur, kWave / K)) t_before += t;
+        if (S == 1 || t < best) {
+          best = t;
+          bestS = S;
+        }
+      }
+      t_after += best;
+      (dir == 0 ? gl.Sf : gl.Sb) = bestS;
+    }
+  }
+  *before_ms += t_before;
+  *after_ms += t_after;
+  return 0;
+}
+
+template <int K>
+int tune_slices(lapgpu_ctx* c, double* before_ms, double* after_ms) {
+  int rc = multi_prepare(c, K);
+  if (rc) return rc;
+  auto& m = c->m;
+  if (c->precision == LAPGPU_FP32_INNER)
+    return tune_slices_t<K, float>(c, m.y32, m.z32, c->gFw32, c->gBw32, before_ms, after_ms);
+  return tune_slices_t<K, double>(c, m.y, m.z, c->gFw, c->gBw, before_ms, after_ms);
+}
+
+int tune_lanes(lapgpu_ctx* c, double* before_ms, double* after_ms) {
+  if (c->precision == LAPGPU_FP32_INNER)
+    return tune_lanes_t<float>(c, c->r, c->y32, c->z32, c->gFw32, c->gBw32, before_ms, after_ms);
+  return tune_lanes_t<double>(c, c->r, c->y, c->zF, c->gFw, c->gBw, before_ms, after_ms);
+}
+
+// device time (ms) of one LDLinv apply (sweeps only) with the current schedule
+float apply_time(lapgpu_ctx* c) { return tune_time(c, [&] { sweeps(c, c->r); }); }
+
+template <int K>
+float mapply_time(lapgpu_ctx* c) {
+  auto& m = c->m;
+  // no column stopped (a solve sets these states again when it starts)
+  (void)hipMemsetAsync(m.flags, 0, 4 * sizeof(int), c->stream);
+  (void)hipMemsetAsync(m.st, 0, sizeof(DevState) * K, c->stream);
+  return tune_time(c, [&] { msweeps<K>(c, m.r); });
+}
+
+// rebuild the level groups with the context's budget (captured iterations
+// and the FP32 group weights follow)
+int regroup(lapgpu_ctx* c) {
+  drop_graphs(c);
+  int rc = upload_groups(c);
+  if (rc) return rc;
+  if (c->f32_ready) {
+    if ((rc = dev_upload(c, &c->gFw32, narrow<float>(c->plan.blocks.fw)))) return rc;
+    if ((rc = dev_upload(c, &c->gBw32, narrow<float>(c->plan.blocks.bw)))) return rc;
+  }
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1260,6 +1435,61 @@ void lapgpu_destroy(lapgpu_ctx* c) {
     if (ev) (void)hipEventDestroy(ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+}
+
+int lapgpu_autotune(lapgpu_ctx* c, int64_t k, int flags, double* times_ms) {
+  if (!c) return -1;
+  if (!c->has_F) return set_err(c, -2, "autotune: upload the Laplacian and LDLinv first");
+  if (k < 0 || k > kMultiMax) return set_err(c, -1, "autotune: k must be in [0, 64]");
+  HIPC(c, hipSetDevice(c->device));
+  int K = 0;
+  if (k > 1) {
+    K = 8;
+    while (K < k) K *= 2;
+  }
+  int rc = 0;
+  double t[4] = {0, 0, 0, 0};
+  // multi-RHS workspace first (the apply timings below need it)
+  if (K) LAPGPU_DISPATCH_K(K, rc = multi_prepare, c, K_);
+  if (rc) return rc;
+  // a running state (a solve sets it again when it starts)
+  HIPC(c, hipMemsetAsync(c->st, 0, sizeof(DevState), c->stream));
+  t[0] = apply_time(c);
+  if (K) LAPGPU_DISPATCH_K(K, t[2] = mapply_time<K_>, c);
+  if (flags & LAPGPU_TUNE_GROUPS) {
+    // the level-group budget: each candidate regroups and times an apply
+    // (of K right-hand sides when K > 0)
+    const int64_t cand[3] = {65536, 196608, 524288};
+    float best = 0.0f;
+    int64_t bestx = c->block_extra;
+    for (int i = 0; i < 3; ++i) {
+      c->block_extra = cand[i];
+      if ((rc = regroup(c))) return rc;
+      float tt = apply_time(c);
+      if (K) LAPGPU_DISPATCH_K(K, tt = mapply_time<K_>, c);
+      if (i == 0 || tt < best) {
+        best = tt;
+        bestx = cand[i];
+      }
+    }
+    c->block_extra = bestx;
+    if ((rc = regroup(c))) return rc;
+  }
+  double b = 0.0, a = 0.0;
+  if ((rc = tune_lanes(c, &b, &a))) return rc;
+  if (K) {
+    LAPGPU_DISPATCH_K(K, rc = tune_slices<K_>, c, &b, &a);
+    if (rc) return rc;
+  }
+  drop_graphs(c);  // the next solve captures the tuned schedule
+  t[1] = apply_time(c);
+  if (K) LAPGPU_DISPATCH_K(K, t[3] = mapply_time<K_>, c);
+  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, hipGetLastError());
+  if (times_ms)
+    for (int i = 0; i < 4; ++i) times_ms[i] = t[i];
+  c->err.clear();
+  return 0;
 }
 
 const char* lapgpu_last_error(lapgpu_ctx* c) {

@@ -30,6 +30,7 @@ STOP_NAMES = {
 VERBOSE = 1
 NO_CENTER = 2
 PROFILE = 4
+TUNE_GROUPS = 1  # lapgpu_autotune: also tune the level-group budget
 FP64 = 0
 FP32_INNER = 1
 
@@ -50,6 +51,7 @@ EXPORTS = (
     "lapgpu_last_sweep_timings",
     "lapgpu_tail_info",
     "lapgpu_group_info",
+    "lapgpu_autotune",
     "lapgpu_last_error",
     "lapgpu_destroy",
     "lapgpu_batch_create",
@@ -99,6 +101,7 @@ def load() -> ctypes.CDLL:
     L.lapgpu_last_sweep_timings.argtypes = [P, pdbl, pdbl, pdbl, pdbl]
     L.lapgpu_tail_info.argtypes = [P, pi64, pi64, pi64, pi64]
     L.lapgpu_group_info.argtypes = [P, pi64, pi64, pi64, pi64]
+    L.lapgpu_autotune.argtypes = [P, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.lapgpu_last_error.argtypes = [P]
     L.lapgpu_last_error.restype = ctypes.c_char_p
     L.lapgpu_destroy.argtypes = [P]
@@ -365,6 +368,17 @@ class Context:
         v = [ctypes.c_int64() for _ in range(4)]
         self._check(self._lib.lapgpu_tail_info(self._h, *(ctypes.byref(t) for t in v)), "lapgpu_tail_info")
         return {"nt": v[0].value, "nsteps": v[1].value, "nslots": v[2].value, "nedges_in": v[3].value}
+
+    def autotune(self, k: int = 0, groups: bool = False) -> dict:
+        """Tune the schedule on this graph (lapgpu_autotune): lanes per row of
+        every level group, slices per row of k-column solves when k > 1, and
+        with groups=True the level-group budget.  Returns the device ms of
+        one LDLinv apply before/after (single and k-column)."""
+        t = (ctypes.c_double * 4)()
+        self._check(self._lib.lapgpu_autotune(self._h, int(k), TUNE_GROUPS if groups else 0, t),
+                    "lapgpu_autotune")
+        return {"apply_ms_before": t[0], "apply_ms_after": t[1], "multi_apply_ms_before": t[2],
+                "multi_apply_ms_after": t[3]}
 
     def group_info(self) -> dict:
         v = [ctypes.c_int64() for _ in range(4)]

@@ -1,0 +1,73 @@
+"""The schedule autotuner (lapgpu_autotune): after tuning the lanes per row
+of every level group (and the multi-RHS slices, and the level-group
+budget), solves still match the CPU oracle on the same factor: one LDLinv
+apply to 1e-12 relative (the lane split only reorders a row's sums), full
+solves with the same stop reason, iterations within 2 and true relres <=
+10 tol, in FP64 and in the FP32-inner mode, single and multi-RHS."""
+
+import numpy as np
+import pytest
+
+import lapgpu
+from lapgpu import _native, graphs
+
+pytestmark = pytest.mark.gpu
+
+
+def _rhs(n, seed):
+    b = np.random.default_rng(seed).standard_normal(n)
+    return b - b.mean()
+
+
+@pytest.fixture(scope="module")
+def case():
+    a = graphs.weighted_grid3(40, 36, 32, np.random.default_rng(21), "expU2")
+    return lapgpu.lap(a), lapgpu.build_ldli(a, seed=1)
+
+
+def test_autotune_single_with_groups(case, gpu_ctx_factory, oracle_mod):
+    la, f = case
+    ctx = gpu_ctx_factory(la, f)
+    t = ctx.autotune(k=0, groups=True)
+    assert t["apply_ms_before"] > 0 and t["apply_ms_after"] > 0
+    assert t["multi_apply_ms_before"] == 0 and t["multi_apply_ms_after"] == 0
+    r = _rhs(la.shape[0], 3)
+    z = ctx.apply_ldli(r)
+    zo = oracle_mod.ldlsolver(f, r)
+    assert np.linalg.norm(z - zo) <= 1e-12 * np.linalg.norm(zo)
+    b = _rhs(la.shape[0], 4)
+    x, info = ctx.solve(b, tol=1e-8, maxits=1000, stag_test=5)
+    _, io = oracle_mod.approxchol_solve(la, f, b, tol=1e-8, maxits=1000, stag_test=5)
+    assert info["stop"] == io["stop"] and abs(info["its"] - io["its"]) <= 2
+    assert np.linalg.norm(la @ x - b) <= 1e-7 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("k", [8, 16, 64])
+def test_autotune_multi(case, gpu_ctx_factory, oracle_mod, k):
+    la, f = case
+    n = la.shape[0]
+    ctx = gpu_ctx_factory(la, f)
+    t = ctx.autotune(k=k)
+    assert t["multi_apply_ms_before"] > 0 and t["multi_apply_ms_after"] > 0
+    B = np.stack([_rhs(n, 10 + j) for j in range(k)], axis=1)
+    X, info = ctx.solve_multi(B, tol=1e-8, maxits=1000)
+    for j in range(k):
+        assert np.linalg.norm(la @ X[:, j] - B[:, j]) <= 1e-7 * np.linalg.norm(B[:, j])
+    _, io = oracle_mod.approxchol_solve(la, f, B[:, 0], tol=1e-8, maxits=1000, stag_test=5)
+    assert info["stop"][0] == io["stop"] and abs(info["its"][0] - io["its"]) <= 2
+
+
+def test_autotune_fp32_inner(case, gpu_ctx_factory):
+    la, f = case
+    n = la.shape[0]
+    ctx = gpu_ctx_factory(la, f)
+    ctx.set_precision(_native.FP32_INNER, 20)
+    ctx.autotune(k=8, groups=True)
+    b = _rhs(n, 5)
+    x, info = ctx.solve(b, tol=1e-8, maxits=1000)
+    assert info["stop"] == "converged"
+    assert np.linalg.norm(la @ x - b) <= 1e-7 * np.linalg.norm(b)
+    B = np.stack([_rhs(n, 30 + j) for j in range(8)], axis=1)
+    X, info = ctx.solve_multi(B, tol=1e-8, maxits=1000)
+    for j in range(8):
+        assert np.linalg.norm(la @ X[:, j] - B[:, j]) <= 1e-7 * np.linalg.norm(B[:, j])
