@@ -397,8 +397,10 @@ def mode_multi(args) -> int:
         c1.close()
     plain_wall = k * t_big + len(smalls) * float(np.mean(t_small))
     plain_sps = (k + len(smalls)) / plain_wall
-    best_name = max(out_modes, key=lambda m: out_modes[m]["solves_per_s"])
+    # the workload names FP64 outer / FP32 inner; the FP64 leg is reported beside it
+    best_name = "fp32_inner"
     best = out_modes[best_name]
+    faster = max(out_modes, key=lambda m: out_modes[m]["solves_per_s"])
     emit({
         "metric": "solves/s (64-RHS block + 256 small grids, relres<=1e-8) vs plain FP64 GPU PCG",
         "value": best["solves_per_s"],
@@ -410,12 +412,13 @@ def mode_multi(args) -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if best_name == "fp64" else "f64 outer / f32 inner",
+        "dtype": "f64 outer / f32 inner",
         "data": "synthetic",
         "config": {
             "workload": "grid2 1024x1024 exp(N(0,1)) weights, 64 RHS N(0,1) sum-zero + 256 unit 2-D grids "
                         "(n log-uniform [1e3,1e5], aspect [1,8]), seed 4, tol 1e-8",
             "reported_mode": best_name,
+            "faster_mode": faster,
             "autotune": tune,
             "fp32_inner": out_modes["fp32_inner"],
             "fp64": out_modes["fp64"],

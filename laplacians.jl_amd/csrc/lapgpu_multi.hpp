@@ -537,37 +537,103 @@ k_mspmv_mixed(int n, const int* __restrict__ Lptr, const int* __restrict__ Lidx,
   const int wave = (blockIdx.x * kBlock + threadIdx.x) / kWave;
   const int nwaves = gridDim.x * (kBlock / kWave);
   double a[1] = {0.0};
-  for (int row = wave * R + sub; row < n; row += nwaves * R) {
-    const int k0 = wave_uniform<K>(Lptr[row]), k1 = wave_uniform<K>(Lptr[row + 1]);
-    double a64 = 0.0;
-    float a32 = 0.0f;
-    for (int base = k0; base < k1; base += K) {
-      const int kk = base + j;
-      int myi = 0;
-      double myw = 0.0;
-      float myw32 = 0.0f;
-      if (kk < k1) {
-        myi = Lidx[kk];
-        myw = Lval[kk];
-        myw32 = Lval32[kk];
-      }
-      const int cnt = min(K, k1 - base);
-      for (int e = 0; e < cnt; ++e) {
-        int i;
-        double w;
-        float w32;
-        mbcast<K>(myi, myw, sub * K + e, i, w);
-        int i2;
-        mbcast<K>(myi, myw32, sub * K + e, i2, w32);
-        if (f64)
-          a64 += w * p[(int64_t)i * K + j];
-        else
-          a32 += w32 * pf[(int64_t)i * K + j];
+  // RU rows in flight (as mrows_in_flight): pointers, first K indices and
+  // both weights, then up to 8 gathers per row, each lane from its
+  // column's precision; sums in entry order, in that precision
+  constexpr int RU = 2, U = 8;
+  const int stride = nwaves * R, g0 = sub * K;
+  for (int r0 = wave * R + sub; r0 < n; r0 += RU * stride) {
+    int k0[RU], k1[RU], myi[RU];
+    double myw[RU];
+    float myw32[RU];
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+      const int row = r0 + u * stride;
+      k0[u] = row < n ? wave_uniform<K>(Lptr[row]) : 0;
+      k1[u] = row < n ? wave_uniform<K>(Lptr[row + 1]) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+      const int kk = k0[u] + j;
+      myi[u] = 0;
+      myw[u] = 0.0;
+      myw32[u] = 0.0f;
+      if (kk < k1[u]) {
+        myi[u] = Lidx[kk];
+        myw[u] = Lval[kk];
+        myw32[u] = Lval32[kk];
       }
     }
-    const double acc = f64 ? a64 : (double)a32;
-    q[(int64_t)row * K + j] = acc;
-    a[0] += p[(int64_t)row * K + j] * acc;
+    int ii[RU][U];
+    double we[RU][U], v64[RU][U];
+    float we32[RU][U], v32[RU][U];
+#pragma unroll
+    for (int u = 0; u < RU; ++u)
+#pragma unroll
+      for (int e = 0; e < U; ++e) {
+        int i2;
+        mbcast<K>(myi[u], myw[u], g0 + e, ii[u][e], we[u][e]);
+        mbcast<K>(myi[u], myw32[u], g0 + e, i2, we32[u][e]);
+      }
+#pragma unroll
+    for (int u = 0; u < RU; ++u)
+#pragma unroll
+      for (int e = 0; e < U; ++e)
+        if (e < k1[u] - k0[u]) {
+          if (f64)
+            v64[u][e] = p[(int64_t)ii[u][e] * K + j];
+          else
+            v32[u][e] = pf[(int64_t)ii[u][e] * K + j];
+        }
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+      const int row = r0 + u * stride;
+      const int cnt = k1[u] - k0[u];
+      double a64 = 0.0;
+      float a32 = 0.0f;
+#pragma unroll
+      for (int e = 0; e < U; ++e)
+        if (e < cnt) {
+          if (f64)
+            a64 += we[u][e] * v64[u][e];
+          else
+            a32 += we32[u][e] * v32[u][e];
+        }
+      // the rest of the row (longer rows; rare)
+      for (int base = k0[u]; base < k1[u]; base += K) {
+        int mi = myi[u];
+        double mw = myw[u];
+        float mw32 = myw32[u];
+        if (base != k0[u]) {
+          const int kk = base + j;
+          mi = 0;
+          mw = 0.0;
+          mw32 = 0.0f;
+          if (kk < k1[u]) {
+            mi = Lidx[kk];
+            mw = Lval[kk];
+            mw32 = Lval32[kk];
+          }
+        }
+        const int c = min(K, k1[u] - base);
+        for (int e = base == k0[u] ? U : 0; e < c; ++e) {
+          int i, i2;
+          double w;
+          float w32;
+          mbcast<K>(mi, mw, g0 + e, i, w);
+          mbcast<K>(mi, mw32, g0 + e, i2, w32);
+          if (f64)
+            a64 += w * p[(int64_t)i * K + j];
+          else
+            a32 += w32 * pf[(int64_t)i * K + j];
+        }
+      }
+      if (row < n) {
+        const double acc = f64 ? a64 : (double)a32;
+        q[(int64_t)row * K + j] = acc;
+        a[0] += p[(int64_t)row * K + j] * acc;
+      }
+    }
   }
   const int slots[1] = {S_PQ};
   mblock_sum<K, 1>(a, part, slots);
