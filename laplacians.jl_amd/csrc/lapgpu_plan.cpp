@@ -23,6 +23,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <thread>
@@ -400,44 +401,96 @@ bool build_tail(Plan& P, int64_t max_steps, int64_t cap, int max_elog, int lanes
 
 namespace {
 
-// One composite row: its terms in insertion order, merged by index.  The
-// sort is stable in effect (the key holds the insertion number), so equal
-// indices are summed in insertion order starting from the first term: the
-// same sums, bit for bit, as a dense accumulator filled in that order.
-struct RowMerge {
-  struct Term {
-    int64_t key;  // index << 32 | insertion number
-    double w;
-  };
-  std::vector<Term> t;
-  uint32_t seq = 0;
-  void begin() {
-    t.clear();
-    seq = 0;
-  }
-  void add(int32_t i, double w) { t.push_back({((int64_t)i << 32) | seq++, w}); }
-  // append the merged row; returns its length
-  int32_t take(std::vector<int32_t>& idx, std::vector<double>& w) {
-    std::sort(t.begin(), t.end(), [](const Term& a, const Term& b) { return a.key < b.key; });
-    int32_t len = 0;
-    for (size_t k = 0; k < t.size();) {
-      const int32_t i = (int32_t)(t[k].key >> 32);
-      double sum = t[k].w;
-      for (++k; k < t.size() && (int32_t)(t[k].key >> 32) == i; ++k) sum += t[k].w;
-      idx.push_back(i);
-      w.push_back(sum);
-      ++len;
+// One composite row: its terms in insertion order, merged by index in a
+// per-thread table of positions (-1: index absent), so that equal indices
+// are summed in insertion order starting from the first term; the row is
+// emitted in index order.
+std::vector<int32_t>& merge_where() {
+  thread_local std::vector<int32_t> where;
+  return where;
+}
+
+// Sort distinct values that arrive as ascending runs (a composite row is a
+// union of sorted rows): pairwise merges of neighbouring runs, O(m log runs).
+void sort_runs(int32_t* a, size_t m, std::vector<int32_t>& tmp, std::vector<size_t>& bounds) {
+  bounds.clear();
+  bounds.push_back(0);
+  for (size_t k = 1; k < m; ++k)
+    if (a[k] < a[k - 1]) bounds.push_back(k);
+  bounds.push_back(m);
+  if (bounds.size() <= 2) return;
+  tmp.resize(m);
+  int32_t *src = a, *dst = tmp.data();
+  while (bounds.size() > 2) {
+    size_t nb = 0, r = 0;
+    for (; r + 2 < bounds.size(); r += 2) {
+      std::merge(src + bounds[r], src + bounds[r + 1], src + bounds[r + 1], src + bounds[r + 2],
+                 dst + bounds[r]);
+      bounds[nb++] = bounds[r];
     }
-    return len;
+    if (r + 1 < bounds.size()) {
+      std::copy(src + bounds[r], src + bounds[r + 1], dst + bounds[r]);
+      bounds[nb++] = bounds[r];
+    }
+    bounds[nb++] = m;
+    bounds.resize(nb);
+    std::swap(src, dst);
+  }
+  if (src != a) std::copy(src, src + m, a);
+}
+
+struct RowMerge {
+  std::vector<int32_t>& where;
+  std::vector<int32_t> ti, tmp;
+  std::vector<size_t> bounds;
+  std::vector<double> tw;
+  explicit RowMerge(int64_t span) : where(merge_where()) {
+    if ((int64_t)where.size() < span) where.assign((size_t)span, -1);
+  }
+  void begin() {
+    ti.clear();
+    tw.clear();
+  }
+  void add(int32_t i, double w) {
+    int32_t& p = where[i];
+    if (p < 0) {
+      p = (int32_t)ti.size();
+      ti.push_back(i);
+      tw.push_back(w);
+    } else {
+      tw[p] += w;
+    }
+  }
+  // append the merged row; returns its length (the table is left all -1)
+  int32_t take(std::vector<int32_t>& idx, std::vector<double>& w) {
+    const size_t m = ti.size(), at = idx.size();
+    idx.insert(idx.end(), ti.begin(), ti.end());
+    sort_runs(idx.data() + at, m, tmp, bounds);
+    w.resize(at + m);
+    for (size_t k = 0; k < m; ++k) {
+      const int32_t i = idx[at + k];
+      w[at + k] = tw[where[i]];
+      where[i] = -1;
+    }
+    return (int32_t)m;
   }
 };
 
-// Rows of one level computed in chunks of consecutive positions; the
-// chunks' outputs are concatenated in position order.
+// Rows of consecutive positions computed by one task.
 struct RowChunk {
   std::vector<int32_t> idx, len;
   std::vector<double> w;
   int64_t orig = 0;
+};
+
+// The composite rows of one sweep while they are built: the chunks stay in
+// place, and each position points at its row inside one of them.
+struct RowTable {
+  std::vector<std::unique_ptr<RowChunk>> chunks;
+  std::vector<const int32_t*> idx;
+  std::vector<const double*> w;
+  std::vector<int32_t> len;
+  explicit RowTable(int64_t rows) : idx(rows, nullptr), w(rows, nullptr), len(rows, 0) {}
 };
 
 }  // namespace
@@ -456,49 +509,22 @@ void build_blocks(Plan& P, int max_depth, double growth, int64_t max_extra) {
   }
   PlanThreads threads;
   Team team(threads.n);
-  constexpr int64_t kRows = 256;
-  std::vector<RowChunk> outs;
-  // grow v to n elements, doubling its capacity when it must grow
-  auto grow_to = [](auto& v, size_t n) {
-    if (n > v.capacity()) v.reserve(std::max(n, 2 * v.capacity()));
-    v.resize(n);
-  };
-  // append the chunks' rows [lo, lo + rows) to (ptr, idx, w): ptr[v + 1]
-  // (relative to `origin`) gets each row's end, the copies run in parallel
-  std::vector<int64_t> at;
-  auto append_rows = [&](int64_t nc, int64_t lo, std::vector<int64_t>& ends,
-                         int64_t origin, std::vector<int32_t>& idx, std::vector<double>& w) {
-    at.assign(nc + 1, (int64_t)idx.size());
-    for (int64_t c = 0; c < nc; ++c) at[c + 1] = at[c] + (int64_t)outs[c].idx.size();
-    grow_to(idx, (size_t)at[nc]);
-    grow_to(w, (size_t)at[nc]);
+  // rows [lo, hi) in chunks of `per` rows: row(v, rm) adds v's terms and
+  // returns its original entries
+  using Chunks = std::vector<std::unique_ptr<RowChunk>>;
+  auto run_rows = [&](int64_t lo, int64_t hi, int64_t per, Chunks& out, const auto& row) {
+    const int64_t nc = (hi - lo + per - 1) / per;
+    out.resize(nc);
     team.for_each(nc, 1, [&](int64_t c0, int64_t c1) {
+      RowMerge rm(n + T0);
       for (int64_t c = c0; c < c1; ++c) {
-        const RowChunk& O = outs[c];
-        std::copy(O.idx.begin(), O.idx.end(), idx.begin() + at[c]);
-        std::copy(O.w.begin(), O.w.end(), w.begin() + at[c]);
-        int64_t e = at[c], v = lo + c * kRows - origin;
-        for (int32_t len : O.len) {
-          e += len;
-          ends[v++] = e;
-        }
-      }
-    });
-  };
-  // rows [lo, hi): row(v, rm) adds v's terms and returns its original
-  // entries; returns the number of chunks
-  auto run_rows = [&](int64_t lo, int64_t hi, const auto& row) {
-    const int64_t nc = (hi - lo + kRows - 1) / kRows;
-    if ((int64_t)outs.size() < nc) outs.resize(nc);
-    team.for_each(nc, 1, [&](int64_t c0, int64_t c1) {
-      RowMerge rm;
-      for (int64_t c = c0; c < c1; ++c) {
-        RowChunk& O = outs[c];
+        if (!out[c]) out[c] = std::make_unique<RowChunk>();
+        RowChunk& O = *out[c];
         O.idx.clear();
         O.w.clear();
         O.len.clear();
         O.orig = 0;
-        const int64_t a = lo + c * kRows, b = std::min(hi, a + kRows);
+        const int64_t a = lo + c * per, b = std::min(hi, a + per);
         for (int64_t v = a; v < b; ++v) {
           rm.begin();
           O.orig += row(v, rm);
@@ -506,38 +532,73 @@ void build_blocks(Plan& P, int max_depth, double growth, int64_t max_extra) {
         }
       }
     });
-    return nc;
+  };
+  // rows per chunk: about four chunks per thread, 16..256 rows
+  auto per_chunk = [&](int64_t rows) {
+    const int64_t want = (rows + 4 * team.size() - 1) / (4 * team.size());
+    return std::max<int64_t>(16, std::min<int64_t>(256, want));
+  };
+  // hand the chunks of rows [lo, ...) to the table
+  auto commit = [](RowTable& T, int64_t lo, int64_t per, Chunks& cs) {
+    for (size_t c = 0; c < cs.size(); ++c) {
+      const RowChunk& O = *cs[c];
+      int64_t v = lo + (int64_t)c * per, e = 0;
+      for (int32_t len : O.len) {
+        T.idx[v] = O.idx.data() + e;
+        T.w[v] = O.w.data() + e;
+        T.len[v] = len;
+        e += len;
+        ++v;
+      }
+      T.chunks.push_back(std::move(cs[c]));
+    }
+    cs.clear();
+  };
+  // the table's rows [0, T0) in position order
+  auto copy_out = [&](const RowTable& T, std::vector<int32_t>& ptr, std::vector<int32_t>& idx,
+                      std::vector<double>& w) {
+    for (int64_t v = 0; v < T0; ++v) ptr[v + 1] = ptr[v] + T.len[v];
+    idx.resize((size_t)ptr[T0]);
+    w.resize((size_t)ptr[T0]);
+    team.for_each(T0, 4096, [&](int64_t a, int64_t b) {
+      for (int64_t v = a; v < b; ++v) {
+        std::copy(T.idx[v], T.idx[v] + T.len[v], idx.begin() + ptr[v]);
+        std::copy(T.w[v], T.w[v] + T.len[v], w.begin() + ptr[v]);
+      }
+    });
   };
 
   // forward composites and the grouping (forward rows of a group's earlier
-  // levels do not change when the group grows at its end); rows are stored
-  // in position order as each level is settled
+  // levels do not change when the group grows at its end)
+  RowTable F(T0);
+  Chunks cur;
   int64_t L0 = 0;
   B.lev.push_back(0);
   int64_t orig_block = 0, comp_block = 0;
-  std::vector<int64_t> fend(T0 + 1, 0);  // row ends, position v at v + 1
   for (int64_t l = 0; l < L; ++l) {
     const int64_t lo = P.lvl[l], hi = P.lvl[l + 1];
-    int64_t nc = 0, orig = 0, comp = 0;
+    const int64_t per = per_chunk(hi - lo);
+    int64_t orig = 0, comp = 0;
     auto compose = [&](int64_t start) {
       const int32_t s0 = P.lvl[start];
-      nc = run_rows(lo, hi, [&](int64_t v, RowMerge& rm) -> int64_t {
+      run_rows(lo, hi, per, cur, [&](int64_t v, RowMerge& rm) -> int64_t {
         rm.add((int32_t)(n + v), 1.0);
         for (int32_t k = P.Fptr[v]; k < P.Fptr[v + 1]; ++k) {
           const int32_t c = P.Fidx[k];
           if (c < s0) {
             rm.add(c, P.Fw[k]);
           } else {
-            for (int32_t e = B.fptr[c]; e < B.fptr[c + 1]; ++e)
-              rm.add(B.fidx[e], P.Fw[k] * B.fw[e]);
+            const int32_t* ci = F.idx[c];
+            const double* cw = F.w[c];
+            for (int32_t e = 0; e < F.len[c]; ++e) rm.add(ci[e], P.Fw[k] * cw[e]);
           }
         }
         return P.Fptr[v + 1] - P.Fptr[v] + 1;
       });
       orig = comp = 0;
-      for (int64_t c = 0; c < nc; ++c) {
-        orig += outs[c].orig;
-        comp += (int64_t)outs[c].idx.size();
+      for (const auto& O : cur) {
+        orig += O->orig;
+        comp += (int64_t)O->idx.size();
       }
     };
     bool joined = false;
@@ -557,57 +618,39 @@ void build_blocks(Plan& P, int max_depth, double growth, int64_t max_extra) {
       orig_block = orig;
       comp_block = comp;
     }
-    append_rows(nc, lo, fend, -1, B.fidx, B.fw);
-    for (int64_t v = lo; v < hi; ++v) B.fptr[v + 1] = (int32_t)fend[v + 1];
+    commit(F, lo, per, cur);
   }
   B.lev.push_back((int32_t)L);
-  // backward composites per group (top level first), kept per group and
-  // then stored in position order
-  std::vector<int64_t> gend;  // group rows: end of position base + i at i
-  std::vector<int32_t> glen, gidx;
-  std::vector<double> gw;
+  copy_out(F, B.fptr, B.fidx, B.fw);
+  F = RowTable(0);
+  // backward composites per group, top level first: a row's in-group
+  // terms expand into the rows of the group's higher levels
+  RowTable Bk(T0);
   for (size_t g = 0; g + 1 < B.lev.size(); ++g) {
     const int64_t g0 = B.lev[g], g1 = B.lev[g + 1];
     const int32_t top = P.lvl[g1];
-    const int64_t base = P.lvl[g0];
-    gend.assign(top - base, 0);
-    glen.assign(top - base, 0);
-    gidx.clear();
-    gw.clear();
     for (int64_t l = g1 - 1; l >= g0; --l) {
       const int64_t lo = P.lvl[l], hi = P.lvl[l + 1];
-      const int64_t nc = run_rows(lo, hi, [&](int64_t v, RowMerge& rm) -> int64_t {
+      const int64_t per = per_chunk(hi - lo);
+      run_rows(lo, hi, per, cur, [&](int64_t v, RowMerge& rm) -> int64_t {
         rm.add((int32_t)(n + v), P.s[v]);
         for (int32_t k = P.Bptr[v]; k < P.Bptr[v + 1]; ++k) {
           const int32_t j = P.Bidx[k];
           if (j >= top) {
             rm.add(j, P.Bw[k]);
           } else {
-            const int64_t o = gend[j - base] - glen[j - base];
-            for (int32_t e = 0; e < glen[j - base]; ++e)
-              rm.add(gidx[o + e], P.Bw[k] * gw[o + e]);
+            const int32_t* ji = Bk.idx[j];
+            const double* jw = Bk.w[j];
+            for (int32_t e = 0; e < Bk.len[j]; ++e) rm.add(ji[e], P.Bw[k] * jw[e]);
           }
         }
         return 0;
       });
-      append_rows(nc, lo, gend, base, gidx, gw);
-      for (int64_t c = 0; c < nc; ++c) {
-        int64_t v = lo + c * kRows - base;
-        for (int32_t len : outs[c].len) glen[v++] = len;
-      }
+      commit(Bk, lo, per, cur);
     }
-    // the group's rows in position order
-    for (int64_t v = base; v < top; ++v) B.bptr[v + 1] = B.bptr[v] + glen[v - base];
-    grow_to(B.bidx, (size_t)B.bptr[top]);
-    grow_to(B.bw, (size_t)B.bptr[top]);
-    team.for_each(top - base, 1024, [&](int64_t a, int64_t b) {
-      for (int64_t i = a; i < b; ++i) {
-        const int64_t o = gend[i] - glen[i];
-        std::copy(gidx.begin() + o, gidx.begin() + o + glen[i], B.bidx.begin() + B.bptr[base + i]);
-        std::copy(gw.begin() + o, gw.begin() + o + glen[i], B.bw.begin() + B.bptr[base + i]);
-      }
-    });
   }
+  copy_out(Bk, B.bptr, B.bidx, B.bw);
+  merge_where() = std::vector<int32_t>();
 }
 
 bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
