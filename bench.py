@@ -545,7 +545,7 @@ def mode_serve(args) -> int:
         f"(sum {sum(g.shape[0] for g in G)}), factored in {time.time() - t0:.1f}s")
     srv = Server(0, hbm_budget=200 * 2**30, plain_workers=args.serve_workers,
                  factor_workers=args.factor_workers, batch_window=args.batch_window,
-                 max_wait=args.max_wait)
+                 max_wait=args.max_wait, quick_workers=args.quick_workers)
     for i, (g, f) in enumerate(zip(G, F)):
         assert srv.register(i, g, ldli=f)
     # request stream
@@ -568,7 +568,7 @@ def mode_serve(args) -> int:
     srv.run([Request(-1 - i, 0.0, wrng.standard_normal(G[i].shape[0]), graph=i) for i in range(len(G))],
             realtime=False)
     t0 = time.perf_counter()
-    res = srv.run(reqs)
+    res = srv.run(reqs, progress=lambda d, t, e: log(f"  served {d} of {t} requests at {e:.0f} s"))
     wall = time.perf_counter() - t0
     s = summary(res, wall)
     paths = {p: sum(o.path == p for o in res) for p in {o.path for o in res}}
@@ -678,6 +678,8 @@ def main() -> int:
     ap.add_argument("--new-frac", type=float, default=0.2)
     ap.add_argument("--serve-workers", type=int, default=8, help="serving: graphs solved concurrently")
     ap.add_argument("--factor-workers", type=int, default=8, help="serving: host threads factoring new graphs")
+    ap.add_argument("--quick-workers", type=int, default=4,
+                    help="serving: workers for solves of graphs with n <= 3e5 (0: share the others)")
     ap.add_argument("--batch-window", type=float, default=2.0, help="serving: wait for more requests, x last solve time")
     ap.add_argument("--max-wait", type=float, default=3.0, help="serving: longest wait for more requests (s)")
     ap.add_argument("--hw-queues", type=int, default=16, help="serving: HIP hardware queues (0: HIP's default)")

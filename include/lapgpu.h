@@ -51,6 +51,10 @@ enum lapgpu_stop {
 /* lapgpu_solve flags */
 #define LAPGPU_VERBOSE 1      /* print the reference's verbose lines        */
 #define LAPGPU_NO_CENTER 2    /* skip the closure's b .- mean(b) (plain pcg) */
+#define LAPGPU_EAGER 8        /* launch this solve's iterations directly; do
+                                 not capture them as a HIP graph (a context
+                                 solved once skips the capture; no reference
+                                 counterpart)                                */
 
 /* Precision modes (lapgpu_set_precision). */
 #define LAPGPU_FP64 0         /* everything in FP64 (the reference's path)   */
@@ -129,6 +133,20 @@ int lapgpu_spmv(lapgpu_ctx* ctx, const double* x, double* y, int64_t n);
 
 /* Select the precision mode for later solves (LAPGPU_FP64 by default). */
 int lapgpu_set_precision(lapgpu_ctx* ctx, int mode, int replace_every);
+
+/* Level-group budget of the LDLinv schedule (no reference counterpart; the
+ * reference has no device schedule).  The levels before the tail are run as
+ * groups, one launch per group and sweep; a group spans at most max_depth
+ * levels, its composite rows hold at most max_growth times the entries of
+ * the rows they replace, and each level joined to a group adds at most
+ * max_extra entries.  Defaults 32 / 16 / 196608 (measured on 3-D grids:
+ * ~4 us per launch against ~8 ps per entry).  A smaller budget plans faster
+ * on the host (fewer composite entries) at more launches per iteration:
+ * 1 / 1 / 0 gives one group per level.  Applies to the next
+ * lapgpu_upload_ldli; with a factor already uploaded the groups are rebuilt
+ * at once (waits for the context's stream). */
+int lapgpu_set_group_budget(lapgpu_ctx* ctx, int max_depth, double max_growth,
+                            int64_t max_extra);
 
 /* Schedule facts of the uploaded factor: number of levels, number of levels
  * run one launch each, the first vertex position of the single-workgroup

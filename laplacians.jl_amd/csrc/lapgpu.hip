@@ -727,7 +727,8 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
   int64_t enq = 0, polled = 0;
   bool done = c->hst->stopped != 0;
   const int rep_every = f32 ? c->replace_every : 0;
-  if (!prof && c->use_graph) {
+  const bool captured = !prof && c->use_graph && !(flags & LAPGPU_EAGER);
+  if (captured) {
     for (int v = 0; v < (f32 ? 2 : 1); ++v) {
       hipGraphExec_t* ex = v ? &c->iter_exec_rep : &c->iter_exec;
       if (*ex) continue;
@@ -745,7 +746,7 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
     // iteration number enq + 1 of this solve (itcnt on the device)
     const bool replace = rep_every > 0 && ((enq + 1) % rep_every == 0);
     hipGraphExec_t ex = replace ? c->iter_exec_rep : c->iter_exec;
-    if (!prof && ex)
+    if (captured && ex)
       HIPC(c, hipGraphLaunch(ex, c->stream));
     else
       enqueue_iteration(c, prof, replace);
@@ -1075,11 +1076,12 @@ int solve_multi_chunk(lapgpu_ctx* c, const double* d_B, double* d_X, int k, doub
   bool done = c->hflags[0] != 0;
 
   const int rep_every = f32 ? c->replace_every : 0;
-  if (!prof && c->use_graph && m.exec_K != K) {
+  const bool captured = !prof && c->use_graph && !(flags & LAPGPU_EAGER);
+  if (captured && m.exec_K != K) {
     drop_graphs(c);
     m.exec_K = K;
   }
-  if (!prof && c->use_graph) {
+  if (captured) {
     for (int v = 0; v < (f32 ? 2 : 1); ++v) {
       if (m.exec[v]) continue;
       LAPGPU_LOCK();
@@ -1096,7 +1098,7 @@ int solve_multi_chunk(lapgpu_ctx* c, const double* d_B, double* d_X, int k, doub
   while (!done) {
     const bool replace = rep_every > 0 && ((enq + 1) % rep_every == 0);
     hipGraphExec_t ex = m.exec[replace ? 1 : 0];
-    if (!prof && ex)
+    if (captured && ex)
       HIPC(c, hipGraphLaunch(ex, c->stream));
     else
       enqueue_miter<K>(c, prof, replace);
@@ -1665,6 +1667,25 @@ int lapgpu_set_precision(lapgpu_ctx* c, int mode, int replace_every) {
   if (mode == LAPGPU_FP32_INNER && c->has_L) {
     int rc = prepare_f32(c);
     if (rc) return rc;
+  }
+  c->err.clear();
+  return 0;
+}
+
+int lapgpu_set_group_budget(lapgpu_ctx* c, int max_depth, double max_growth,
+                            int64_t max_extra) {
+  if (!c) return -1;
+  if (max_depth < 1 || !(max_growth >= 1.0) || max_extra < 0)
+    return set_err(c, -1, "set_group_budget: need max_depth >= 1, max_growth >= 1, max_extra >= 0");
+  HIPC(c, hipSetDevice(c->device));
+  c->block_depth = max_depth;
+  c->block_growth = max_growth;
+  c->block_extra = max_extra;
+  if (c->has_F) {
+    HIPC(c, hipStreamSynchronize(c->stream));
+    int rc = regroup(c);
+    if (rc) return rc;
+    HIPC(c, hipStreamSynchronize(c->stream));
   }
   c->err.clear();
   return 0;

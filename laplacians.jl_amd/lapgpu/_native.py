@@ -30,6 +30,7 @@ STOP_NAMES = {
 VERBOSE = 1
 NO_CENTER = 2
 PROFILE = 4
+EAGER = 8
 TUNE_GROUPS = 1  # lapgpu_autotune: also tune the level-group budget
 FP64 = 0
 FP32_INNER = 1
@@ -46,6 +47,7 @@ EXPORTS = (
     "lapgpu_apply_ldli",
     "lapgpu_spmv",
     "lapgpu_set_precision",
+    "lapgpu_set_group_budget",
     "lapgpu_factor_info",
     "lapgpu_last_timings",
     "lapgpu_last_sweep_timings",
@@ -96,6 +98,7 @@ def load() -> ctypes.CDLL:
     L.lapgpu_apply_ldli.argtypes = [P, P, P, i64]
     L.lapgpu_spmv.argtypes = [P, P, P, i64]
     L.lapgpu_set_precision.argtypes = [P, ci, ci]
+    L.lapgpu_set_group_budget.argtypes = [P, ci, dbl, i64]
     L.lapgpu_factor_info.argtypes = [P, pi64, pi64, pi64, pi64]
     L.lapgpu_last_timings.argtypes = [P, pdbl, pdbl, pdbl, pdbl, pi64]
     L.lapgpu_last_sweep_timings.argtypes = [P, pdbl, pdbl, pdbl, pdbl]
@@ -349,6 +352,12 @@ class Context:
 
     def set_precision(self, mode: int, replace_every: int = 20) -> None:
         self._check(self._lib.lapgpu_set_precision(self._h, int(mode), int(replace_every)), "lapgpu_set_precision")
+
+    def set_group_budget(self, max_depth: int = 32, max_growth: float = 16.0, max_extra: int = 196608) -> None:
+        """Level-group budget of the LDLinv schedule (applies to the next
+        upload_ldli; regroups an uploaded factor at once)."""
+        self._check(self._lib.lapgpu_set_group_budget(self._h, int(max_depth), float(max_growth), int(max_extra)),
+                    "lapgpu_set_group_budget")
 
     def factor_info(self) -> dict:
         a, b, c, d = (ctypes.c_int64() for _ in range(4))

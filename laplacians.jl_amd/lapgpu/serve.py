@@ -23,6 +23,9 @@ over time; each is answered by the same solve closure as approxchol_lap's
   device keeps solving; the same threads create a large graph's context and
   upload it (its HBM is reserved first), and one more thread closes dropped
   contexts, so the dispatch loop itself never waits on an upload or a free.
+  A graph that comes with its request is planned with a small level-group
+  budget (`new_graph_budget`): fewer composite rows to build on the host, a
+  few more launches on the device for its one solve.
 
 Every result carries its latency (arrival to completion); `summary` gives
 solves/s and p50/p99 latency.
@@ -107,7 +110,8 @@ class Server:
     def __init__(self, device: int = 0, hbm_budget: int | None = None, wg_max_n: int = 20_000,
                  max_launch: int = 512, plain_workers: int = 4, factor_workers: int = 4,
                  keep_x: bool = False, max_rhs: int = 64, batch_window: float = 0.5,
-                 max_wait: float = 1.0):
+                 max_wait: float = 1.0, new_graph_budget: tuple | None = (8, 2.0, 16384),
+                 eager_new: bool = True, quick_workers: int = 4, quick_max_n: int = 300_000):
         self.device = device
         self.batch = _native.Batch(device)
         self.hbm_budget = hbm_budget if hbm_budget is not None else 200 * 2**30
@@ -120,6 +124,12 @@ class Server:
         # that has fewer than max_rhs requests queued collects more first
         self.batch_window = batch_window
         self.max_wait = max_wait
+        # level-group budget (lapgpu_set_group_budget) of graphs that come
+        # with a request and are dropped after it: planning composite rows
+        # costs host time in proportion to their entries, and a graph solved
+        # once recovers little of it (None: the library's default budget)
+        self.new_graph_budget = new_graph_budget
+        self.eager_new = eager_new
         # (n, columns, seconds, first on its context) of each plain / multi-RHS solve
         self.solve_log: list = []
         # seconds the dispatch loop spent admitting and dropping graphs
@@ -131,6 +141,11 @@ class Server:
         self.graphs: "OrderedDict[object, _Graph]" = OrderedDict()
         self.hbm_used = 0
         self._plain_pool = ThreadPoolExecutor(max_workers=plain_workers)
+        # solves of graphs with at most quick_max_n vertices (tens of ms) get
+        # workers of their own, so they never queue behind the multi-second
+        # multi-RHS solves of large graphs
+        self.quick_max_n = quick_max_n
+        self._quick_pool = ThreadPoolExecutor(max_workers=quick_workers) if quick_workers > 0 else None
         self._factor_pool = ThreadPoolExecutor(max_workers=factor_workers)
         self._close_pool = ThreadPoolExecutor(max_workers=1)
         self._anon = 0
@@ -165,9 +180,11 @@ class Server:
                 need += multi_bytes_estimate(n, k)
         return need, k
 
-    def _upload_context(self, la, ldli):
+    def _upload_context(self, la, ldli, budget=None):
         t0 = time.perf_counter()
         ctx = _native.Context(self.device)
+        if budget is not None:
+            ctx.set_group_budget(*budget)
         t1 = time.perf_counter()
         ctx.upload_lap(la)
         t2 = time.perf_counter()
@@ -217,9 +234,12 @@ class Server:
         del self.graphs[g.key]
 
     # --------------------------------------------------------------- run ---
-    def run(self, requests: list[Request], realtime: bool = True) -> list[Result]:
+    def run(self, requests: list[Request], realtime: bool = True, progress=None,
+            progress_every: float = 20.0) -> list[Result]:
         """Serve `requests` (sorted by arrival).  With realtime=False every
-        request is taken to have arrived at time 0 (throughput mode)."""
+        request is taken to have arrived at time 0 (throughput mode).
+        progress(done, total, seconds), if given, is called by the dispatch
+        loop every progress_every seconds."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         results: dict[int, Result] = {}
         arrival_abs = {}
@@ -233,6 +253,7 @@ class Server:
         t0 = time.perf_counter()
         nxt = 0
         total = len(reqs)
+        next_progress = progress_every
 
         def finish(r: Request, info: dict, x, path: str):
             lat = time.perf_counter() - arrival_abs[r.rid]
@@ -260,12 +281,15 @@ class Server:
                    and g.pending[0].maxits == take[0].maxits):
                 take.append(g.pending.popleft())
             r0 = take[0]
+            # a graph that came with its request is solved without capturing
+            # its iteration as a HIP graph (the capture would be used once)
+            flags = _native.EAGER if (self.eager_new and not g.resident) else 0
             t_start = time.perf_counter()
             if len(take) == 1:
-                x, info = g.handle.solve(r0.b, r0.tol, r0.maxits, float("inf"), 5, 0)
+                x, info = g.handle.solve(r0.b, r0.tol, r0.maxits, float("inf"), 5, flags)
             else:
                 B = np.stack([r.b for r in take], axis=1)
-                x, info = g.handle.solve_multi(B, r0.tol, r0.maxits, float("inf"), 5, 0)
+                x, info = g.handle.solve_multi(B, r0.tol, r0.maxits, float("inf"), 5, flags)
             dt = time.perf_counter() - t_start
             self.solve_log.append((g.n, len(take), dt, g.solves == 0))
             g.solves += 1
@@ -281,7 +305,9 @@ class Server:
                 return
             waiting.discard(g.key)
             g.busy = 1
-            plain_futs[self._plain_pool.submit(solve_queued, g)] = g
+            quick = self._quick_pool is not None and g.n <= self.quick_max_n
+            pool = self._quick_pool if quick else self._plain_pool
+            plain_futs[pool.submit(solve_queued, g)] = g
 
         while len(results) < total:
             now = time.perf_counter() - t0
@@ -317,7 +343,7 @@ class Server:
                 else:
                     # reserved now, uploaded on a worker thread
                     self.hbm_used += need
-                    upload_futs[self._factor_pool.submit(self._upload_context, la, ldli)] = (
+                    upload_futs[self._factor_pool.submit(self._upload_context, la, ldli, self.new_graph_budget)] = (
                         key, r, la.shape[0], need, k)
                 self.stats["admit_s"] += time.perf_counter() - ta
                 self.stats["admits"] += 1
@@ -376,6 +402,9 @@ class Server:
                 self._drop(self.graphs[key])
                 self.stats["drop_s"] += time.perf_counter() - td
                 self.stats["drops"] += 1
+            if progress is not None and time.perf_counter() - t0 >= next_progress:
+                progress(len(results), total, time.perf_counter() - t0)
+                next_progress += progress_every
             if not progressed:
                 time.sleep(50e-6)
         return [results[r.rid] for r in requests]
@@ -389,6 +418,8 @@ class Server:
 
     def close(self):
         self._plain_pool.shutdown(wait=True)
+        if self._quick_pool is not None:
+            self._quick_pool.shutdown(wait=True)
         self._factor_pool.shutdown(wait=True)
         self._close_pool.shutdown(wait=True)
         for g in list(self.graphs.values()):

@@ -71,3 +71,43 @@ def test_autotune_fp32_inner(case, gpu_ctx_factory):
     X, info = ctx.solve_multi(B, tol=1e-8, maxits=1000)
     for j in range(8):
         assert np.linalg.norm(la @ X[:, j] - B[:, j]) <= 1e-7 * np.linalg.norm(B[:, j])
+
+
+def test_group_budget(case, gpu_ctx_factory, oracle_mod):
+    """lapgpu_set_group_budget before an upload and on an uploaded factor:
+    one group per level at 1 / 1 / 0 (no composite entries beyond the
+    rows' own), and every budget applies the same LDLinv as the oracle."""
+    la, f = case
+    n = la.shape[0]
+    r = _rhs(n, 5)
+    zo = oracle_mod.ldlsolver(f, r)
+    ref = gpu_ctx_factory(la, f)
+    g_def = ref.group_info()
+    ctx = lapgpu.Context(0)
+    try:
+        ctx.set_group_budget(1, 1.0, 0)
+        ctx.upload_lap(la)
+        ctx.upload_ldli(f)
+        g_one = ctx.group_info()
+        assert g_one["rows"] == g_def["rows"]
+        assert g_one["ngroups"] >= g_def["ngroups"]
+        assert g_one["fentries"] <= g_def["fentries"] and g_one["bentries"] <= g_def["bentries"]
+        assert g_one["ngroups"] == ctx.factor_info()["nwide"]
+        for c in (ref, ctx):
+            z = c.apply_ldli(r)
+            assert np.linalg.norm(z - zo) <= 1e-12 * np.linalg.norm(zo)
+        ctx.set_group_budget(8, 2.0, 16384)
+        g_mid = ctx.group_info()
+        assert g_def["ngroups"] <= g_mid["ngroups"] <= g_one["ngroups"]
+        z = ctx.apply_ldli(r)
+        assert np.linalg.norm(z - zo) <= 1e-12 * np.linalg.norm(zo)
+        b = _rhs(n, 6)
+        x, info = ctx.solve(b, tol=1e-8, maxits=1000, stag_test=5)
+        _, io = oracle_mod.approxchol_solve(la, f, b, tol=1e-8, maxits=1000, stag_test=5)
+        assert info["stop"] == io["stop"] and abs(info["its"] - io["its"]) <= 2
+        assert np.linalg.norm(la @ x - b) <= 1e-7 * np.linalg.norm(b)
+        for bad in ((0, 2.0, 0), (4, 0.5, 0), (4, 2.0, -1)):
+            with pytest.raises(_native.LapGPUError):
+                ctx.set_group_budget(*bad)
+    finally:
+        ctx.close()
