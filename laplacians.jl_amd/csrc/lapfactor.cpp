@@ -19,6 +19,7 @@
 #include "lapfactor.h"
 
 #include <algorithm>
+#include <utility>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -63,6 +64,28 @@ struct Rng {
   }
   double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
 };
+
+// Stable sort of a short index list by key(x): insertion sort (no buffer
+// allocation), std::stable_sort for long lists.  Both keep equal keys in
+// their input order, so the result is the same.
+template <class Key>
+void stable_sort_by(std::vector<int64_t>& v, size_t len, const Key& key) {
+  if (len > 48) {
+    std::stable_sort(v.begin(), v.begin() + len,
+                     [&](int64_t x, int64_t y) { return key(x) < key(y); });
+    return;
+  }
+  for (size_t a = 1; a < len; ++a) {
+    const int64_t x = v[a];
+    const auto kx = key(x);
+    size_t b = a;
+    while (b > 0 && kx < key(v[b - 1])) {
+      v[b] = v[b - 1];
+      --b;
+    }
+    v[b] = x;
+  }
+}
 
 struct Edge {
   int64_t row;
@@ -322,9 +345,7 @@ int lapfactor_build(int64_t n, const int64_t* colptr_in,
 
       // compressCol! (src/approxChol.jl:415-449): merge multi-edges by row,
       // then order by weight (ties: ascending row)
-      std::stable_sort(cs.begin(), cs.end(), [&](int64_t x, int64_t y) {
-        return E[x].row < E[y].row;
-      });
+      stable_sort_by(cs, cs.size(), [&](int64_t x) { return E[x].row; });
       size_t len = 0;
       int64_t currow = -1;
       for (size_t t = 0; t < cs.size(); ++t) {
@@ -339,9 +360,7 @@ int lapfactor_build(int64_t n, const int64_t* colptr_in,
         }
       }
       cs.resize(len);
-      std::stable_sort(cs.begin(), cs.end(), [&](int64_t x, int64_t y) {
-        return E[x].val < E[y].val;
-      });
+      stable_sort_by(cs, len, [&](int64_t x) { return E[x].val; });
 
       vals.resize(len);
       cum.resize(len);
@@ -511,7 +530,7 @@ int lapfactor_build_split(int64_t n, const int64_t* colptr_in, const int64_t* ro
       // mean weight (both directions); rows with more than `merge` copies
       // keep `merge` copies carrying the whole weight, the rest are dropped;
       // then order by (weight, row)
-      std::stable_sort(cs.begin(), cs.end(), [&](int64_t x, int64_t y) { return E[x].row < E[y].row; });
+      stable_sort_by(cs, cs.size(), [&](int64_t x) { return E[x].row; });
       size_t len = 0;
       for (size_t a0 = 0; a0 < cs.size();) {
         size_t a1 = a0;
@@ -534,9 +553,7 @@ int lapfactor_build_split(int64_t n, const int64_t* colptr_in, const int64_t* ro
         a0 = a1;
       }
       cs.resize(len);
-      std::stable_sort(cs.begin(), cs.end(), [&](int64_t x, int64_t y) {
-        return E[x].val < E[y].val || (E[x].val == E[y].val && E[x].row < E[y].row);
-      });
+      stable_sort_by(cs, len, [&](int64_t x) { return std::make_pair(E[x].val, E[x].row); });
       vals.resize(len);
       crev.assign(len + 1, 0.0);
       double csum = 0.0;
