@@ -545,7 +545,8 @@ def mode_serve(args) -> int:
         f"(sum {sum(g.shape[0] for g in G)}), factored in {time.time() - t0:.1f}s")
     srv = Server(0, hbm_budget=200 * 2**30, plain_workers=args.serve_workers,
                  factor_workers=args.factor_workers, batch_window=args.batch_window,
-                 max_wait=args.max_wait, quick_workers=args.quick_workers)
+                 max_wait=args.max_wait, quick_workers=args.quick_workers,
+                 blocking_waits=not args.spin_waits)
     for i, (g, f) in enumerate(zip(G, F)):
         assert srv.register(i, g, ldli=f)
     # request stream
@@ -567,9 +568,15 @@ def mode_serve(args) -> int:
     wrng = np.random.default_rng(55)
     srv.run([Request(-1 - i, 0.0, wrng.standard_normal(G[i].shape[0]), graph=i) for i in range(len(G))],
             realtime=False)
+    import resource
+
+    ru0 = resource.getrusage(resource.RUSAGE_SELF)
     t0 = time.perf_counter()
     res = srv.run(reqs, progress=lambda d, t, e: log(f"  served {d} of {t} requests at {e:.0f} s"))
     wall = time.perf_counter() - t0
+    ru1 = resource.getrusage(resource.RUSAGE_SELF)
+    cpu_s = (ru1.ru_utime - ru0.ru_utime) + (ru1.ru_stime - ru0.ru_stime)
+    log(f"host CPU during the run: {cpu_s:.1f} s over {wall:.1f} s wall ({cpu_s / wall:.1f} cores busy on average)")
     s = summary(res, wall)
     paths = {p: sum(o.path == p for o in res) for p in {o.path for o in res}}
     log(f"served: {s} paths {paths}")
@@ -587,7 +594,13 @@ def mode_serve(args) -> int:
         if len(lg):
             log(f"  new graphs, {name}: {len(lg)} x, median {np.median(lg) * 1e3:.1f} ms, "
                 f"max {lg.max() * 1e3:.1f} ms, sum {lg.sum():.1f} s")
-    sl = np.array(srv.solve_log[len(G):], dtype=float) if len(srv.solve_log) > len(G) else np.zeros((0, 4))
+    fc = np.array(srv.factor_cpu_log)
+    if len(fc):
+        log(f"  new graphs, CPU of the calling thread: factor {fc.sum():.1f} s, "
+            f"context upload {np.sum(srv.upload_cpu_log):.1f} s (planner helper threads not counted)")
+    sl = np.array(srv.solve_log[len(G):], dtype=float) if len(srv.solve_log) > len(G) else np.zeros((0, 5))
+    if len(sl):
+        log(f"  solve workers: {sl[:, 2].sum():.1f} s waiting in solves, {sl[:, 4].sum():.1f} s of it on the CPU")
     for lo_n, hi_n in ((0, 1e5), (1e5, 1e6), (1e6, 1e8)):
         m = (sl[:, 0] >= lo_n) & (sl[:, 0] < hi_n) if len(sl) else np.zeros(0, bool)
         if m.any():
@@ -678,6 +691,8 @@ def main() -> int:
     ap.add_argument("--new-frac", type=float, default=0.2)
     ap.add_argument("--serve-workers", type=int, default=8, help="serving: graphs solved concurrently")
     ap.add_argument("--factor-workers", type=int, default=8, help="serving: host threads factoring new graphs")
+    ap.add_argument("--spin-waits", action="store_true",
+                    help="serving: solve workers spin while waiting for the device (default: they sleep)")
     ap.add_argument("--quick-workers", type=int, default=4,
                     help="serving: workers for solves of graphs with n <= 3e5 (0: share the others)")
     ap.add_argument("--batch-window", type=float, default=2.0, help="serving: wait for more requests, x last solve time")

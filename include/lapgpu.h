@@ -134,6 +134,17 @@ int lapgpu_spmv(lapgpu_ctx* ctx, const double* x, double* y, int64_t n);
 /* Select the precision mode for later solves (LAPGPU_FP64 by default). */
 int lapgpu_set_precision(lapgpu_ctx* ctx, int mode, int replace_every);
 
+/* How the calling thread waits for the device inside a solve (no reference
+ * counterpart).  LAPGPU_WAIT_SPIN (default): HIP's polling wait, the lowest
+ * latency, one busy core per waiting thread.  LAPGPU_WAIT_BLOCKING: the
+ * thread queries the device's progress and sleeps 20 us between queries
+ * (a wait ends up to ~20 us late), for processes that run many solves at
+ * once on host threads and need the cores for other work (a serving
+ * process factoring new graphs). */
+#define LAPGPU_WAIT_SPIN 0
+#define LAPGPU_WAIT_BLOCKING 1
+int lapgpu_set_wait_mode(lapgpu_ctx* ctx, int mode);
+
 /* Level-group budget of the LDLinv schedule (no reference counterpart; the
  * reference has no device schedule).  The levels before the tail are run as
  * groups, one launch per group and sweep; a group spans at most max_depth

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <array>
 #include <chrono>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -164,6 +165,9 @@ struct lapgpu_ctx {
   int32_t* hflags = nullptr;      // pinned ring of stop flags
   static constexpr int kRing = 4;
   hipEvent_t ring_ev[kRing] = {};
+  // waits on the device: spin (false) or sleep on blocking-sync events
+  bool blocking = false;
+  hipEvent_t sync_ev = nullptr;
 
   // multi-RHS workspace (lapgpu_solve_multi): K interleaved right-hand
   // sides of length n; the tail CSRs are built on first use
@@ -211,6 +215,25 @@ int set_err(lapgpu_ctx* c, int code, const std::string& msg) {
       return set_err(ctx, -10, std::string(#call) + ": " +                   \
                                    hipGetErrorString(e_));                   \
   } while (0)
+
+// Wait for an event: HIP's own wait, or, in the blocking wait mode, queries
+// with 20 us sleeps in between (the thread leaves its core to other work;
+// a wait ends at most ~20 us after the event).
+hipError_t wait_event(lapgpu_ctx* c, hipEvent_t ev) {
+  if (!c->blocking) return hipEventSynchronize(ev);
+  for (;;) {
+    const hipError_t e = hipEventQuery(ev);
+    if (e != hipErrorNotReady) return e;
+    std::this_thread::sleep_for(std::chrono::microseconds(20));
+  }
+}
+
+// Wait for everything queued on the context's stream.
+hipError_t sync_stream(lapgpu_ctx* c) {
+  if (!c->blocking) return hipStreamSynchronize(c->stream);
+  hipError_t e = hipEventRecord(c->sync_ev, c->stream);
+  return e != hipSuccess ? e : wait_event(c, c->sync_ev);
+}
 
 // Device buffers are stream-ordered on the context's stream (a free waits
 // for the context's own work only, never for the whole device).
@@ -715,7 +738,7 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
 
   // the setup may already have stopped (b == 0, maxits == 0)
   HIPC(c, hipMemcpyAsync(c->hst, c->st, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, sync_stream(c));
   if (verbose) {
     printf("<b, LDLi b> is %.17g\n", c->hst->nb == 0.0 ? 0.0 : c->hst->rho0);
     fflush(stdout);
@@ -778,7 +801,7 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
     // keep one iteration in flight: wait for the flags of all older ones
     while (!done && polled < enq - 1) {
       const int ps = (int)(polled % lapgpu_ctx::kRing);
-      HIPC(c, hipEventSynchronize(c->ring_ev[ps]));
+      HIPC(c, wait_event(c, c->ring_ev[ps]));
       ++polled;
       if (c->hflags[ps]) done = true;
     }
@@ -788,7 +811,7 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
   hipLaunchKernelGGL(k_scatter_perm, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
                      c->perm, c->bestx, d_x);
   HIPC(c, hipMemcpyAsync(c->hst, c->st, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, sync_stream(c));
   HIPC(c, hipGetLastError());
   const DevState& h = *c->hst;
   const double secs =
@@ -1072,7 +1095,7 @@ int solve_multi_chunk(lapgpu_ctx* c, const double* d_B, double* d_X, int k, doub
   HIPC(c, hipGetLastError());
   HIPC(c, hipMemcpyAsync(&c->hflags[0], m.flags, sizeof(int32_t), hipMemcpyDeviceToHost,
                          c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, sync_stream(c));
   bool done = c->hflags[0] != 0;
 
   const int rep_every = f32 ? c->replace_every : 0;
@@ -1130,7 +1153,7 @@ int solve_multi_chunk(lapgpu_ctx* c, const double* d_B, double* d_X, int k, doub
     }
     while (!done && polled < enq - 1) {
       const int ps = (int)(polled % lapgpu_ctx::kRing);
-      HIPC(c, hipEventSynchronize(c->ring_ev[ps]));
+      HIPC(c, wait_event(c, c->ring_ev[ps]));
       ++polled;
       if (c->hflags[ps]) done = true;
     }
@@ -1138,7 +1161,7 @@ int solve_multi_chunk(lapgpu_ctx* c, const double* d_B, double* d_X, int k, doub
   hipLaunchKernelGGL(k_mfinal<K>, dim3(kMRed), dim3(kBlock), 0, c->stream, n, k, c->perm, m.x,
                      m.bestx, d_X, m.st);
   HIPC(c, hipMemcpyAsync(m.hst, m.st, sizeof(DevState) * K, hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, sync_stream(c));
   HIPC(c, hipGetLastError());
   for (int j = 0; j < k; ++j) {
     const DevState& h = m.hst[j];
@@ -1435,6 +1458,7 @@ void lapgpu_destroy(lapgpu_ctx* c) {
     if (ev) (void)hipEventDestroy(ev);
   for (auto& ev : c->ev)
     if (ev) (void)hipEventDestroy(ev);
+  if (c->sync_ev) (void)hipEventDestroy(c->sync_ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1564,7 +1588,7 @@ int lapgpu_solve(lapgpu_ctx* c, const double* b, double* x_out, int64_t n,
                         relres_out, stop_out);
   if (rc) return rc;
   HIPC(c, hipMemcpyAsync(x_out, c->io, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, sync_stream(c));
   return 0;
 }
 
@@ -1606,7 +1630,7 @@ int lapgpu_solve_multi(lapgpu_ctx* c, const double* B, double* X_out, int64_t n,
     if (rc) return rc;
     HIPC(c, hipMemcpyAsync(X_out + c0 * n, c->m.io, sizeof(double) * n * kc,
                            hipMemcpyDeviceToHost, c->stream));
-    HIPC(c, hipStreamSynchronize(c->stream));
+    HIPC(c, sync_stream(c));
   }
   c->err.clear();
   return 0;
@@ -1668,6 +1692,27 @@ int lapgpu_set_precision(lapgpu_ctx* c, int mode, int replace_every) {
     int rc = prepare_f32(c);
     if (rc) return rc;
   }
+  c->err.clear();
+  return 0;
+}
+
+int lapgpu_set_wait_mode(lapgpu_ctx* c, int mode) {
+  if (!c) return -1;
+  if (mode != LAPGPU_WAIT_SPIN && mode != LAPGPU_WAIT_BLOCKING)
+    return set_err(c, -1, "set_wait_mode: unknown mode");
+  HIPC(c, hipSetDevice(c->device));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  LAPGPU_LOCK();
+  const unsigned fl = hipEventDisableTiming;
+  for (auto& ev : c->ring_ev) {
+    if (ev) (void)hipEventDestroy(ev);
+    ev = nullptr;
+    HIPC(c, hipEventCreateWithFlags(&ev, fl));
+  }
+  if (c->sync_ev) (void)hipEventDestroy(c->sync_ev);
+  c->sync_ev = nullptr;
+  HIPC(c, hipEventCreateWithFlags(&c->sync_ev, fl));
+  c->blocking = mode == LAPGPU_WAIT_BLOCKING;
   c->err.clear();
   return 0;
 }

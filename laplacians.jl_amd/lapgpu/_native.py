@@ -31,6 +31,8 @@ VERBOSE = 1
 NO_CENTER = 2
 PROFILE = 4
 EAGER = 8
+WAIT_SPIN = 0
+WAIT_BLOCKING = 1
 TUNE_GROUPS = 1  # lapgpu_autotune: also tune the level-group budget
 FP64 = 0
 FP32_INNER = 1
@@ -48,6 +50,7 @@ EXPORTS = (
     "lapgpu_spmv",
     "lapgpu_set_precision",
     "lapgpu_set_group_budget",
+    "lapgpu_set_wait_mode",
     "lapgpu_factor_info",
     "lapgpu_last_timings",
     "lapgpu_last_sweep_timings",
@@ -99,6 +102,7 @@ def load() -> ctypes.CDLL:
     L.lapgpu_spmv.argtypes = [P, P, P, i64]
     L.lapgpu_set_precision.argtypes = [P, ci, ci]
     L.lapgpu_set_group_budget.argtypes = [P, ci, dbl, i64]
+    L.lapgpu_set_wait_mode.argtypes = [P, ci]
     L.lapgpu_factor_info.argtypes = [P, pi64, pi64, pi64, pi64]
     L.lapgpu_last_timings.argtypes = [P, pdbl, pdbl, pdbl, pdbl, pi64]
     L.lapgpu_last_sweep_timings.argtypes = [P, pdbl, pdbl, pdbl, pdbl]
@@ -358,6 +362,11 @@ class Context:
         upload_ldli; regroups an uploaded factor at once)."""
         self._check(self._lib.lapgpu_set_group_budget(self._h, int(max_depth), float(max_growth), int(max_extra)),
                     "lapgpu_set_group_budget")
+
+    def set_wait_mode(self, blocking: bool) -> None:
+        """Wait for the device by sleeping (blocking=True) or spinning."""
+        self._check(self._lib.lapgpu_set_wait_mode(self._h, WAIT_BLOCKING if blocking else WAIT_SPIN),
+                    "lapgpu_set_wait_mode")
 
     def factor_info(self) -> dict:
         a, b, c, d = (ctypes.c_int64() for _ in range(4))

@@ -111,7 +111,8 @@ class Server:
                  max_launch: int = 512, plain_workers: int = 4, factor_workers: int = 4,
                  keep_x: bool = False, max_rhs: int = 64, batch_window: float = 0.5,
                  max_wait: float = 1.0, new_graph_budget: tuple | None = (8, 2.0, 16384),
-                 eager_new: bool = True, quick_workers: int = 4, quick_max_n: int = 300_000):
+                 eager_new: bool = True, quick_workers: int = 4, quick_max_n: int = 300_000,
+                 blocking_waits: bool = True):
         self.device = device
         self.batch = _native.Batch(device)
         self.hbm_budget = hbm_budget if hbm_budget is not None else 200 * 2**30
@@ -130,13 +131,19 @@ class Server:
         # once recovers little of it (None: the library's default budget)
         self.new_graph_budget = new_graph_budget
         self.eager_new = eager_new
-        # (n, columns, seconds, first on its context) of each plain / multi-RHS solve
+        # solve workers sleep while the device works (lapgpu_set_wait_mode)
+        # instead of spinning, leaving the cores to factoring and planning
+        self.blocking_waits = blocking_waits
+        # (n, columns, seconds, first on its context, CPU seconds of the
+        # worker thread) of each plain / multi-RHS solve
         self.solve_log: list = []
         # seconds the dispatch loop spent admitting and dropping graphs
-        self.stats = {"admit_s": 0.0, "drop_s": 0.0, "admits": 0, "drops": 0}
+        self.stats = {"admit_s": 0.0, "drop_s": 0.0, "admits": 0, "drops": 0, "loop_cpu_s": 0.0}
         # worker seconds per new graph: factoring; context upload (total,
         # create, Laplacian, factor)
         self.factor_log: list = []
+        self.factor_cpu_log: list = []
+        self.upload_cpu_log: list = []
         self.upload_log: list = []
         self.graphs: "OrderedDict[object, _Graph]" = OrderedDict()
         self.hbm_used = 0
@@ -181,8 +188,11 @@ class Server:
         return need, k
 
     def _upload_context(self, la, ldli, budget=None):
+        c0 = time.thread_time()
         t0 = time.perf_counter()
         ctx = _native.Context(self.device)
+        if self.blocking_waits:
+            ctx.set_wait_mode(True)
         if budget is not None:
             ctx.set_group_budget(*budget)
         t1 = time.perf_counter()
@@ -191,6 +201,7 @@ class Server:
         ctx.upload_ldli(ldli)
         t3 = time.perf_counter()
         self.upload_log.append((t3 - t0, t1 - t0, t2 - t1, t3 - t2))
+        self.upload_cpu_log.append(time.thread_time() - c0)
         return ctx
 
     def _admit_wg(self, key, la, ldli, resident) -> _Graph:
@@ -251,6 +262,7 @@ class Server:
         waiting_admission: deque = deque()
         waiting: set = set()  # idle graphs collecting a batch
         t0 = time.perf_counter()
+        c0 = time.thread_time()
         nxt = 0
         total = len(reqs)
         next_progress = progress_every
@@ -284,6 +296,7 @@ class Server:
             # a graph that came with its request is solved without capturing
             # its iteration as a HIP graph (the capture would be used once)
             flags = _native.EAGER if (self.eager_new and not g.resident) else 0
+            c_start = time.thread_time()
             t_start = time.perf_counter()
             if len(take) == 1:
                 x, info = g.handle.solve(r0.b, r0.tol, r0.maxits, float("inf"), 5, flags)
@@ -291,7 +304,7 @@ class Server:
                 B = np.stack([r.b for r in take], axis=1)
                 x, info = g.handle.solve_multi(B, r0.tol, r0.maxits, float("inf"), 5, flags)
             dt = time.perf_counter() - t_start
-            self.solve_log.append((g.n, len(take), dt, g.solves == 0))
+            self.solve_log.append((g.n, len(take), dt, g.solves == 0, time.thread_time() - c_start))
             g.solves += 1
             g.last_solve_s = dt
             return take, x, info
@@ -407,13 +420,16 @@ class Server:
                 next_progress += progress_every
             if not progressed:
                 time.sleep(50e-6)
+        self.stats["loop_cpu_s"] += time.thread_time() - c0
         return [results[r.rid] for r in requests]
 
     def _factor(self, a):
+        c = time.thread_time()
         t = time.perf_counter()
         a = sp.csc_matrix(a)
         out = lap(a), build_ldli(a, seed=1)
         self.factor_log.append(time.perf_counter() - t)
+        self.factor_cpu_log.append(time.thread_time() - c)
         return out
 
     def close(self):

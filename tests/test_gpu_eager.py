@@ -64,3 +64,32 @@ def test_eager_multi_matches_captured(case, gpu_ctx_factory):
     Ye, je = ctx.solve_multi(B16, tol=1e-8, maxits=1000, flags=_native.EAGER)
     Yc, jc = ctx.solve_multi(B16, tol=1e-8, maxits=1000)
     _close(la, B16, Ye, Yc, je, jc)
+
+
+def test_blocking_wait_mode(case, gpu_ctx_factory, oracle_mod):
+    """lapgpu_set_wait_mode: solves that sleep on blocking-sync events give
+    the spin-wait solves' results (to rounding), single and multi-RHS, and
+    switching back works; an unknown mode is rejected."""
+    la, f = case
+    n = la.shape[0]
+    b = _rhs(n, 1, 4)[:, 0]
+    B = _rhs(n, 8, 5)
+    ctx = gpu_ctx_factory(la, f)
+    xs, is_ = ctx.solve(b, tol=1e-8, maxits=1000)
+    Xs, js = ctx.solve_multi(B, tol=1e-8, maxits=1000)
+    ctx.set_wait_mode(True)
+    xb, ib = ctx.solve(b, tol=1e-8, maxits=1000)
+    xb2, ib2 = ctx.solve(b, tol=1e-8, maxits=1000, flags=_native.EAGER)
+    Xb, jb = ctx.solve_multi(B, tol=1e-8, maxits=1000)
+    for x, i in ((xb, ib), (xb2, ib2)):
+        assert i["stop"] == is_["stop"] and abs(i["its"] - is_["its"]) <= 1
+        assert np.linalg.norm(la @ x - b) <= 1e-7 * np.linalg.norm(b)
+        assert np.linalg.norm(x - xs) <= 1e-6 * np.linalg.norm(xs)
+    _close(la, B, Xb, Xs, jb, js)
+    _, io = oracle_mod.approxchol_solve(la, f, b, tol=1e-8, maxits=1000, stag_test=5)
+    assert ib["stop"] == io["stop"] and abs(ib["its"] - io["its"]) <= 2
+    ctx.set_wait_mode(False)
+    xs2, _ = ctx.solve(b, tol=1e-8, maxits=1000)
+    assert np.linalg.norm(xs2 - xs) <= 1e-6 * np.linalg.norm(xs)
+    with pytest.raises(_native.LapGPUError):
+        ctx._check(ctx._lib.lapgpu_set_wait_mode(ctx._h, 7), "lapgpu_set_wait_mode")
