@@ -682,7 +682,7 @@ def main() -> int:
     ap.add_argument("--cpu-its", type=int, default=0, help="oracle iterations for the CPU baseline sample")
     ap.add_argument("--rhs", type=int, default=64)
     ap.add_argument("--small", type=int, default=256)
-    ap.add_argument("--requests", type=int, default=1000)
+    ap.add_argument("--requests", type=int, default=10000)
     ap.add_argument("--qps", type=float, default=200.0)
     ap.add_argument("--serve-graphs", type=int, default=32)
     ap.add_argument("--serve-min-n", type=float, default=1e4)
