@@ -376,10 +376,16 @@ __device__ __forceinline__ void run_program(const TailStepD* __restrict__ steps,
 }
 
 // Steps per register set for E entries per lane (2 D E slots per lane in
-// registers).
+// registers).  Measured on MI355X with 512 tail threads and E = 4 (grid3(128)
+// tail per iteration; path + 4 matchings n = 1e6 FP64 tail): depth 4 (D = 1)
+// 0.96 / 5.55 ms, 8 (D = 2) 0.94 / 5.73 ms, 16 (D = 4) 1.10 / 6.64 ms; the
+// tail's steps are not bound by their slot loads.
+#ifndef LAPGPU_TAIL_DEPTH
+#define LAPGPU_TAIL_DEPTH 8
+#endif
 template <int E>
 __host__ __device__ constexpr int tail_depth() {
-  return (kTailThreads <= 256 ? 16 : 8) / E;
+  return (kTailThreads <= 256 ? 16 : LAPGPU_TAIL_DEPTH) / E;
 }
 
 }  // namespace lapgpu
