@@ -210,6 +210,10 @@ int64_t row_lanes(int64_t dg, int elog) {
 // Entries per lane of a program from the mean length of its non-empty
 // rows: more entries per lane for longer rows, so that a long row takes
 // fewer lanes and fewer cross-lane reduction stages.
+// Entries per lane from the mean row length.  Eight entries per lane
+// (fewer segment sums for long rows) measured slower on MI355X: the tail of
+// grid3(128) (81 entries per row) 0.94 -> 1.11 ms per iteration, that of
+// path + 4 matchings n = 1e6 (495 per row) 5.73 -> 6.23 ms.
 int choose_elog(int64_t rows, int64_t ent, int max_elog) {
   if (rows == 0) return 0;
   const double avg = (double)ent / (double)rows;

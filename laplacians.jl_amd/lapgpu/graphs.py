@@ -134,6 +134,19 @@ def path_matchings(n: int, k: int, rng: np.random.Generator, weights: str = "log
     return from_edges(n, lo, hi, _weights(len(lo), rng, weights))
 
 
+def random_dense(n: int, deg: int, rng: np.random.Generator) -> sp.csc_matrix:
+    """About n * deg / 2 uniformly random edges (self-pairs dropped, repeats
+    summed) plus the path 0-1-...-(n-1), which keeps it connected; weights
+    U(0.5, 2).  Its factor's last columns are long (tens to hundreds of
+    entries), the case of long tail rows."""
+    m = n * deg // 2
+    i = rng.integers(0, n, m)
+    j = rng.integers(0, n, m)
+    keep = i != j
+    a = from_edges(n, i[keep], j[keep], rng.uniform(0.5, 2.0, int(keep.sum())))
+    return sp.csc_matrix(a + path_graph(n))
+
+
 def random_tree(n: int, rng: np.random.Generator, weighted: bool = True) -> sp.csc_matrix:
     """Random recursive tree: vertex v > 0 attaches to a uniform earlier vertex."""
     v = np.arange(1, n)

@@ -62,7 +62,7 @@ int main(int argc, char** argv) {
   H(s); H(lvl); H(blocks.lev); H(blocks.fptr); H(blocks.fidx); H(blocks.fw);
   H(blocks.bptr); H(blocks.bidx); H(blocks.bw); H(tail.fsteps); H(tail.fslots);
   H(tail.bsteps); H(tail.bslots); H(tail.Eptr); H(tail.Eidx); H(tail.Ew);
-  printf("levels %lld groups %zu tail_vertices %lld\n", (long long)P.nlev,
-         P.blocks.lev.size() - 1, (long long)P.tail.NT);
+  printf("levels %lld groups %zu tail_vertices %lld elog %d\n", (long long)P.nlev,
+         P.blocks.lev.size() - 1, (long long)P.tail.NT, P.tail.elog);
   return 0;
 }

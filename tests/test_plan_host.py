@@ -54,3 +54,17 @@ def test_plan_same_for_any_thread_count(driver, tmp_path, name):
     assert int(last[1]) > 1 and int(last[3]) > 0
     if name != "path_matchings":
         assert int(last[3]) > 1, "expected several level groups"
+
+
+def test_entries_per_lane_follow_row_length(driver, tmp_path):
+    """The tail program packs 2^elog entries per lane, chosen from its mean
+    row length: 4 for long rows (a dense random graph, ~150 entries per
+    row), 1 for a 2-D grid (~4)."""
+    rng = np.random.default_rng(12)
+    for name, a, want in (("dense", graphs.random_dense(1500, 120, rng), 2),
+                          ("grid2", graphs.weighted_grid2(60, 50, rng, "expU2"), 0)):
+        d = str(tmp_path / name)
+        _dump(a, d)
+        r = subprocess.run([driver, d], capture_output=True, text=True, check=True)
+        elog = int(r.stdout.strip().splitlines()[-1].split()[7])
+        assert elog == want, name
