@@ -140,7 +140,7 @@ int lapgpu_set_precision(lapgpu_ctx* ctx, int mode, int replace_every);
  * thread queries the device's progress and sleeps 20 us between queries
  * (a wait ends up to ~20 us late), for processes that run many solves at
  * once on host threads and need the cores for other work (a serving
- * process factoring new graphs). */
+ * process factoring new graphs).  Call it between solves on the context. */
 #define LAPGPU_WAIT_SPIN 0
 #define LAPGPU_WAIT_BLOCKING 1
 int lapgpu_set_wait_mode(lapgpu_ctx* ctx, int mode);
