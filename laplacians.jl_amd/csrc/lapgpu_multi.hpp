@@ -429,10 +429,19 @@ k_mvsum(int n, const T* __restrict__ z, double* __restrict__ part,
         const int* __restrict__ flags) {
   if (flags[0]) return;
   const int64_t nK = (int64_t)n * K;
+  const int64_t S = (int64_t)gridDim.x * kBlock;
   double a[1] = {0.0};
-  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nK;
-       e += (int64_t)gridDim.x * kBlock)
-    a[0] += (double)z[e];
+  int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  // four grid strides per pass with their loads issued together; the sums
+  // run in the same order as one stride at a time
+  for (; e + 3 * S < nK; e += 4 * S) {
+    T v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = z[e + u * S];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a[0] += (double)v[u];
+  }
+  for (; e < nK; e += S) a[0] += (double)z[e];
   const int slots[1] = {S_SUM};
   mblock_sum<K, 1>(a, part, slots);
 }
@@ -456,16 +465,28 @@ k_mzdot(int n, const T* __restrict__ z, const double* __restrict__ r,
   const int j = threadIdx.x % K;
   const double mu = mst[j].mu;
   const int64_t nK = (int64_t)n * K;
+  const int64_t S = (int64_t)gridDim.x * kBlock;
   double a[3] = {0.0, 0.0, 0.0};
-  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nK;
-       e += (int64_t)gridDim.x * kBlock) {
-    const double t = (double)z[e] - mu;
-    a[0] += t * r[e];
+  auto one = [&](double zv, double rv, int64_t e) {
+    const double t = zv - mu;
+    a[0] += t * rv;
     if (chk) {
       a[1] += fabs(t);
       a[2] += fabs(p[e]);
     }
+  };
+  int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  for (; e + 3 * S < nK; e += 4 * S) {
+    double zv[4], rv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      zv[u] = (double)z[e + u * S];
+      rv[u] = r[e + u * S];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) one(zv[u], rv[u], e + u * S);
   }
+  for (; e < nK; e += S) one((double)z[e], r[e], e);
   const int slots[3] = {S_ZR, S_AZ, S_AP2};
   mblock_sum<K, 3>(a, part, slots);
 }
@@ -672,9 +693,7 @@ k_mupdate(int n, double* __restrict__ x, double* __restrict__ bestx,
   const int64_t nK = (int64_t)n * K;
   double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (run) {
-    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nK;
-         e += (int64_t)gridDim.x * kBlock) {
-      const double xo = x[e], pi = p[e], qi = q[e], ro = r[e];
+    auto one = [&](int64_t e, double xo, double pi, double qi, double ro) {
       if (cp) bestx[e] = xo;
       x[e] = xo + al * pi;
       const double rn = ro + (-al) * qi;
@@ -686,7 +705,24 @@ k_mupdate(int n, double* __restrict__ x, double* __restrict__ bestx,
         a[3] += fabs(qi);
         a[4] += fabs(ro);
       }
+    };
+    // four grid strides per pass with their loads issued together; the
+    // sums run in the same order as one stride at a time
+    const int64_t S = (int64_t)gridDim.x * kBlock;
+    int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    for (; e + 3 * S < nK; e += 4 * S) {
+      double xo[4], pi[4], qi[4], ro[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        xo[u] = x[e + u * S];
+        pi[u] = p[e + u * S];
+        qi[u] = q[e + u * S];
+        ro[u] = r[e + u * S];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) one(e + u * S, xo[u], pi[u], qi[u], ro[u]);
     }
+    for (; e < nK; e += S) one(e, x[e], p[e], q[e], r[e]);
   }
   const int slots[5] = {S_RR, S_AP, S_AX, S_AQ, S_AR};
   mblock_sum<K, 5>(a, part, slots);
@@ -790,8 +826,23 @@ k_mpupdate(int n, const T* __restrict__ z, double* __restrict__ p, float* __rest
   if (mst[j].stopped) return;
   const double mu = mst[j].mu, beta = mst[j].beta;
   const int64_t nK = (int64_t)n * K;
-  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nK;
-       e += (int64_t)gridDim.x * kBlock) {
+  const int64_t S = (int64_t)gridDim.x * kBlock;
+  int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  for (; e + 3 * S < nK; e += 4 * S) {
+    double zv[4], pv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      zv[u] = (double)z[e + u * S];
+      pv[u] = p[e + u * S];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double t = (zv[u] - mu) + beta * pv[u];
+      p[e + u * S] = t;
+      if (pf) pf[e + u * S] = (float)t;
+    }
+  }
+  for (; e < nK; e += S) {
     const double t = ((double)z[e] - mu) + beta * p[e];
     p[e] = t;
     if (pf) pf[e] = (float)t;
