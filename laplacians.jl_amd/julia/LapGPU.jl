@@ -164,4 +164,51 @@ function harmonic_interp_gpu(adj_mat, S::Vector, vals::Vector; tol = 1e-6)
     return x
 end
 
+# ---- entry points beyond the reference (include/lapgpu.h) ----------------
+
+# k right-hand sides (the columns of B) in every launch, each with the
+# closure's stopping rules; returns X and per-column iteration counts
+function solve_multi(ctx::Ctx, B::Matrix{Float64}; tol = 1e-6, maxits = 1000,
+                     maxtime = Inf, stag_test = 5, flags = 0)
+    n, k = size(B)
+    X = zeros(n, k)
+    its = zeros(Int64, k)
+    nr = zeros(Float64, k)
+    stop = zeros(Cint, k)
+    mi = isinf(maxits) ? Int64(-1) : Int64(maxits)
+    check(ctx, ccall((:lapgpu_solve_multi, libpath), Cint,
+                     (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Int64, Float64, Int64,
+                      Float64, Cint, Cint, Ptr{Int64}, Ptr{Float64}, Ptr{Cint}),
+                     ctx.h, B, X, n, k, tol, mi, Float64(maxtime), stag_test, flags, its, nr,
+                     stop), "lapgpu_solve_multi")
+    return X, its
+end
+
+# FP32 SpMV and LDLinv apply with FP64 dots, alpha/beta and a residual
+# replacement every `replace_every` iterations (mode 1), or FP64 (mode 0)
+set_precision!(ctx::Ctx, mode::Integer, replace_every::Integer = 20) =
+    check(ctx, ccall((:lapgpu_set_precision, libpath), Cint, (Ptr{Cvoid}, Cint, Cint),
+                     ctx.h, mode, replace_every), "lapgpu_set_precision")
+
+# per-graph schedule autotuning (k > 1: also the k-column multi-RHS kernels;
+# groups: also the level-group budget); returns the apply times before and
+# after (ms)
+function autotune!(ctx::Ctx; k::Integer = 0, groups::Bool = false)
+    t = zeros(Float64, 4)
+    check(ctx, ccall((:lapgpu_autotune, libpath), Cint, (Ptr{Cvoid}, Int64, Cint, Ptr{Float64}),
+                     ctx.h, k, groups ? 1 : 0, t), "lapgpu_autotune")
+    return t
+end
+
+# level-group budget of the schedule (regroups an uploaded factor)
+set_group_budget!(ctx::Ctx, depth::Integer, growth::Real, extra::Integer) =
+    check(ctx, ccall((:lapgpu_set_group_budget, libpath), Cint,
+                     (Ptr{Cvoid}, Cint, Float64, Int64), ctx.h, depth, growth, extra),
+          "lapgpu_set_group_budget")
+
+# the solving thread sleeps between event queries (blocking = true)
+set_wait_mode!(ctx::Ctx, blocking::Bool) =
+    check(ctx, ccall((:lapgpu_set_wait_mode, libpath), Cint, (Ptr{Cvoid}, Cint), ctx.h,
+                     blocking ? 1 : 0), "lapgpu_set_wait_mode")
+
 end # module
