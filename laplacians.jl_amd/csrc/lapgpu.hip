@@ -29,6 +29,7 @@
 using namespace lapgpu;
 
 static_assert(kTailThreads == kTailLanes, "tail workgroup width = lanes of a tail step");
+static_assert(kTailCapLds == kTailCap, "tail capacity: planner and kernels agree");
 
 namespace lapgpu {
 

@@ -193,6 +193,11 @@ __device__ __forceinline__ double wave_total(const double* part, int slot) {
 // largest segment bounds the DPP stages run.
 
 constexpr int kTailCapV = 18432;
+// vertices of the single-workgroup tail of a context (= the planner's kTailCap)
+#ifndef LAPGPU_TAIL_CAP
+#define LAPGPU_TAIL_CAP 18432
+#endif
+constexpr int kTailCapLds = LAPGPU_TAIL_CAP;
 
 struct alignas(16) TailSlotD {
   double w;

@@ -273,7 +273,7 @@ k_tail(int T0, int NT, const T* __restrict__ ext, const T* __restrict__ s,
        const S* __restrict__ fslots, const TailStepD* __restrict__ bsteps,
        int nbsteps, const S* __restrict__ bslots,
        const DevState* __restrict__ st) {
-  __shared__ T ly[kTailCapV];
+  __shared__ T ly[kTailCapLds];
   if (st->stopped) return;
   for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] = ext[i];
   __syncthreads();

@@ -408,7 +408,7 @@ k_mtail(int T0, int NT, const T* __restrict__ ext, const T* __restrict__ s, T* _
         const TailStepD* __restrict__ fsteps, int nfsteps, const S* __restrict__ fslots,
         const TailStepD* __restrict__ bsteps, int nbsteps, const S* __restrict__ bslots,
         const DevState* __restrict__ mst, const int* __restrict__ flags) {
-  __shared__ T ly[kTailCapV];
+  __shared__ T ly[kTailCapLds];
   const int j = blockIdx.x;
   if (flags[0] || mst[j].stopped) return;
   for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] = ext[(int64_t)i * K + j];

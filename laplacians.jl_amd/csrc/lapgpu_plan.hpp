@@ -58,7 +58,10 @@ inline int32_t step_meta(int64_t count, bool end, int elog, int maxlog) {
 constexpr int kTailLanes = LAPGPU_TAIL_THREADS;
 // Lanes per step the extent rule of build_tail counts steps in.
 constexpr int kExtentLanes = 1024;
-constexpr int64_t kTailCap = 18432;  // tail vertices held in LDS (147 KiB)
+#ifndef LAPGPU_TAIL_CAP
+#define LAPGPU_TAIL_CAP 18432
+#endif
+constexpr int64_t kTailCap = LAPGPU_TAIL_CAP;  // tail vertices held in LDS (147 KiB)
 
 struct TailProgram {
   int64_t T0 = 0, NT = 0;
