@@ -255,43 +255,50 @@ void pack_level(std::vector<RowEntries>& rows, int elog, int64_t lanes,
   }
   std::stable_sort(chunks.begin(), chunks.end(),
                    [](const Chunk& a, const Chunk& b) { return a.size > b.size; });
-  // pass p: E planes of kTailLanes slots, plane-major
-  std::vector<std::vector<TailSlot>> pass;
-  std::vector<int64_t> ptr;
+  // placement: pass (step) p and lane offset of every chunk, bump pointers
+  std::vector<int64_t> ptr, at(chunks.size());
   std::vector<int> maxlog;
-  for (const Chunk& C : chunks) {
+  std::vector<int32_t> in(chunks.size());
+  for (size_t c = 0; c < chunks.size(); ++c) {
+    const Chunk& C = chunks[c];
     size_t p = 0;
-    while (p < pass.size() && ptr[p] + C.size > lanes) ++p;
-    if (p == pass.size()) {
-      pass.emplace_back(E * lanes, TailSlot{0.0, 0, 0});
+    while (p < ptr.size() && ptr[p] + C.size > lanes) ++p;
+    if (p == ptr.size()) {
       ptr.push_back(0);
       maxlog.push_back(0);
     }
-    const int64_t off = ptr[p];
+    in[c] = (int32_t)p;
+    at[c] = ptr[p];
     ptr[p] += C.size;
+    maxlog[p] = std::max(maxlog[p], log2_exact(C.size));
+  }
+  // step p: E planes of cnt_p slots, plane-major, written in place (unused
+  // slots stay {0, 0, 0})
+  std::vector<int64_t> base(ptr.size()), cnt(ptr.size());
+  for (size_t p = 0; p < ptr.size(); ++p) {
+    cnt[p] = (ptr[p] + 63) / 64 * 64;
+    base[p] = (int64_t)slots.size();
+    steps.push_back(TailStep{(int32_t)base[p],
+                             step_meta(cnt[p], p + 1 == ptr.size(), elog, maxlog[p])});
+    slots.resize(slots.size() + E * cnt[p], TailSlot{0.0, 0, 0});
+  }
+  for (size_t c = 0; c < chunks.size(); ++c) {
+    const Chunk& C = chunks[c];
+    const int64_t p = in[c], off = at[c];
     const int lg = log2_exact(C.size);
-    maxlog[p] = std::max(maxlog[p], lg);
+    TailSlot* S0 = slots.data() + base[p];
     for (int64_t t = 0; t < C.size; ++t) {
-      TailSlot& M = pass[p][off + t];
-      M.meta = slot_meta((uint32_t)C.row, (uint32_t)lg,
-                         t == C.size - 1 ? (C.atomic ? 2u : 1u) : 0u);
+      S0[off + t].meta = slot_meta((uint32_t)C.row, (uint32_t)lg,
+                                   t == C.size - 1 ? (C.atomic ? 2u : 1u) : 0u);
       for (int64_t e = 0; e < E; ++e) {
         const int64_t k = C.k0 + t * E + e;
         if (k < C.k1) {
-          TailSlot& S = pass[p][e * lanes + off + t];
+          TailSlot& S = S0[e * cnt[p] + off + t];
           S.src = (uint32_t)C.R->e[k].first;
           S.w = C.R->e[k].second;
         }
       }
     }
-  }
-  for (size_t p = 0; p < pass.size(); ++p) {
-    const int64_t cnt = (ptr[p] + 63) / 64 * 64;
-    steps.push_back(TailStep{(int32_t)slots.size(),
-                             step_meta(cnt, p + 1 == pass.size(), elog, maxlog[p])});
-    for (int64_t e = 0; e < E; ++e)
-      slots.insert(slots.end(), pass[p].begin() + e * lanes,
-                   pass[p].begin() + e * lanes + cnt);
   }
 }
 
