@@ -546,7 +546,7 @@ def mode_serve(args) -> int:
     srv = Server(0, hbm_budget=200 * 2**30, plain_workers=args.serve_workers,
                  factor_workers=args.factor_workers, batch_window=args.batch_window,
                  max_wait=args.max_wait, quick_workers=args.quick_workers,
-                 blocking_waits=not args.spin_waits)
+                 blocking_waits=not args.spin_waits, wg_max_n=args.wg_max_n)
     for i, (g, f) in enumerate(zip(G, F)):
         assert srv.register(i, g, ldli=f)
     # request stream
@@ -691,6 +691,8 @@ def main() -> int:
     ap.add_argument("--new-frac", type=float, default=0.2)
     ap.add_argument("--serve-workers", type=int, default=8, help="serving: graphs solved concurrently")
     ap.add_argument("--factor-workers", type=int, default=8, help="serving: host threads factoring new graphs")
+    ap.add_argument("--wg-max-n", type=int, default=50000,
+                    help="serving: graphs up to this size are solved by the batched engine")
     ap.add_argument("--spin-waits", action="store_true",
                     help="serving: solve workers spin while waiting for the device (default: they sleep)")
     ap.add_argument("--quick-workers", type=int, default=4,

@@ -107,7 +107,7 @@ def multi_bytes_estimate(n: int, k: int) -> int:
 
 
 class Server:
-    def __init__(self, device: int = 0, hbm_budget: int | None = None, wg_max_n: int = 20_000,
+    def __init__(self, device: int = 0, hbm_budget: int | None = None, wg_max_n: int = 50_000,
                  max_launch: int = 512, plain_workers: int = 4, factor_workers: int = 4,
                  keep_x: bool = False, max_rhs: int = 64, batch_window: float = 0.5,
                  max_wait: float = 1.0, new_graph_budget: tuple | None = (8, 2.0, 16384),
