@@ -223,7 +223,12 @@ void lapgpu_destroy(lapgpu_ctx* ctx);
  * hold many graphs or many right-hand sides (blockSolver's components,
  * src/solverInterface.jl:152-188; serving).  Each request runs the complete
  * pcg of src/pcg.jl:256-445 with LDLsolver in one workgroup, all requests in
- * one launch. */
+ * one launch.  A batch may be used from several threads: every entry point
+ * holds the batch's lock, except that lapgpu_batch_add_graph plans the graph
+ * on the calling thread before it takes the lock (to upload and register
+ * it), so graphs can be added while other threads submit and collect.
+ * lapgpu_batch_last_error reports the calling thread's last failure only
+ * when no other thread fails meanwhile. */
 typedef struct lapgpu_batch lapgpu_batch;
 
 int lapgpu_batch_create(lapgpu_batch** out, int device);

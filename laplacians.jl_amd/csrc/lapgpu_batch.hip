@@ -12,6 +12,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -70,6 +71,9 @@ struct BatchSlot {
 };
 
 struct lapgpu_batch {
+  // every entry point holds it, except add_graph while it plans on the host
+  // (so graphs can be planned on other threads while launches go on)
+  std::recursive_mutex mu;
   int device = 0;
   hipStream_t stream = nullptr;  // uploads and the synchronous entry points
   double wallclock_khz = 100000.0;
@@ -90,6 +94,11 @@ int berr(lapgpu_batch* B, int code, const std::string& m) {
     g_batch_err = m;
   return code;
 }
+
+// the batch's lock for the rest of the calling entry point (none for a null B)
+#define BATCH_LOCK(B)                                      \
+  std::unique_lock<std::recursive_mutex> batch_lk_;        \
+  if (B) batch_lk_ = std::unique_lock<std::recursive_mutex>((B)->mu)
 
 #define BHIP(B, call)                                                          \
   do {                                                                         \
@@ -223,21 +232,24 @@ int lapgpu_batch_add_graph(lapgpu_batch* B, int64_t n, const int64_t* colptr,
                            const double* fval, const double* d, int index_base,
                            int32_t* graph_id) {
   if (!B || !graph_id) return berr(B, -1, "lapgpu_batch_add_graph: null argument");
-  BHIP(B, hipSetDevice(B->device));
   std::string err;
   HostCsr L;
-  if (!csc_to_csr(n, colptr, rowval, nzval, index_base, L, err)) return berr(B, -1, err);
   Plan P;
-  if (!build_plan(L, nnzF, col, fcolptr, frowval, fval, d, index_base, 0, P, err))
+  if (!csc_to_csr(n, colptr, rowval, nzval, index_base, L, err) ||
+      !build_plan(L, nnzF, col, fcolptr, frowval, fval, d, index_base, 0, P, err)) {
+    std::lock_guard<std::recursive_mutex> lk(B->mu);
     return berr(B, -1, err);
-  if (!build_tail(P, INT64_MAX, INT64_MAX, kWgElog, kWgThreads) || P.tail.T0 != 0)
-    return berr(B, -1, "lapgpu_batch_add_graph: could not build the slot program");
+  }
+  const bool planned = build_tail(P, INT64_MAX, INT64_MAX, kWgElog, kWgThreads) && P.tail.T0 == 0;
+  std::lock_guard<std::recursive_mutex> lk(B->mu);
+  if (!planned) return berr(B, -1, "lapgpu_batch_add_graph: could not build the slot program");
   GraphStore G;
   G.n = (int)n;
   G.nf = (int)P.tail.fsteps.size();
   G.nb = (int)P.tail.bsteps.size();
   static_assert(sizeof(TailStepD) == sizeof(TailStep), "step layout");
   static_assert(sizeof(TailSlotD) == sizeof(TailSlot), "slot layout");
+  BHIP(B, hipSetDevice(B->device));
   int rc = 0;
   if ((rc = up(B, &G.perm, P.perm.data(), P.perm.size(), G.bytes)) ||
       (rc = up(B, &G.Lptr, P.Lptr.data(), P.Lptr.size(), G.bytes)) ||
@@ -270,6 +282,7 @@ int lapgpu_batch_add_graph(lapgpu_batch* B, int64_t n, const int64_t* colptr,
 }
 
 int lapgpu_batch_remove_graph(lapgpu_batch* B, int32_t graph_id) {
+  BATCH_LOCK(B);
   if (!B || graph_id < 0 || graph_id >= (int32_t)B->graphs.size() || !B->graphs[graph_id].live)
     return berr(B, -1, "lapgpu_batch_remove_graph: unknown graph");
   BHIP(B, hipSetDevice(B->device));
@@ -284,6 +297,7 @@ int lapgpu_batch_remove_graph(lapgpu_batch* B, int32_t graph_id) {
 
 int lapgpu_batch_graph_info(lapgpu_batch* B, int32_t graph_id, int64_t* n,
                             int64_t* device_bytes) {
+  BATCH_LOCK(B);
   if (!B || graph_id < 0 || graph_id >= (int32_t)B->graphs.size() || !B->graphs[graph_id].live)
     return berr(B, -1, "lapgpu_batch_graph_info: unknown graph");
   if (n) *n = B->graphs[graph_id].n;
@@ -382,6 +396,7 @@ int lapgpu_batch_submit(lapgpu_batch* B, int64_t nreq, const int32_t* graph_ids,
                         const double* const* d_b, double* const* d_x, double tol,
                         int64_t maxits, double maxtime, int stag_test, int flags,
                         int32_t* slot_out) {
+  BATCH_LOCK(B);
   if (!B || nreq <= 0 || !graph_ids || !d_b || !d_x || !slot_out)
     return berr(B, -1, "lapgpu_batch_submit: bad arguments");
   BHIP(B, hipSetDevice(B->device));
@@ -401,6 +416,7 @@ int lapgpu_batch_submit(lapgpu_batch* B, int64_t nreq, const int32_t* graph_ids,
 int lapgpu_batch_submit_host(lapgpu_batch* B, int64_t nreq, const int32_t* graph_ids,
                              const double* b_concat, double tol, int64_t maxits,
                              double maxtime, int stag_test, int flags, int32_t* slot_out) {
+  BATCH_LOCK(B);
   if (!B || nreq <= 0 || !graph_ids || !b_concat || !slot_out)
     return berr(B, -1, "lapgpu_batch_submit_host: bad arguments");
   BHIP(B, hipSetDevice(B->device));
@@ -437,6 +453,7 @@ int lapgpu_batch_submit_host(lapgpu_batch* B, int64_t nreq, const int32_t* graph
 }
 
 int lapgpu_batch_query(lapgpu_batch* B, int32_t slot, int* done) {
+  BATCH_LOCK(B);
   if (!B || slot < 0 || slot >= (int32_t)B->slots.size() || !done)
     return berr(B, -1, "lapgpu_batch_query: bad arguments");
   BatchSlot& S = B->slots[slot];
@@ -453,6 +470,7 @@ int lapgpu_batch_query(lapgpu_batch* B, int32_t slot, int* done) {
 
 int lapgpu_batch_collect(lapgpu_batch* B, int32_t slot, double* x_concat, int64_t* its_out,
                          double* relres_out, int32_t* stop_out) {
+  BATCH_LOCK(B);
   if (!B || slot < 0 || slot >= (int32_t)B->slots.size())
     return berr(B, -1, "lapgpu_batch_collect: bad arguments");
   BatchSlot& S = B->slots[slot];
@@ -477,6 +495,7 @@ int lapgpu_batch_solve_device(lapgpu_batch* B, int64_t nreq,
                               double tol, int64_t maxits, double maxtime,
                               int stag_test, int flags, int64_t* its_out,
                               double* relres_out, int32_t* stop_out) {
+  BATCH_LOCK(B);
   if (!B || nreq < 0) return berr(B, -1, "lapgpu_batch_solve: bad arguments");
   if (nreq == 0) return 0;
   int32_t slot = -1;
@@ -493,6 +512,7 @@ int lapgpu_batch_solve(lapgpu_batch* B, int64_t nreq, const int32_t* graph_ids,
                        const double* b_concat, double* x_concat, double tol,
                        int64_t maxits, double maxtime, int stag_test, int flags,
                        int64_t* its_out, double* relres_out, int32_t* stop_out) {
+  BATCH_LOCK(B);
   if (!B || nreq < 0 || (nreq && (!graph_ids || !b_concat || !x_concat)))
     return berr(B, -1, "lapgpu_batch_solve: bad arguments");
   if (nreq == 0) return 0;
