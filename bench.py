@@ -690,7 +690,7 @@ def main() -> int:
     ap.add_argument("--serve-new-max-n", type=float, default=2e5)
     ap.add_argument("--new-frac", type=float, default=0.2)
     ap.add_argument("--serve-workers", type=int, default=8, help="serving: graphs solved concurrently")
-    ap.add_argument("--factor-workers", type=int, default=8, help="serving: host threads factoring new graphs")
+    ap.add_argument("--factor-workers", type=int, default=12, help="serving: host threads factoring new graphs")
     ap.add_argument("--wg-max-n", type=int, default=50000,
                     help="serving: graphs up to this size are solved by the batched engine")
     ap.add_argument("--spin-waits", action="store_true",
