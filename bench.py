@@ -689,13 +689,13 @@ def main() -> int:
     ap.add_argument("--serve-max-n", type=float, default=4e6)
     ap.add_argument("--serve-new-max-n", type=float, default=2e5)
     ap.add_argument("--new-frac", type=float, default=0.2)
-    ap.add_argument("--serve-workers", type=int, default=8, help="serving: graphs solved concurrently")
+    ap.add_argument("--serve-workers", type=int, default=10, help="serving: graphs solved concurrently")
     ap.add_argument("--factor-workers", type=int, default=12, help="serving: host threads factoring new graphs")
     ap.add_argument("--wg-max-n", type=int, default=50000,
                     help="serving: graphs up to this size are solved by the batched engine")
     ap.add_argument("--spin-waits", action="store_true",
                     help="serving: solve workers spin while waiting for the device (default: they sleep)")
-    ap.add_argument("--quick-workers", type=int, default=4,
+    ap.add_argument("--quick-workers", type=int, default=6,
                     help="serving: workers for solves of graphs with n <= 3e5 (0: share the others)")
     ap.add_argument("--batch-window", type=float, default=2.0, help="serving: wait for more requests, x last solve time")
     ap.add_argument("--max-wait", type=float, default=3.0, help="serving: longest wait for more requests (s)")
