@@ -427,7 +427,8 @@ void free_multi(lapgpu_ctx* c) {
   void* dp[] = {m.r, m.x, m.bestx, m.p, m.q, m.y, m.z, m.io, m.part, m.ext,
                 m.y32, m.z32, m.pf, m.ext32, m.r0, m.flags, m.st};
   for (void* a : dp) dev_free(c, a);
-  // the context's stream is idle whenever the multi workspace is dropped
+  // the host mirror may still be the target of a copy on the stream
+  if (m.hst) (void)hipStreamSynchronize(c->stream);
   lapgpu::pinned_put(m.hst, sizeof(DevState) * kMultiMax);
   m = lapgpu_ctx::Multi();
 }
@@ -575,6 +576,9 @@ int upload_plan(lapgpu_ctx* c) {
   Plan& P = c->plan;
   const int64_t n = P.n;
   int rc;
+  // the multi-RHS workspace is sized by the previous plan (n * K vectors,
+  // NT * K tail inputs): drop it, the next multi-RHS solve sizes it again
+  free_multi(c);
   if ((rc = dev_upload(c, &c->perm, P.perm))) return rc;
   if ((rc = dev_upload(c, &c->Lptr, P.Lptr))) return rc;
   if ((rc = dev_upload(c, &c->Lidx, P.Lidx))) return rc;

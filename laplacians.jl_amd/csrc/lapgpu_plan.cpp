@@ -846,6 +846,10 @@ bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
       P.s[P.iperm[v]] = dv != 0.0 ? Pc[v] * Pc[v] / dv : Pc[v] * Pc[v];
     }
   });
+  // the never-eliminated root: LDLsolver divides y[root] by d[root] when it
+  // is nonzero (src/approxChol.jl:1351-1355); the reference's builders
+  // leave it 0, a caller's LDLinv may not
+  if (root >= 0 && n > 1 && d[root] != 0.0) P.s[P.iperm[root]] = 1.0 / d[root];
 
   // ---- forward rows: in-edges of v, in elimination order of the source ----
   P.Fptr.assign(n + 1, 0);

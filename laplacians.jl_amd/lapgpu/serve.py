@@ -400,9 +400,19 @@ class Server:
                 progressed = True
             # start a batched launch with everything queued, if a slot is free
             while small:
-                take = [small.popleft() for _ in range(min(len(small), self.max_launch))]
+                # one launch runs every request with the same tol and maxits:
+                # take the queued requests that share the oldest one's, in
+                # arrival order, and leave the others queued
+                key = (small[0][1].tol, small[0][1].maxits)
+                take, rest = [], deque()
+                while small and len(take) < self.max_launch:
+                    item = small.popleft()
+                    (take if (item[1].tol, item[1].maxits) == key else rest).append(item)
+                rest.extend(small)
+                small.clear()
+                small.extend(rest)
                 h = self.batch.submit_host([g.handle for g, _ in take], [r.b for _, r in take],
-                                           tol=take[0][1].tol, maxits=take[0][1].maxits)
+                                           tol=key[0], maxits=key[1])
                 if h is None:
                     for item in reversed(take):
                         small.appendleft(item)

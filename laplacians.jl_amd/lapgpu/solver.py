@@ -87,7 +87,10 @@ class GpuLapSolver:
         flags = _native.VERBOSE if verbose else 0
         x, info = self.ctx.solve(b, tol=tol, maxits=mi, maxtime=float(maxtime), stag_test=self.stag_test, flags=flags)
         self.last = info
-        _set_its(pcgIts, info["its"])
+        # a zero right-hand side returns before pcg writes pcgIts
+        # (src/pcg.jl:269-271)
+        if info["stop"] != "zero_rhs":
+            _set_its(pcgIts, info["its"])
         return x
 
 

@@ -1,6 +1,8 @@
 // Test driver: builds the device plan of one graph + factor on the host and
 // prints a hash of every plan array (FNV-1a over the bytes), so that plans
 // built with different thread counts can be compared bit for bit.
+// With a second argument "root_s" it prints the diagonal scale of the
+// never-eliminated vertex instead.
 // Input: a directory with raw little-endian arrays cp, rv (int64), nz
 // (float64) of la in CSC, and col, fcp, frv (int64), fv, d (float64) of the
 // LDLinv, all 0-based.
@@ -55,6 +57,14 @@ int main(int argc, char** argv) {
                   dd.data(), 0, 24, P, err)) {
     printf("error %s\n", err.c_str());
     return 1;
+  }
+  if (argc > 2 && std::string(argv[2]) == "root_s") {
+    // the diagonal scale of the never-eliminated vertex, at its position
+    std::vector<char> elim(n, 0);
+    for (int64_t c : col) elim[c] = 1;
+    for (int64_t v = 0; v < n; ++v)
+      if (!elim[v]) printf("root %lld s %.17g\n", (long long)v, P.s[P.iperm[v]]);
+    return 0;
   }
   build_blocks(P, 16, 4.0, 1 << 22);
 #define H(v) printf("%s %zu %016llx\n", #v, P.v.size(), fnv(P.v))

@@ -10,6 +10,7 @@ import pytest
 import scipy.sparse as sp
 
 import lapgpu
+from _bars import assert_relres, assert_relres_cols
 
 pytestmark = pytest.mark.gpu
 
@@ -31,4 +32,4 @@ def test_gpu_matches_golden(path, gpu_ctx_factory):
     x, info = ctx.solve(b, tol=1e-8, maxits=1000, stag_test=5)
     assert abs(info["its"] - int(z["its"])) <= 2 and info["stop"] == str(z["stop"])
     assert np.linalg.norm(x - z["x"]) <= 1e-6 * np.linalg.norm(z["x"])
-    assert np.linalg.norm(la @ x - b) / np.linalg.norm(b) <= 1e-7
+    assert_relres(la, x, b, 1e-8, f)

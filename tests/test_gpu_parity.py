@@ -14,6 +14,7 @@ import pytest
 
 import lapgpu
 from lapgpu import graphs
+from _bars import assert_relres, assert_relres_cols
 
 pytestmark = pytest.mark.gpu
 
@@ -71,9 +72,7 @@ def test_solve(case, gpu_ctx_factory, oracle_mod, tol):
     relres = np.linalg.norm(la @ x - b) / np.linalg.norm(b)
     assert info["stop"] == io["stop"], (name, info, io)
     assert abs(info["its"] - io["its"]) <= 2, (name, info, io)
-    # the recursive residual is what pcg tests; the true one may differ by
-    # rounding, so allow a small factor on the true residual
-    assert relres <= 10 * tol, (name, relres)
+    assert_relres(la, x, b, tol, f, what=name)
     assert np.linalg.norm(x - xo) <= 1e-4 * np.linalg.norm(xo), name
 
 
@@ -121,7 +120,7 @@ def test_closure_dropin(gpu_ctx_factory):
     its = [0]
     x = f(b, pcgIts=its)
     assert its[0] > 0
-    assert np.linalg.norm(la @ x - b) / np.linalg.norm(b) <= 1e-7
+    assert_relres(la, x, b, 1e-8, f.ldli)
     x2 = f(b, tol=1e-3, pcgIts=its)
     assert np.linalg.norm(la @ x2 - b) / np.linalg.norm(b) <= 1e-2
 
@@ -159,7 +158,7 @@ def test_large_grid3_true_residual(gpu_ctx_factory):
     b = _rhs(a.shape[0], 0)
     x, info = ctx.solve(b, tol=1e-8, maxits=1000)
     assert info["stop"] == "converged"
-    assert np.linalg.norm(la @ x - b) / np.linalg.norm(b) <= 1e-7
+    assert_relres(la, x, b, 1e-8, f)
     assert math.isfinite(info["nr"]) and info["nr"] < 1e-8
 
 
@@ -187,3 +186,73 @@ def test_tol_zero_ends_by_stagnation(gpu_ctx_factory, oracle_mod):
         assert info["its"] % 10 == 0
     assert abs(info["its"] - io["its"]) <= 10
     assert np.linalg.norm(la @ x - b) <= 1e-12 * np.linalg.norm(b)
+
+
+def test_zero_rhs_leaves_pcgits():
+    """pcg returns before it writes pcgIts when b is zero after the
+    closure's mean removal (src/pcg.jl:269-271): the caller's value stays."""
+    a = graphs.grid2(12)
+    f = lapgpu.approxchol_lap_gpu(a, tol=1e-8)
+    its = [7]
+    x = f(np.full(144, 2.0), pcgIts=its)
+    assert its == [7] and np.all(x == 0)
+    x = f(_rhs(144, 1), pcgIts=its)
+    assert its[0] > 0 and its[0] != 7
+
+
+def test_apply_with_nonzero_root_d(gpu_ctx_factory, oracle_mod):
+    """A caller's LDLinv with d[root] != 0: LDLsolver divides y[root] by it
+    (src/approxChol.jl:1351-1355), and so does the device apply, on a
+    right-hand side that does not sum to zero (y[root] gathers its sum)."""
+    import dataclasses
+
+    a = graphs.weighted_grid2(30, 20, np.random.default_rng(13), "expU2")
+    la = lapgpu.lap(a)
+    f = lapgpu.build_ldli(a, seed=1)
+    root = int(np.setdiff1d(np.arange(a.shape[0]), f.col)[0])
+    d = f.d.copy()
+    d[root] = 2.5
+    g = dataclasses.replace(f, d=d)
+    ctx = gpu_ctx_factory(la, g)
+    r = np.random.default_rng(8).standard_normal(la.shape[0]) + 0.5
+    z = ctx.apply_ldli(r)
+    ref = oracle_mod.ldlsolver(g, r)
+    # the root's scale adds the same constant to every entry (backward rows
+    # are convex combinations), which the closing mean removal cancels, so
+    # the output equals the builders' factor's up to rounding; the plan's
+    # root scale itself is pinned by tests/test_plan_host.py
+    assert np.linalg.norm(z - ref) <= APPLY_RTOL * np.linalg.norm(ref)
+    assert np.linalg.norm(ref - oracle_mod.ldlsolver(f, r)) <= 1e-12 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("which", ["inf_rho", "inf_beta"])
+def test_overflow_exits_on_every_device_path(which, gpu_ctx_factory, oracle_mod):
+    """pcg's isinf(rho) and isinf(beta) exits (src/pcg.jl:385-402) on the
+    hand-derived inputs of tests/_exits.py (raw pcg: no mean removal of b),
+    on the single-RHS, multi-RHS and batched paths: the oracle's stop at
+    iteration 1 and its final rho."""
+    import _exits as E
+
+    dd, b = (E.D_INF_RHO, E.B_INF_RHO) if which == "inf_rho" else (E.D_INF_BETA, E.B_INF_BETA)
+    a, f = E.factor_with_d(lapgpu, *dd)
+    la = lapgpu.lap(a)
+    with np.errstate(all="ignore"):
+        xo, io = oracle_mod.pcg_ldl(la, f, b, tol=1e-8, maxits=30, stag_test=5)
+    assert io["stop"] == which and io["its"] == 1
+    from lapgpu import _native
+
+    ctx = gpu_ctx_factory(la, f)
+    x, info = ctx.solve(b, tol=1e-8, maxits=30, stag_test=5, flags=_native.NO_CENTER)
+    assert info["stop"] == which and info["its"] == 1, info
+    assert np.allclose(x, xo, rtol=1e-12, atol=0)
+    X, mi = ctx.solve_multi(np.stack([b, b], axis=1), tol=1e-8, maxits=30, stag_test=5, flags=_native.NO_CENTER)
+    assert mi["stop"] == [which, which] and mi["its"] == [1, 1], mi
+    assert np.allclose(X[:, 0], xo, rtol=1e-12, atol=0)
+    batch = lapgpu.Batch(0)
+    try:
+        gid = batch.add_graph(la, f)
+        xs, infos = batch.solve([gid], [b], tol=1e-8, maxits=30, stag_test=5, flags=_native.NO_CENTER)
+    finally:
+        batch.close()
+    assert infos[0]["stop"] == which and infos[0]["its"] == 1, infos
+    assert np.allclose(xs[0], xo, rtol=1e-12, atol=0)

@@ -14,6 +14,7 @@ import pytest
 import lapgpu
 from lapgpu import graphs
 from lapgpu.serve import Request, Server, summary
+from _bars import assert_relres
 
 pytestmark = pytest.mark.gpu
 
@@ -57,7 +58,7 @@ def test_serve_mixed_stream_matches_alone():
             a = G[r.graph] if r.graph is not None else newg
             la = lapgpu.lap(a)
             assert out.stop == "converged", out
-            assert np.linalg.norm(la @ out.x - r.b) / np.linalg.norm(r.b) <= 1e-7
+            assert_relres(la, out.x, r.b, 1e-8, lambda: lapgpu.build_ldli(a, seed=1))
         paths = {o.path for o in res}
         assert "batched" in paths and paths & {"plain", "multi"}, paths
         # the same requests alone
@@ -92,7 +93,7 @@ def test_serve_burst_uses_multi_rhs():
         la = lapgpu.lap(a)
         for r, out in zip(reqs, res):
             assert out.stop == "converged"
-            assert np.linalg.norm(la @ out.x - r.b) / np.linalg.norm(r.b) <= 1e-7
+            assert_relres(la, out.x, r.b, 1e-8, lambda: lapgpu.build_ldli(a, seed=1))
         for r, out in list(zip(reqs, res))[:4]:
             alone = srv.run([Request(100 + r.rid, 0.0, r.b, graph="g", tol=1e-8)])[0]
             assert alone.path == "plain"
@@ -110,5 +111,32 @@ def test_admission_by_hbm_budget():
         # far too large for the budget
         assert not srv.register("big", G["big"], resident=False)
         assert srv.hbm_used <= srv.hbm_budget
+    finally:
+        srv.close()
+
+
+def test_serve_mixed_tolerances_each_meet_their_own():
+    """Requests with different tol and maxits queued together for batched
+    launches: each is solved to its own tol (its true residual, the bar of tests/_bars.py) and with its
+    own maxits, as if served alone."""
+    rng = np.random.default_rng(17)
+    a = graphs.weighted_grid2(40, 30, rng)
+    la = lapgpu.lap(a)
+    srv = Server(0, wg_max_n=20000, keep_x=True)
+    try:
+        assert srv.register("g", a)
+        tols = [1e-4, 1e-10, 1e-6, 1e-10, 1e-4, 1e-8]
+        reqs = [Request(i, 0.0, _rhs(1200, rng), graph="g", tol=t) for i, t in enumerate(tols)]
+        reqs.append(Request(len(tols), 0.0, _rhs(1200, rng), graph="g", tol=1e-12, maxits=3))
+        res = srv.run(reqs, realtime=False)
+        assert {o.path for o in res} == {"batched"}
+        for r, out in zip(reqs, res):
+            if r.maxits == 3:
+                assert out.stop == "maxits" and out.its == 3, out
+                continue
+            assert out.stop == "converged", out
+            assert_relres(la, out.x, r.b, r.tol, lambda: lapgpu.build_ldli(a, seed=1))
+            alone = srv.run([Request(100 + r.rid, 0.0, r.b, graph="g", tol=r.tol)])[0]
+            assert alone.its == out.its and np.array_equal(alone.x, out.x), (r.tol, alone.its, out.its)
     finally:
         srv.close()

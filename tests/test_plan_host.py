@@ -68,3 +68,31 @@ def test_entries_per_lane_follow_row_length(driver, tmp_path):
         r = subprocess.run([driver, d], capture_output=True, text=True, check=True)
         elog = int(r.stdout.strip().splitlines()[-1].split()[7])
         assert elog == want, name
+
+
+def test_root_scale_follows_its_d(driver, tmp_path):
+    """LDLsolver divides y[root] by d[root] when it is nonzero
+    (src/approxChol.jl:1351-1355).  The builders leave d[root] = 0 (scale 1);
+    a caller's LDLinv with d[root] = 2.5 must give the root the scale
+    1/2.5 = 0.4, and the root keeps scale 1 when d[root] = 0."""
+    import dataclasses
+
+    a = graphs.weighted_grid2(20, 15, np.random.default_rng(13), "expU2")
+    f = lapgpu.build_ldli(a, seed=1)
+    root = int(np.setdiff1d(np.arange(a.shape[0]), f.col)[0])
+    assert f.d[root] == 0.0
+    for droot, want in ((0.0, 1.0), (2.5, 0.4)):
+        d = f.d.copy()
+        d[root] = droot
+        g = dataclasses.replace(f, d=d)
+        out = str(tmp_path / f"root_{droot}")
+        la = lapgpu.lap(a)
+        os.makedirs(out, exist_ok=True)
+        for k, v in (("cp", la.indptr), ("rv", la.indices), ("nz", la.data), ("col", g.col), ("fcp", g.colptr),
+                     ("frv", g.rowval), ("fv", g.fval), ("d", g.d)):
+            np.ascontiguousarray(v, dtype=np.float64 if k in ("nz", "fv", "d") else np.int64).tofile(
+                os.path.join(out, k))
+        r = subprocess.run([driver, out, "root_s"], capture_output=True, text=True, check=True)
+        words = r.stdout.split()
+        assert int(words[1]) == root
+        assert float(words[3]) == want, (droot, r.stdout)
