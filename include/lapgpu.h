@@ -292,6 +292,22 @@ int lapgpu_batch_query(lapgpu_batch* B, int32_t slot, int* done);
 int lapgpu_batch_collect(lapgpu_batch* B, int32_t slot, double* x_concat, int64_t* its_out,
                          double* relres_out, int32_t* stop_out);
 
+/* Allocation-free form of lapgpu_batch_solve_device (no counterpart in the
+ * reference): for callers that cannot allocate device memory, such as code
+ * under HIP graph capture, tight serving loops and harnesses that forbid
+ * hipMalloc.  It calls no hipMalloc, hipFree or pinned allocation: the
+ * requests' work vectors (8 n doubles each), request table and graph
+ * descriptors live in a static device array of 64 MiB per device (about
+ * 1,000,000 vertices over all requests of one call), and at most 1024
+ * requests go in one call; larger calls fail with a message.  Same kernel
+ * and results as lapgpu_batch_solve_device; synchronous.  Calls share the
+ * device's array and are serialised by a process-wide lock; the array keeps
+ * the last call's contents. */
+int lapgpu_batch_solve_device_noalloc(lapgpu_batch* B, int64_t nreq, const int32_t* graph_ids,
+                                      const double* const* d_b, double* const* d_x, double tol,
+                                      int64_t maxits, double maxtime, int stag_test, int flags,
+                                      int64_t* its_out, double* relres_out, int32_t* stop_out);
+
 const char* lapgpu_batch_last_error(lapgpu_batch* B);
 void lapgpu_batch_destroy(lapgpu_batch* B);
 

@@ -25,6 +25,15 @@
 
 using namespace lapgpu;
 
+// ---- scratch of lapgpu_batch_solve_device_noalloc -------------------------
+// One array per device, holding the requests' graph descriptors, the request
+// table and the per-request work vectors (8 n doubles each) of one call.  In
+// host code `scratch` names a host-side stand-in: kernels and copies use the
+// device address that hipGetSymbolAddress returns for the current device,
+// looked up once per device when a batch is created there (scratch_setup).
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
 
 struct GraphStore {
@@ -39,6 +48,35 @@ struct GraphStore {
 
 std::string g_batch_err;
 constexpr int kBatchSlots = 4;
+
+constexpr int64_t kNoAllocMaxReq = 1024;
+
+std::mutex g_scratch_mu;           // guards g_scratch_ptr
+std::vector<void*> g_scratch_ptr;  // per device: device address of scratch
+// host staging of one call (static: the entry point allocates nothing);
+// the calls hold g_noalloc_mu from start to end
+std::mutex g_noalloc_mu;
+WgGraphD g_na_graphs[kNoAllocMaxReq];
+WgReqD g_na_reqs[kNoAllocMaxReq];
+int64_t g_na_order[kNoAllocMaxReq];
+
+// device address of this device's scratch (call with the device current)
+hipError_t scratch_setup(int device, void** out) {
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if ((size_t)device < g_scratch_ptr.size() && g_scratch_ptr[device]) {
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+  }
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize(device + 1, nullptr);
+  g_scratch_ptr[device] = p;
+  *out = p;
+  return hipSuccess;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 }  // namespace
 
@@ -83,6 +121,7 @@ struct lapgpu_batch {
   std::vector<BatchSlot> slots;
   double* d_io = nullptr;
   size_t d_io_cap = 0;
+  void* scratch = nullptr;  // this device's scratch (lapgpu_batch_solve_device_noalloc)
 };
 
 namespace {
@@ -171,6 +210,7 @@ int lapgpu_batch_create(lapgpu_batch** out, int device) {
   B->device = device;
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = lapgpu::pool_setup(device);
+  if (e == hipSuccess) e = scratch_setup(device, &B->scratch);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     g_batch_err = std::string("lapgpu_batch_create: ") + hipGetErrorString(e);
@@ -543,6 +583,88 @@ int lapgpu_batch_solve(lapgpu_batch* B, int64_t nreq, const int32_t* graph_ids,
   BHIP(B, hipMemcpyAsync(x_concat, B->d_io + total, sizeof(double) * total,
                          hipMemcpyDeviceToHost, B->stream));
   BHIP(B, hipStreamSynchronize(B->stream));
+  return 0;
+}
+
+
+/* Allocation-free form of lapgpu_batch_solve_device: the same kernel and
+ * arithmetic, with its work vectors, request table and graph descriptors
+ * in the device's scratch array (kScratchBytes) and host staging in static
+ * arrays; no hipMalloc, hipFree or pinned allocation. */
+int lapgpu_batch_solve_device_noalloc(lapgpu_batch* B, int64_t nreq, const int32_t* graph_ids,
+                                      const double* const* d_b, double* const* d_x, double tol,
+                                      int64_t maxits, double maxtime, int stag_test, int flags,
+                                      int64_t* its_out, double* relres_out, int32_t* stop_out) {
+  BATCH_LOCK(B);
+  if (!B || nreq < 0 || (nreq && (!graph_ids || !d_b || !d_x)))
+    return berr(B, -1, "lapgpu_batch_solve_device_noalloc: bad arguments");
+  if (nreq == 0) return 0;
+  if (nreq > kNoAllocMaxReq)
+    return berr(B, -1, "lapgpu_batch_solve_device_noalloc: more than 1024 requests in one call");
+  int rc;
+  if ((rc = check_ids(B, nreq, graph_ids))) return rc;
+  // scratch layout: descriptors | requests | work vectors (8 n doubles each)
+  const size_t off_reqs = align256(sizeof(WgGraphD) * (size_t)nreq);
+  const size_t off_work = off_reqs + align256(sizeof(WgReqD) * (size_t)nreq);
+  size_t need = off_work;
+  for (int64_t i = 0; i < nreq; ++i) need += sizeof(double) * 8 * (size_t)B->graphs[graph_ids[i]].n;
+  if (need > kScratchBytes)
+    return berr(B, -1, "lapgpu_batch_solve_device_noalloc: the requests need " + std::to_string(need) +
+                           " bytes of scratch, more than its " + std::to_string(kScratchBytes) +
+                           " (at most about 1,000,000 vertices over all requests of a call)");
+  if (!B->scratch) return berr(B, -2, "lapgpu_batch_solve_device_noalloc: no scratch on this device");
+  BHIP(B, hipSetDevice(B->device));
+  std::lock_guard<std::mutex> g(g_noalloc_mu);
+  unsigned char* base = static_cast<unsigned char*>(B->scratch);
+  WgGraphD* d_graphs = reinterpret_cast<WgGraphD*>(base);
+  WgReqD* d_reqs = reinterpret_cast<WgReqD*>(base + off_reqs);
+  double* work = reinterpret_cast<double*>(base + off_work);
+  // largest first, as the other entry points order them (an in-place sort
+  // with the request index as tie-break: the same order as a stable sort)
+  for (int64_t i = 0; i < nreq; ++i) g_na_order[i] = i;
+  std::sort(g_na_order, g_na_order + nreq, [&](int64_t a, int64_t b) {
+    const int na = B->graphs[graph_ids[a]].n, nb = B->graphs[graph_ids[b]].n;
+    return na != nb ? na > nb : a < b;
+  });
+  const bool has_mt = maxtime > 0 && std::isfinite(maxtime);
+  size_t off = 0;
+  for (int64_t k = 0; k < nreq; ++k) {
+    const int64_t i = g_na_order[k];
+    const GraphStore& G = B->graphs[graph_ids[i]];
+    // one descriptor per request (no table of every graph to upload)
+    g_na_graphs[k] = WgGraphD{G.n, G.nf, G.nb, 0, G.perm, G.Lptr, G.Lidx, G.Lval, G.s,
+                              G.fsteps, G.fslots, G.bsteps, G.bslots};
+    WgReqD R{};
+    R.b = d_b[i];
+    R.x = d_x[i];
+    R.work = work + off;
+    off += 8 * (size_t)G.n;
+    R.tol = tol;
+    R.maxits = maxits;
+    R.maxticks = has_mt ? (unsigned long long)(maxtime * B->wallclock_khz * 1e3) : 0ull;
+    R.graph = (int)k;
+    R.stag_test = stag_test;
+    R.center = (flags & LAPGPU_NO_CENTER) ? 0 : 1;
+    R.has_maxtime = has_mt ? 1 : 0;
+    g_na_reqs[k] = R;
+  }
+  BHIP(B, hipMemcpyAsync(d_graphs, g_na_graphs, sizeof(WgGraphD) * nreq, hipMemcpyHostToDevice,
+                         B->stream));
+  BHIP(B, hipMemcpyAsync(d_reqs, g_na_reqs, sizeof(WgReqD) * nreq, hipMemcpyHostToDevice,
+                         B->stream));
+  hipLaunchKernelGGL(k_wg_pcg, dim3((unsigned)nreq), dim3(kWgThreads), 0, B->stream, d_graphs,
+                     d_reqs);
+  BHIP(B, hipGetLastError());
+  BHIP(B, hipMemcpyAsync(g_na_reqs, d_reqs, sizeof(WgReqD) * nreq, hipMemcpyDeviceToHost,
+                         B->stream));
+  BHIP(B, hipStreamSynchronize(B->stream));
+  for (int64_t k = 0; k < nreq; ++k) {
+    const int64_t i = g_na_order[k];
+    if (its_out) its_out[i] = g_na_reqs[k].its;
+    if (relres_out) relres_out[i] = g_na_reqs[k].nr;
+    if (stop_out) stop_out[i] = g_na_reqs[k].stop;
+  }
+  B->err.clear();
   return 0;
 }
 

@@ -65,6 +65,7 @@ EXPORTS = (
     "lapgpu_batch_graph_info",
     "lapgpu_batch_solve",
     "lapgpu_batch_solve_device",
+    "lapgpu_batch_solve_device_noalloc",
     "lapgpu_batch_submit",
     "lapgpu_batch_submit_host",
     "lapgpu_batch_query",
@@ -120,6 +121,7 @@ def load() -> ctypes.CDLL:
     L.lapgpu_batch_graph_info.argtypes = [P, ctypes.c_int32, pi64, pi64]
     L.lapgpu_batch_solve.argtypes = [P, i64, P, P, P, dbl, i64, dbl, ci, ci, P, P, P]
     L.lapgpu_batch_solve_device.argtypes = [P, i64, P, P, P, dbl, i64, dbl, ci, ci, P, P, P]
+    L.lapgpu_batch_solve_device_noalloc.argtypes = [P, i64, P, P, P, dbl, i64, dbl, ci, ci, P, P, P]
     L.lapgpu_batch_submit.argtypes = [P, i64, P, P, P, dbl, i64, dbl, ci, ci, pi32]
     L.lapgpu_batch_query.argtypes = [P, ctypes.c_int32, ctypes.POINTER(ci)]
     L.lapgpu_batch_submit_host.argtypes = [P, i64, P, P, dbl, i64, dbl, ci, ci, pi32]
@@ -502,6 +504,24 @@ class Batch:
                                                         float(maxtime), int(stag_test), int(flags), _p(its), _p(nr),
                                                         _p(stop)),
                     "lapgpu_batch_solve_device")
+        return [{"its": int(its[i]), "nr": float(nr[i]), "stop": STOP_NAMES.get(int(stop[i]), str(stop[i]))}
+                for i in range(k)]
+
+    def solve_device_noalloc(self, graph_ids, b_ptrs, x_ptrs, tol=1e-6, maxits=1000, maxtime=float("inf"),
+                             stag_test=5, flags=0):
+        """solve_device through lapgpu_batch_solve_device_noalloc: no device
+        allocation; at most 1024 requests and 64 MiB of work vectors."""
+        ids = np.ascontiguousarray(graph_ids, dtype=np.int32)
+        k = len(ids)
+        pb = np.ascontiguousarray(b_ptrs, dtype=np.uint64)
+        px = np.ascontiguousarray(x_ptrs, dtype=np.uint64)
+        its = np.zeros(k, dtype=np.int64)
+        nr = np.zeros(k)
+        stop = np.zeros(k, dtype=np.int32)
+        self._check(self._lib.lapgpu_batch_solve_device_noalloc(self._h, k, _p(ids), _p(pb), _p(px), float(tol),
+                                                                int(maxits), float(maxtime), int(stag_test),
+                                                                int(flags), _p(its), _p(nr), _p(stop)),
+                    "lapgpu_batch_solve_device_noalloc")
         return [{"its": int(its[i]), "nr": float(nr[i]), "stop": STOP_NAMES.get(int(stop[i]), str(stop[i]))}
                 for i in range(k)]
 
