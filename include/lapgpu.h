@@ -159,6 +159,20 @@ int lapgpu_set_wait_mode(lapgpu_ctx* ctx, int mode);
 int lapgpu_set_group_budget(lapgpu_ctx* ctx, int max_depth, double max_growth,
                             int64_t max_extra);
 
+/* The dense tail block (no counterpart in the reference; part of this
+ * library's LDLinv apply).  The deepest levels of the single-workgroup tail
+ * are applied as one symmetric matrix D = (I - W^T)^-1 S (I - W)^-1 of their
+ * K vertices (built on the device at upload by running their sweeps on unit
+ * vectors), a matrix-vector product (many right-hand sides: a matrix-core
+ * product) instead of their dependent steps.  The planner chooses K by a
+ * cost model, at most max_k (0 disables; default 18432, the tail's LDS
+ * capacity; env LAPGPU_DENSE_MAX).  Takes effect at the next
+ * lapgpu_upload_ldli.  D costs 8 K^2 bytes (12 K^2 in the FP32-inner mode). */
+int lapgpu_set_dense_tail(lapgpu_ctx* ctx, int64_t max_k);
+/* K of the uploaded factor's dense block (0: none), the tail vertices
+ * before it, and the device bytes of D. */
+int lapgpu_dense_info(lapgpu_ctx* ctx, int64_t* k, int64_t* sparse_vertices, int64_t* bytes);
+
 /* Schedule facts of the uploaded factor: number of levels, number of levels
  * run one launch each, the first vertex position of the single-workgroup
  * tail, nnzF. Any pointer may be NULL. */

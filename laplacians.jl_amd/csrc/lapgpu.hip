@@ -150,6 +150,13 @@ struct lapgpu_ctx {
   double *Ew = nullptr, *ext = nullptr;
   TailStepD *fsteps = nullptr, *bsteps = nullptr;
   TailSlotD *fslots = nullptr, *bslots = nullptr;
+  // the tail's dense block (P.dense): D (K rows, leading dimension ldd) and
+  // the block's inputs (ldd entries, zero beyond K); the largest block the
+  // planner may choose (0: none)
+  int64_t dense_max = kTailCap;
+  int64_t ldd = 0;
+  double *D = nullptr, *dvec = nullptr;
+  float *D32 = nullptr, *dvec32 = nullptr;
   // FP32-inner mode copies (built by lapgpu_set_precision)
   bool f32_ready = false;
   float *Lval32 = nullptr, *s32 = nullptr, *Ew32 = nullptr, *y32 = nullptr, *ext32 = nullptr,
@@ -189,6 +196,10 @@ struct lapgpu_ctx {
     int64_t cap32 = 0;
     float *y32 = nullptr, *z32 = nullptr, *pf = nullptr, *ext32 = nullptr;
     double* r0 = nullptr;
+    // dense block: its inputs (ldd x K, zero beyond row K) and the split
+    // partial products
+    double *dvec = nullptr, *dpart = nullptr;
+    float *dvec32 = nullptr, *dpart32 = nullptr;
     // captured iteration (exec[1]: FP32 mode with a residual replacement)
     hipGraphExec_t exec[2] = {nullptr, nullptr};
   } m;
@@ -268,6 +279,24 @@ int dev_alloc(lapgpu_ctx* c, T** dst, size_t count) {
   return 0;
 }
 
+// count elements, zero-filled (at upload time; never inside a capture)
+template <typename T>
+int dev_zeros(lapgpu_ctx* c, T** dst, size_t count) {
+  int rc = dev_alloc(c, dst, count);
+  if (rc) return rc;
+  HIPC(c, hipMemsetAsync(*dst, 0, sizeof(T) * (count ? count : 1), c->stream));
+  return 0;
+}
+
+// tail inputs: NT entries, and up to ldd (zero) when the dense block takes
+// them as they are (no sparse part before it)
+size_t ext_len(const lapgpu_ctx* c) {
+  const TailProgram& TP = c->plan.tail;
+  size_t n = (size_t)std::max<int64_t>(TP.NT, 1);
+  if (c->plan.dense.K > 0 && TP.NT1 == 0) n = std::max<size_t>(n, (size_t)c->ldd);
+  return n;
+}
+
 int pick_group(double avg) {
   // lanes per row: the power of two nearest to the mean row length
   int g = 2;
@@ -333,6 +362,19 @@ void launch_group(lapgpu_ctx* c, int lo, int hi, const int* heavy, int nh, const
                      c->stream, (int)c->n, lo, hi, heavy, nh, ptr, idx, w, A, B, out);
 }
 
+// the dense block in precision T: D and its input vector
+template <typename T>
+struct DenseArgs {
+  const T* D;
+  T* dvec;
+};
+template <typename T>
+DenseArgs<T> dense_args(lapgpu_ctx* c);
+template <>
+DenseArgs<double> dense_args<double>(lapgpu_ctx* c) { return {c->D, c->dvec}; }
+template <>
+DenseArgs<float> dense_args<float>(lapgpu_ctx* c) { return {c->D32, c->dvec32}; }
+
 template <typename T, typename S>
 void launch_sweeps(lapgpu_ctx* c, const double* rin, T* y, T* z, const T* gFw, const T* gBw,
                    const T* sv, const T* Ew, T* ext, const S* fslots, const S* bslots) {
@@ -353,10 +395,30 @@ void launch_sweeps(lapgpu_ctx* c, const double* rin, T* y, T* z, const T* gFw, c
       hipLaunchKernelGGL((k_tail_ext<32, T>), dim3(grid_for(TP.NT * 32, kBlock)), dim3(kBlock),
                          0, c->stream, (int)TP.T0, (int)TP.NT, c->Eptr, c->Eidx, Ew, rin, y, ext);
     if (c->prof_sweeps) (void)hipEventRecord(c->ev[6], c->stream);
-    LAPGPU_DISPATCH_E(TP.elog, hipLaunchKernelGGL, (k_tail<T, S, E_>), dim3(1),
-                      dim3(kTailThreads), 0, c->stream, (int)TP.T0, (int)TP.NT, ext, sv, z,
-                      c->fsteps, (int)TP.fsteps.size(), fslots, c->bsteps,
-                      (int)TP.bsteps.size(), bslots, c->st);
+    const DenseBlock& DB = P.dense;
+    if (DB.K == 0) {
+      LAPGPU_DISPATCH_E(TP.elog, hipLaunchKernelGGL, (k_tail<T, S, E_>), dim3(1),
+                        dim3(kTailThreads), 0, c->stream, (int)TP.T0, (int)TP.NT, ext, sv, z,
+                        c->fsteps, (int)TP.fsteps.size(), fslots, c->bsteps,
+                        (int)TP.bsteps.size(), bslots, c->st);
+    } else {
+      // sparse part forward, D u, sparse part backward
+      const DenseArgs<T> da = dense_args<T>(c);
+      const int NT1 = (int)TP.NT1;
+      T* u = NT1 > 0 ? da.dvec : ext;
+      if (NT1 > 0)
+        LAPGPU_DISPATCH_E(TP.elog, hipLaunchKernelGGL, (k_tail_fwd<T, S, E_>), dim3(1),
+                          dim3(kTailThreads), 0, c->stream, (int)TP.T0, (int)TP.NT, NT1, ext, sv, z,
+                          da.dvec, c->fsteps, (int)TP.fsteps.size(), fslots, c->st);
+      const int waves = (int)((DB.K + kDenseRows - 1) / kDenseRows);
+      hipLaunchKernelGGL(k_dense<T>, dim3((waves + kBlock / kWave - 1) / (kBlock / kWave)),
+                         dim3(kBlock), 0, c->stream, (int)DB.K, (long)c->ldd, da.D, u, z + DB.T1,
+                         c->st);
+      if (NT1 > 0)
+        LAPGPU_DISPATCH_E(TP.elog, hipLaunchKernelGGL, (k_tail_bwd<T, S, E_>), dim3(1),
+                          dim3(kTailThreads), 0, c->stream, (int)TP.T0, (int)TP.NT, NT1, z,
+                          c->bsteps, (int)TP.bsteps.size(), bslots, c->st);
+    }
   } else if (c->prof_sweeps) {
     (void)hipEventRecord(c->ev[6], c->stream);
   }
@@ -425,7 +487,8 @@ void free_multi(lapgpu_ctx* c) {
   for (auto& e : m.exec)
     if (e) (void)hipGraphExecDestroy(e);
   void* dp[] = {m.r, m.x, m.bestx, m.p, m.q, m.y, m.z, m.io, m.part, m.ext,
-                m.y32, m.z32, m.pf, m.ext32, m.r0, m.flags, m.st};
+                m.y32, m.z32, m.pf, m.ext32, m.r0, m.flags, m.st,
+                m.dvec, m.dpart, m.dvec32, m.dpart32};
   for (void* a : dp) dev_free(c, a);
   // the host mirror may still be the target of a copy on the stream
   if (m.hst) (void)hipStreamSynchronize(c->stream);
@@ -435,9 +498,10 @@ void free_multi(lapgpu_ctx* c) {
 
 void free_f32(lapgpu_ctx* c) {
   void* fp[] = {c->Lval32, c->gFw32, c->gBw32, c->s32, c->Ew32, c->y32, c->z32, c->ext32, c->pf,
-                c->fslots32, c->bslots32, c->r0};
+                c->fslots32, c->bslots32, c->r0, c->D32, c->dvec32};
   for (void* a : fp) dev_free(c, a);
   c->Lval32 = c->gFw32 = c->gBw32 = c->s32 = c->Ew32 = c->y32 = c->z32 = c->ext32 = c->pf = nullptr;
+  c->D32 = c->dvec32 = nullptr;
   c->fslots32 = c->bslots32 = nullptr;
   c->r0 = nullptr;
   c->f32_ready = false;
@@ -447,10 +511,13 @@ int free_graph(lapgpu_ctx* c) {
   drop_graphs(c);
   free_f32(c);
   free_multi(c);
-  void* tp[] = {c->Eptr, c->Eidx, c->Ew, c->ext, c->fsteps, c->bsteps, c->fslots, c->bslots};
+  void* tp[] = {c->Eptr, c->Eidx, c->Ew, c->ext, c->fsteps, c->bsteps, c->fslots, c->bslots,
+                c->D, c->dvec};
   for (void* a : tp) dev_free(c, a);
   c->Eptr = c->Eidx = nullptr;
   c->Ew = c->ext = nullptr;
+  c->D = c->dvec = nullptr;
+  c->ldd = 0;
   c->fsteps = c->bsteps = nullptr;
   c->fslots = c->bslots = nullptr;
   int* ip[] = {c->perm, c->Lptr, c->Lidx, c->lvl, c->gFptr, c->gFidx, c->gBptr, c->gBidx,
@@ -497,8 +564,54 @@ int prepare_f32(lapgpu_ctx* c) {
   if ((rc = dev_alloc(c, &c->y32, (size_t)n))) return rc;
   if ((rc = dev_alloc(c, &c->pf, (size_t)n))) return rc;
   if ((rc = dev_alloc(c, &c->r0, (size_t)n))) return rc;
-  if ((rc = dev_alloc(c, &c->ext32, (size_t)std::max<int64_t>(P.tail.NT, 1)))) return rc;
+  if ((rc = dev_zeros(c, &c->ext32, ext_len(c)))) return rc;
+  if (P.dense.K > 0) {
+    const size_t cnt = (size_t)P.dense.K * (size_t)c->ldd;
+    if ((rc = dev_alloc(c, &c->D32, cnt))) return rc;
+    hipLaunchKernelGGL(k_narrow, dim3(2048), dim3(kBlock), 0, c->stream, cnt, c->D, c->D32);
+    HIPC(c, hipGetLastError());
+    if ((rc = dev_zeros(c, &c->dvec32, (size_t)c->ldd))) return rc;
+  }
   c->f32_ready = true;
+  return 0;
+}
+
+// D of the dense block: its programs are uploaded for the build only and
+// run on every unit vector, one workgroup per column (columns in chunks of
+// kBasisChunk per launch); the FP32 copy follows in prepare_f32.
+constexpr int kBasisChunk = 2048;
+
+int build_dense(lapgpu_ctx* c) {
+  const DenseBlock& DB = c->plan.dense;
+  if (DB.K == 0) return 0;
+  int rc;
+  const size_t cnt = (size_t)DB.K * (size_t)c->ldd;
+  if ((rc = dev_alloc(c, &c->D, cnt))) return rc;
+  if ((rc = dev_zeros(c, &c->dvec, (size_t)c->ldd))) return rc;
+  std::vector<TailStepD> fs(DB.fsteps.size()), bs(DB.bsteps.size());
+  std::memcpy(fs.data(), DB.fsteps.data(), sizeof(TailStep) * fs.size());
+  std::memcpy(bs.data(), DB.bsteps.data(), sizeof(TailStep) * bs.size());
+  std::vector<TailSlotD> fl(DB.fslots.size()), bl(DB.bslots.size());
+  std::memcpy(fl.data(), DB.fslots.data(), sizeof(TailSlot) * fl.size());
+  std::memcpy(bl.data(), DB.bslots.data(), sizeof(TailSlot) * bl.size());
+  TailStepD *dfs = nullptr, *dbs = nullptr;
+  TailSlotD *dfl = nullptr, *dbl = nullptr;
+  if ((rc = dev_upload(c, &dfs, fs)) || (rc = dev_upload(c, &dbs, bs)) ||
+      (rc = dev_upload(c, &dfl, fl)) || (rc = dev_upload(c, &dbl, bl)))
+    return rc;
+  // P.s in position order: the block's scales are s[T1 ...] (uploaded first)
+  for (int64_t j0 = 0; j0 < DB.K; j0 += kBasisChunk) {
+    const int nb = (int)std::min<int64_t>(kBasisChunk, DB.K - j0);
+    LAPGPU_DISPATCH_E(DB.elog, hipLaunchKernelGGL, (k_dense_basis<E_>), dim3(nb), dim3(kTailThreads),
+                      0, c->stream, (int)DB.K, (int)DB.T1, (int)j0, c->s, dfs, (int)fs.size(), dfl,
+                      dbs, (int)bs.size(), dbl, c->D, (long)c->ldd);
+    HIPC(c, hipGetLastError());
+  }
+  dev_free(c, dfs);
+  dev_free(c, dbs);
+  dev_free(c, dfl);
+  dev_free(c, dbl);
+  HIPC(c, hipStreamSynchronize(c->stream));
   return 0;
 }
 
@@ -589,7 +702,9 @@ int upload_plan(lapgpu_ctx* c) {
   if ((rc = dev_upload(c, &c->Eptr, P.tail.Eptr))) return rc;
   if ((rc = dev_upload(c, &c->Eidx, P.tail.Eidx))) return rc;
   if ((rc = dev_upload(c, &c->Ew, P.tail.Ew))) return rc;
-  if ((rc = dev_alloc(c, &c->ext, (size_t)std::max<int64_t>(P.tail.NT, 1)))) return rc;
+  c->ldd = P.dense.K > 0 ? (P.dense.K + 255) / 256 * 256 : 0;
+  if ((rc = dev_zeros(c, &c->ext, ext_len(c)))) return rc;
+  if ((rc = build_dense(c))) return rc;
   {
     static_assert(sizeof(TailStepD) == sizeof(TailStep), "step layout");
     static_assert(sizeof(TailSlotD) == sizeof(TailSlot), "slot layout");
@@ -869,12 +984,35 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
     default: { constexpr int K_ = 64; F(__VA_ARGS__); break; }      \
   }
 
+// the dense block for K columns in precision T: D, its inputs and the
+// split partials
+template <typename T>
+struct MDenseArgs {
+  const T* D;
+  T* dvec;
+  T* part;
+};
+template <typename T>
+MDenseArgs<T> mdense_args(lapgpu_ctx* c);
+template <>
+MDenseArgs<double> mdense_args<double>(lapgpu_ctx* c) { return {c->D, c->m.dvec, c->m.dpart}; }
+template <>
+MDenseArgs<float> mdense_args<float>(lapgpu_ctx* c) { return {c->D32, c->m.dvec32, c->m.dpart32}; }
+
+// k splits of the multi-RHS dense product: enough workgroups for the chip
+// (about four per CU) with at least two slabs per split
+int dense_splits(const lapgpu_ctx* c) {
+  const int64_t tiles = (c->plan.dense.K + kMDenseRows - 1) / kMDenseRows;
+  int64_t ns = std::max<int64_t>(1, (1024 + tiles - 1) / tiles);
+  ns = std::min<int64_t>(ns, std::max<int64_t>(1, c->ldd / (2 * kMDenseSlab)));
+  return (int)std::min<int64_t>(ns, 16);
+}
+
 // Workspace for n * K elements per vector (kept while it is large enough);
 // FP32 buffers only in the FP32-inner mode.
 int multi_prepare(lapgpu_ctx* c, int K) {
   auto& m = c->m;
   const int64_t n = c->n;
-  const int64_t NT = std::max<int64_t>(c->plan.tail.NT, 1);
   int rc;
   if (!m.part) {
     if ((rc = dev_alloc(c, &m.part, (size_t)kNumSlots * kMRed * kMultiMax))) return rc;
@@ -888,7 +1026,11 @@ int multi_prepare(lapgpu_ctx* c, int K) {
     double** ws[] = {&m.r, &m.x, &m.bestx, &m.p, &m.q, &m.y, &m.z, &m.io};
     for (double** w : ws)
       if ((rc = dev_alloc(c, w, (size_t)need))) return rc;
-    if ((rc = dev_alloc(c, &m.ext, (size_t)(NT * K)))) return rc;
+    if ((rc = dev_zeros(c, &m.ext, ext_len(c) * K))) return rc;
+    if (c->plan.dense.K > 0) {
+      if ((rc = dev_zeros(c, &m.dvec, (size_t)c->ldd * K))) return rc;
+      if ((rc = dev_alloc(c, &m.dpart, (size_t)dense_splits(c) * c->plan.dense.K * K))) return rc;
+    }
     m.cap = need;
   }
   if (c->precision == LAPGPU_FP32_INNER && need > m.cap32) {
@@ -896,8 +1038,12 @@ int multi_prepare(lapgpu_ctx* c, int K) {
     float** ws[] = {&m.y32, &m.z32, &m.pf};
     for (float** w : ws)
       if ((rc = dev_alloc(c, w, (size_t)need))) return rc;
-    if ((rc = dev_alloc(c, &m.ext32, (size_t)(NT * K)))) return rc;
+    if ((rc = dev_zeros(c, &m.ext32, ext_len(c) * K))) return rc;
     if ((rc = dev_alloc(c, &m.r0, (size_t)need))) return rc;
+    if (c->plan.dense.K > 0) {
+      if ((rc = dev_zeros(c, &m.dvec32, (size_t)c->ldd * K))) return rc;
+      if ((rc = dev_alloc(c, &m.dpart32, (size_t)dense_splits(c) * c->plan.dense.K * K))) return rc;
+    }
     m.cap32 = need;
   }
   return 0;
@@ -959,10 +1105,32 @@ void msweeps_t(lapgpu_ctx* c, const double* rin, T* y, T* z, T* ext, const T* gF
                        c->stream, (int)TP.T0, (int)TP.NT, c->Eptr, c->Eidx, Ew, rin, y, ext,
                        m.flags);
     if (c->prof_sweeps) (void)hipEventRecord(c->ev[6], c->stream);
-    LAPGPU_DISPATCH_E(TP.elog, hipLaunchKernelGGL, (k_mtail<K, T, S, E_>), dim3(K),
-                      dim3(kTailThreads), 0, c->stream, (int)TP.T0, (int)TP.NT, ext, sv, z,
-                      c->fsteps, (int)TP.fsteps.size(), fslots, c->bsteps,
-                      (int)TP.bsteps.size(), bslots, m.st, m.flags);
+    const DenseBlock& DB = P.dense;
+    if (DB.K == 0) {
+      LAPGPU_DISPATCH_E(TP.elog, hipLaunchKernelGGL, (k_mtail<K, T, S, E_>), dim3(K),
+                        dim3(kTailThreads), 0, c->stream, (int)TP.T0, (int)TP.NT, ext, sv, z,
+                        c->fsteps, (int)TP.fsteps.size(), fslots, c->bsteps,
+                        (int)TP.bsteps.size(), bslots, m.st, m.flags);
+    } else {
+      const MDenseArgs<T> da = mdense_args<T>(c);
+      const int NT1 = (int)TP.NT1;
+      T* u = NT1 > 0 ? da.dvec : ext;
+      if (NT1 > 0)
+        LAPGPU_DISPATCH_E(TP.elog, hipLaunchKernelGGL, (k_mtail_fwd<K, T, S, E_>), dim3(K),
+                          dim3(kTailThreads), 0, c->stream, (int)TP.T0, (int)TP.NT, NT1, ext, sv,
+                          z, da.dvec, c->fsteps, (int)TP.fsteps.size(), fslots, m.st, m.flags);
+      const int tiles = (int)((DB.K + kMDenseRows - 1) / kMDenseRows);
+      const int nsplit = dense_splits(c);
+      const long len = (long)((c->ldd / nsplit + kMDenseSlab - 1) / kMDenseSlab * kMDenseSlab);
+      hipLaunchKernelGGL((k_mdense<K, T>), dim3(tiles, nsplit), dim3(kBlock), 0, c->stream,
+                         (int)DB.K, (long)c->ldd, da.D, u, da.part, len, m.flags);
+      hipLaunchKernelGGL((k_mdense_sum<K, T>), dim3(kMRed), dim3(kBlock), 0, c->stream, (int)DB.K,
+                         nsplit, da.part, z + (size_t)DB.T1 * K, m.flags);
+      if (NT1 > 0)
+        LAPGPU_DISPATCH_E(TP.elog, hipLaunchKernelGGL, (k_mtail_bwd<K, T, S, E_>), dim3(K),
+                          dim3(kTailThreads), 0, c->stream, (int)TP.T0, (int)TP.NT, NT1, z,
+                          c->bsteps, (int)TP.bsteps.size(), bslots, m.st, m.flags);
+    }
   } else if (c->prof_sweeps) {
     (void)hipEventRecord(c->ev[6], c->stream);
   }
@@ -1446,6 +1614,8 @@ int lapgpu_create(lapgpu_ctx** out, int device) {
   if (const char* bd = getenv("LAPGPU_BLOCK_DEPTH")) c->block_depth = atoi(bd);
   if (const char* bg = getenv("LAPGPU_BLOCK_GROWTH")) c->block_growth = atof(bg);
   if (const char* be = getenv("LAPGPU_BLOCK_EXTRA")) c->block_extra = atoll(be);
+  if (const char* dm = getenv("LAPGPU_DENSE_MAX"))
+    c->dense_max = std::max<int64_t>(0, std::min<int64_t>(kTailCap, atoll(dm)));
   *out = c;
   return 0;
 }
@@ -1561,7 +1731,9 @@ int lapgpu_upload_ldli(lapgpu_ctx* c, int64_t n, int64_t nnzF,
   HIPC(c, hipSetDevice(c->device));
   std::string err;
   Plan P;
-  if (!build_plan(c->L, nnzF, col, colptr, rowval, fval, d, index_base, c->max_tail_steps, P, err))
+  DenseModel dm;
+  dm.max_k = c->dense_max;
+  if (!build_plan(c->L, nnzF, col, colptr, rowval, fval, d, index_base, c->max_tail_steps, P, err, dm))
     return set_err(c, -1, err);
   c->plan = std::move(P);
   int rc = upload_plan(c);
@@ -1742,6 +1914,25 @@ int lapgpu_set_group_budget(lapgpu_ctx* c, int max_depth, double max_growth,
     HIPC(c, hipStreamSynchronize(c->stream));
   }
   c->err.clear();
+  return 0;
+}
+
+int lapgpu_set_dense_tail(lapgpu_ctx* c, int64_t max_k) {
+  if (!c) return -1;
+  if (max_k < 0 || max_k > kTailCap)
+    return set_err(c, -1, "set_dense_tail: max_k must be in [0, " + std::to_string(kTailCap) + "]");
+  c->dense_max = max_k;
+  c->err.clear();
+  return 0;
+}
+
+int lapgpu_dense_info(lapgpu_ctx* c, int64_t* k, int64_t* sparse_vertices, int64_t* bytes) {
+  if (!c) return -1;
+  if (!c->has_F) return set_err(c, -2, "dense_info: no LDLinv uploaded");
+  const int64_t K = c->plan.dense.K;
+  if (k) *k = K;
+  if (sparse_vertices) *sparse_vertices = c->plan.tail.NT1;
+  if (bytes) *bytes = K * c->ldd * (int64_t)(sizeof(double) + (c->D32 ? sizeof(float) : 0));
   return 0;
 }
 

@@ -286,6 +286,137 @@ k_tail(int T0, int NT, const T* __restrict__ ext, const T* __restrict__ s,
   for (int i = threadIdx.x; i < NT; i += kTailThreads) yF[T0 + i] = ly[i];
 }
 
+// ---------------------------------------------------------------------------
+// The tail's dense block [T1, n) (see DenseBlock in lapgpu_plan.hpp): its
+// map from the forward inputs u to yF is the symmetric matrix D, stored
+// row-major with leading dimension ldd (a multiple of 256, columns K..ldd
+// zero).  A sweep with a dense block runs k_tail_fwd (the levels before T1,
+// then the block's inputs), the product yF[T1..n) = D u, and k_tail_bwd (the
+// levels before T1, backward).
+
+// Column j = j0 + blockIdx.x of D: the unit vector e_j through the block's
+// forward program, scaling and backward program, in LDS (the sweep's own
+// arithmetic); stored as row j (D is symmetric).  Every vertex's mass
+// reaches the root (the block's last vertex), so D holds s_root 1 1^T; that
+// term only adds the constant s_root sum(u) to every yF (the backward rows
+// are convex combinations, so it reaches every vertex unchanged), which the
+// mean removal of LDLsolver cancels.  It is left out: it is D's largest part
+// and cancels in D u, so D u keeps only the rounding of the rest.
+template <int E>
+__global__ void __launch_bounds__(kTailThreads)
+k_dense_basis(int K, int T1, int j0, const double* __restrict__ s,
+              const TailStepD* __restrict__ fsteps, int nfsteps,
+              const TailSlotD* __restrict__ fslots, const TailStepD* __restrict__ bsteps,
+              int nbsteps, const TailSlotD* __restrict__ bslots, double* __restrict__ D,
+              long ldd) {
+  __shared__ double ly[kTailCapLds];
+  const int j = j0 + blockIdx.x;
+  if (j >= K) return;  // the whole workgroup
+  for (int i = threadIdx.x; i < K; i += kTailThreads) ly[i] = i == j ? 1.0 : 0.0;
+  __syncthreads();
+  run_program<tail_depth<E>(), true, E>(fsteps, nfsteps, fslots, ly);
+  __syncthreads();
+  for (int i = threadIdx.x; i < K; i += kTailThreads) ly[i] *= s[T1 + i];
+  __syncthreads();
+  run_program<tail_depth<E>(), true, E>(bsteps, nbsteps, bslots, ly);
+  __syncthreads();
+  const double c = s[T1 + K - 1];  // the root's scale
+  double* __restrict__ row = D + (size_t)j * ldd;
+  for (int i = threadIdx.x; i < ldd; i += kTailThreads) row[i] = i < K ? ly[i] - c : 0.0;
+}
+
+// FP32 copy of D (FP32-inner mode)
+__global__ void k_narrow(size_t count, const double* __restrict__ src, float* __restrict__ dst) {
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += (size_t)gridDim.x * kBlock)
+    dst[i] = (float)src[i];
+}
+
+// Tail forward with a dense block: the levels of [T0, T1) and the block's
+// inputs; writes s * y0 of [T0, T1) to yF and the block's inputs to u[0..K)
+template <typename T, typename S, int E>
+__global__ void __launch_bounds__(kTailThreads)
+k_tail_fwd(int T0, int NT, int NT1, const T* __restrict__ ext, const T* __restrict__ s,
+           T* __restrict__ yF, T* __restrict__ u, const TailStepD* __restrict__ fsteps,
+           int nfsteps, const S* __restrict__ fslots, const DevState* __restrict__ st) {
+  __shared__ T ly[kTailCapLds];
+  if (st->stopped) return;
+  for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] = ext[i];
+  __syncthreads();
+  run_program<tail_depth<E>(), true, E>(fsteps, nfsteps, fslots, ly);
+  __syncthreads();
+  for (int i = threadIdx.x; i < NT1; i += kTailThreads) yF[T0 + i] = ly[i] * s[T0 + i];
+  for (int i = NT1 + threadIdx.x; i < NT; i += kTailThreads) u[i - NT1] = ly[i];
+}
+
+// Tail backward with a dense block: yF of [T0, T1) from its scaled y0 and
+// the block's results yF[T1..n)
+template <typename T, typename S, int E>
+__global__ void __launch_bounds__(kTailThreads)
+k_tail_bwd(int T0, int NT, int NT1, T* __restrict__ yF, const TailStepD* __restrict__ bsteps,
+           int nbsteps, const S* __restrict__ bslots, const DevState* __restrict__ st) {
+  __shared__ T ly[kTailCapLds];
+  if (st->stopped) return;
+  for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] = yF[T0 + i];
+  __syncthreads();
+  run_program<tail_depth<E>(), true, E>(bsteps, nbsteps, bslots, ly);
+  __syncthreads();
+  for (int i = threadIdx.x; i < NT1; i += kTailThreads) yF[T0 + i] = ly[i];
+}
+
+template <typename T>
+struct Vec16;  // 16 bytes of T
+template <>
+struct Vec16<double> {
+  using type = double2;
+  static constexpr int n = 2;
+  __device__ static double dot(const double2& a, const double2& b) { return a.x * b.x + a.y * b.y; }
+};
+template <>
+struct Vec16<float> {
+  using type = float4;
+  static constexpr int n = 4;
+  __device__ static double dot(const float4& a, const float4& b) {
+    return (double)a.x * b.x + (double)a.y * b.y + (double)a.z * b.z + (double)a.w * b.w;
+  }
+};
+
+// y[0..K) = D u (u zero beyond K up to ldd): kDenseRows rows per wave, each
+// lane a strided share of 16-byte column chunks, summed in double in a fixed
+// order (lane chunks in column order, then the wave's butterfly).
+constexpr int kDenseRows = 8;
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_dense(int K, long ldd, const T* __restrict__ D, const T* __restrict__ u, T* __restrict__ y,
+        const DevState* __restrict__ st) {
+  if (st->stopped) return;
+  using V = Vec16<T>;
+  using VT = typename V::type;
+  const int lane = threadIdx.x % kWave;
+  const int r0 = ((blockIdx.x * kBlock + threadIdx.x) / kWave) * kDenseRows;
+  if (r0 >= K) return;
+  const int nr = min(kDenseRows, K - r0);
+  double acc[kDenseRows];
+#pragma unroll
+  for (int r = 0; r < kDenseRows; ++r) acc[r] = 0.0;
+  const VT* __restrict__ uv = reinterpret_cast<const VT*>(u);
+  const long nv = ldd / V::n;
+  for (long c = lane; c < nv; c += kWave) {
+    const VT x = uv[c];
+    VT d[kDenseRows];
+#pragma unroll
+    for (int r = 0; r < kDenseRows; ++r)
+      if (r < nr) d[r] = reinterpret_cast<const VT*>(D + (size_t)(r0 + r) * ldd)[c];
+#pragma unroll
+    for (int r = 0; r < kDenseRows; ++r)
+      if (r < nr) acc[r] += V::dot(d[r], x);
+  }
+#pragma unroll
+  for (int r = 0; r < kDenseRows; ++r) {
+    const double t = group_sum<kWave>(acc[r]);
+    if (lane == 0 && r < nr) y[r0 + r] = (T)t;
+  }
+}
+
 // partial sum of y (for the mean removal of LDLsolver)
 template <typename T>
 __global__ void k_vsum(int n, const T* __restrict__ y,

@@ -422,6 +422,139 @@ k_mtail(int T0, int NT, const T* __restrict__ ext, const T* __restrict__ s, T* _
   for (int i = threadIdx.x; i < NT; i += kTailThreads) z[(int64_t)(T0 + i) * K + j] = ly[i];
 }
 
+// With a dense block (k_tail_fwd / k_tail_bwd for K columns): the tail
+// forward for column j = blockIdx.x writes s * y0 of [T0, T1) to z and the
+// block's inputs to u (row i of the block at u[i*K + j]) ...
+template <int K, typename T, typename S, int E>
+__global__ void __launch_bounds__(kTailThreads)
+k_mtail_fwd(int T0, int NT, int NT1, const T* __restrict__ ext, const T* __restrict__ s,
+            T* __restrict__ z, T* __restrict__ u, const TailStepD* __restrict__ fsteps,
+            int nfsteps, const S* __restrict__ fslots, const DevState* __restrict__ mst,
+            const int* __restrict__ flags) {
+  __shared__ T ly[kTailCapLds];
+  const int j = blockIdx.x;
+  if (flags[0] || mst[j].stopped) return;
+  for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] = ext[(int64_t)i * K + j];
+  __syncthreads();
+  run_program<tail_depth<E>(), true, E>(fsteps, nfsteps, fslots, ly);
+  __syncthreads();
+  for (int i = threadIdx.x; i < NT1; i += kTailThreads)
+    z[(int64_t)(T0 + i) * K + j] = ly[i] * s[T0 + i];
+  for (int i = NT1 + threadIdx.x; i < NT; i += kTailThreads) u[(int64_t)(i - NT1) * K + j] = ly[i];
+}
+
+// ... and the backward reads them back with the block's results
+template <int K, typename T, typename S, int E>
+__global__ void __launch_bounds__(kTailThreads)
+k_mtail_bwd(int T0, int NT, int NT1, T* __restrict__ z, const TailStepD* __restrict__ bsteps,
+            int nbsteps, const S* __restrict__ bslots, const DevState* __restrict__ mst,
+            const int* __restrict__ flags) {
+  __shared__ T ly[kTailCapLds];
+  const int j = blockIdx.x;
+  if (flags[0] || mst[j].stopped) return;
+  for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] = z[(int64_t)(T0 + i) * K + j];
+  __syncthreads();
+  run_program<tail_depth<E>(), true, E>(bsteps, nbsteps, bslots, ly);
+  __syncthreads();
+  for (int i = threadIdx.x; i < NT1; i += kTailThreads) z[(int64_t)(T0 + i) * K + j] = ly[i];
+}
+
+// The dense block for K columns on the matrix cores: Y = D U with D (Kd x
+// Kd, leading dimension ldd) and U (rows of K, zero from row Kd to ldd).
+// Workgroup (row tile rt, k split ks) computes rows [64 rt, 64 rt + 64) over
+// columns [ks len, ks len + len) of D into partial ks: each of its four
+// waves owns 16 rows and holds K/16 (at least one) 16x16 accumulator tiles;
+// per 64-column slab the slab's U rows are staged in LDS, and each 4-wide
+// k step is one 16x16x4 MFMA per tile (A: D[row l%16][k + l/16] from global,
+// B: U[k + l/16][col l%16] from LDS).  The partials are added in split order
+// by k_mdense_sum, so the result does not depend on scheduling.
+typedef double dense_d4 __attribute__((ext_vector_type(4)));
+typedef float dense_f4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  using acc_t = dense_d4;
+  // output item v of lane l: row l/16 + 4 v, column l%16
+  __device__ static int row(int l, int v) { return l / 16 + 4 * v; }
+  __device__ static acc_t step(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<float> {
+  using acc_t = dense_f4;
+  // output item v of lane l: row 4 (l/16) + v, column l%16
+  __device__ static int row(int l, int v) { return 4 * (l / 16) + v; }
+  __device__ static acc_t step(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int kMDenseRows = 64;   // rows per workgroup (16 per wave)
+constexpr int kMDenseSlab = 64;   // columns of D per LDS stage
+
+template <int K, typename T>
+__global__ void __launch_bounds__(kBlock)
+k_mdense(int Kd, long ldd, const T* __restrict__ D, const T* __restrict__ U,
+         T* __restrict__ part, long len, const int* __restrict__ flags) {
+  if (flags[0]) return;
+  constexpr int NTILE = K >= 16 ? K / 16 : 1;
+  constexpr int KC = NTILE * 16;  // columns covered by the tiles (K = 8: 16, the upper half zero)
+  __shared__ T us[kMDenseSlab][KC];
+  using M = Mfma<T>;
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  const int r0 = blockIdx.x * kMDenseRows + w * 16;
+  const long kb = (long)blockIdx.y * len, ke = min(kb + len, ldd);
+  typename M::acc_t acc[NTILE];
+#pragma unroll
+  for (int t = 0; t < NTILE; ++t) acc[t] = typename M::acc_t{0, 0, 0, 0};
+  const int arow = r0 + lane % 16;
+  const bool arow_ok = arow < Kd;
+  const T* __restrict__ Drow = D + (size_t)(arow_ok ? arow : 0) * ldd;
+  for (long k0 = kb; k0 < ke; k0 += kMDenseSlab) {
+    // stage U rows [k0, k0 + 64) (ldd is a multiple of 64: whole slabs)
+    for (int e = threadIdx.x; e < kMDenseSlab * KC; e += kBlock) {
+      const int kk = e / KC, col = e % KC;
+      us[kk][col] = col < K ? U[(size_t)(k0 + kk) * K + col] : (T)0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int s = 0; s < kMDenseSlab / 4; ++s) {
+      const int kk = 4 * s + lane / 16;
+      const T a = arow_ok ? Drow[k0 + kk] : (T)0;
+#pragma unroll
+      for (int t = 0; t < NTILE; ++t) acc[t] = M::step(a, us[kk][16 * t + lane % 16], acc[t]);
+    }
+    __syncthreads();
+  }
+  T* __restrict__ P = part + (size_t)blockIdx.y * Kd * K;
+#pragma unroll
+  for (int t = 0; t < NTILE; ++t) {
+    const int col = 16 * t + lane % 16;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = r0 + M::row(lane, v);
+      if (row < Kd && col < K) P[(size_t)row * K + col] = acc[t][v];
+    }
+  }
+}
+
+// y[i*K + j] = sum of the nsplit partials, in split order
+template <int K, typename T>
+__global__ void __launch_bounds__(kBlock)
+k_mdense_sum(int Kd, int nsplit, const T* __restrict__ part, T* __restrict__ y,
+             const int* __restrict__ flags) {
+  if (flags[0]) return;
+  const size_t cnt = (size_t)Kd * K;
+  for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < cnt; e += (size_t)gridDim.x * kBlock) {
+    T acc = part[e];
+    for (int s = 1; s < nsplit; ++s) acc += part[(size_t)s * cnt + e];
+    y[e] = acc;
+  }
+}
+
 // partial sums of yF (mean removal of LDLsolver)
 template <int K, typename T>
 __global__ void __launch_bounds__(kBlock)

@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <condition_variable>
 #include <cstdlib>
 #include <cstring>
@@ -304,7 +305,8 @@ void pack_level(std::vector<RowEntries>& rows, int elog, int64_t lanes,
 
 }  // namespace
 
-bool build_tail(Plan& P, int64_t max_steps, int64_t cap, int max_elog, int lanes) {
+bool build_tail(Plan& P, int64_t max_steps, int64_t cap, int max_elog, int lanes,
+                const DenseModel& dm) {
   const int64_t n = P.n;
   // extent: the longest suffix of levels within cap whose levels need at
   // most max_steps steps of kExtentLanes lanes at one entry per lane
@@ -327,9 +329,11 @@ bool build_tail(Plan& P, int64_t max_steps, int64_t cap, int max_elog, int lanes
   }
   TailProgram& T = P.tail;
   T = TailProgram();
+  P.dense = DenseBlock();
   P.tail_level = t;
   T.T0 = P.lvl[t];
   T.NT = n - T.T0;
+  T.NT1 = T.NT;
   const int32_t T0 = (int32_t)T.T0;
   T.Eptr.assign(T.NT + 1, 0);
   int64_t rows = 0, ent = 0;
@@ -349,42 +353,141 @@ bool build_tail(Plan& P, int64_t max_steps, int64_t cap, int max_elog, int lanes
     ent += fin + bin;
   }
   T.elog = choose_elog(rows, ent, max_elog);
-  // levels pack independently (each into its own steps and slots, offsets
-  // local), then concatenate in sweep order
   struct LevelProg {
     std::vector<TailStep> steps;
     std::vector<TailSlot> slots;
   };
-  const int64_t nl = P.nlev - t;
-  std::vector<LevelProg> fp(nl), bp(nl);
-  std::atomic<bool> ok{true};
   PlanThreads threads;
   Team team(threads.n);
-  team.for_each(2 * nl, 4, [&](int64_t lo, int64_t hi) {
-    std::vector<RowEntries> rws;
-    for (int64_t q = lo; q < hi; ++q) {
-      const bool fwd = q < nl;
-      const int64_t l = t + (fwd ? q : q - nl);
-      rws.clear();
-      for (int64_t pos = P.lvl[l]; pos < P.lvl[l + 1]; ++pos) {
-        RowEntries R;
-        R.row = (int32_t)(pos - T0);
-        if (fwd) {
-          for (int32_t k = P.Fptr[pos]; k < P.Fptr[pos + 1]; ++k)
-            if (P.Fidx[k] >= T0) R.e.emplace_back(P.Fidx[k] - T0, P.Fw[k]);
-        } else {
-          for (int32_t k = P.Bptr[pos]; k < P.Bptr[pos + 1]; ++k) {
-            if (P.Bidx[k] < T0) ok = false;
-            R.e.emplace_back(P.Bidx[k] - T0, P.Bw[k]);
+  // programs of levels [l0, nlev) with indices local to `base`: forward rows
+  // gather their in-edges from [base, n), backward rows all their entries
+  // (every level packs on its own, offsets local)
+  std::atomic<bool> ok{true};
+  auto pack_levels = [&](int64_t l0, int64_t base, int elog, std::vector<LevelProg>& fp,
+                         std::vector<LevelProg>& bp) {
+    const int64_t nl = P.nlev - l0;
+    fp.assign(nl, LevelProg());
+    bp.assign(nl, LevelProg());
+    team.for_each(2 * nl, 4, [&](int64_t lo, int64_t hi) {
+      std::vector<RowEntries> rws;
+      for (int64_t q = lo; q < hi; ++q) {
+        const bool fwd = q < nl;
+        const int64_t l = l0 + (fwd ? q : q - nl);
+        rws.clear();
+        for (int64_t pos = P.lvl[l]; pos < P.lvl[l + 1]; ++pos) {
+          RowEntries R;
+          R.row = (int32_t)(pos - base);
+          if (fwd) {
+            for (int32_t k = P.Fptr[pos]; k < P.Fptr[pos + 1]; ++k)
+              if (P.Fidx[k] >= base) R.e.emplace_back((int32_t)(P.Fidx[k] - base), P.Fw[k]);
+          } else {
+            for (int32_t k = P.Bptr[pos]; k < P.Bptr[pos + 1]; ++k) {
+              if (P.Bidx[k] < base) ok = false;
+              R.e.emplace_back((int32_t)(P.Bidx[k] - base), P.Bw[k]);
+            }
           }
+          rws.push_back(std::move(R));
         }
-        rws.push_back(std::move(R));
+        LevelProg& G = fwd ? fp[l - l0] : bp[l - l0];
+        pack_level(rws, elog, lanes, G.steps, G.slots);
       }
-      LevelProg& G = fwd ? fp[l - t] : bp[l - t];
-      pack_level(rws, T.elog, lanes, G.steps, G.slots);
+    });
+  };
+  std::vector<LevelProg> fp, bp;
+  pack_levels(t, T0, T.elog, fp, bp);
+  const int64_t nl = P.nlev - t;
+
+  // ---- the dense block: the split level by the cost model -----------------
+  int64_t L1 = P.nlev;  // first level of the dense block (nlev: none)
+  LevelProg bnd;        // the block's inputs from [T0, T1), one more level
+  if (dm.max_k > 0 && nl > 0) {
+    const double E = (double)(int64_t(1) << T.elog);
+    std::vector<int64_t> steps(nl + 1, 0);  // steps of levels [t, t + q)
+    for (int64_t q = 0; q < nl; ++q)
+      steps[q + 1] = steps[q] + (int64_t)(fp[q].steps.size() + bp[q].steps.size());
+    double best = (double)steps[nl] * dm.step_us;
+    // entries from the sparse part [T0, T1) into the block, kept as the
+    // split moves down one level at a time
+    int64_t cross = 0;
+    double smin = INFINITY, smax = 0.0;
+    for (int64_t l = P.nlev - 1; l >= t; --l) {
+      const int64_t lo = P.lvl[l], hi = P.lvl[l + 1];
+      const int64_t K = n - lo;
+      if (K > dm.max_k) break;
+      for (int64_t pos = lo; pos < hi; ++pos) {
+        const double sv = std::fabs(P.s[pos]);
+        smin = std::min(smin, sv);
+        smax = std::max(smax, sv);
+      }
+      if (!(smax <= dm.max_spread * smin)) break;
+      // level l joins the block: its out-edges into the block stop crossing,
+      // its in-edges from [T0, lo) start to
+      for (int64_t pos = lo; pos < hi; ++pos) {
+        for (int32_t k = P.Bptr[pos]; k < P.Bptr[pos + 1]; ++k)
+          if (P.Bidx[k] >= hi) --cross;
+        for (int32_t k = P.Fptr[pos]; k < P.Fptr[pos + 1]; ++k)
+          if (P.Fidx[k] >= T0 && P.Fidx[k] < lo) ++cross;
+      }
+      const int64_t q = l - t;  // sparse levels [t, l)
+      const double bsteps = std::ceil((double)cross / (E * (double)lanes * 0.7));
+      const double cost = ((double)steps[q] + bsteps) * dm.step_us +
+                          8.0 * (double)K * (double)K / (dm.gbps * 1e3) +
+                          (q > 0 ? 2.0 * dm.launch_us : 0.0);
+      if (cost < best) {
+        best = cost;
+        L1 = l;
+      }
     }
-  });
-  // forward sweep: levels ascending; backward: descending
+  }
+  const int64_t T1 = P.lvl[L1];
+  T.NT1 = T1 - T0;
+  if (L1 < P.nlev) {
+    // the block's inputs from the sparse part, packed as one more level
+    std::vector<RowEntries> rws;
+    for (int64_t pos = T1; pos < n; ++pos) {
+      RowEntries R;
+      R.row = (int32_t)(pos - T0);
+      for (int32_t k = P.Fptr[pos]; k < P.Fptr[pos + 1]; ++k)
+        if (P.Fidx[k] >= T0 && P.Fidx[k] < T1) R.e.emplace_back((int32_t)(P.Fidx[k] - T0), P.Fw[k]);
+      if (!R.e.empty()) rws.push_back(std::move(R));
+    }
+    if (!rws.empty()) pack_level(rws, T.elog, lanes, bnd.steps, bnd.slots);
+    // the block's own programs, local to T1
+    DenseBlock& D = P.dense;
+    D.T1 = T1;
+    D.K = n - T1;
+    D.level = L1;
+    int64_t drows = 0, dent = 0;
+    for (int64_t pos = T1; pos < n; ++pos) {
+      int64_t fin = 0;
+      for (int32_t k = P.Fptr[pos]; k < P.Fptr[pos + 1]; ++k) fin += P.Fidx[k] >= T1;
+      const int64_t bin = P.Bptr[pos + 1] - P.Bptr[pos];
+      drows += (fin > 0) + (bin > 0);
+      dent += fin + bin;
+    }
+    D.elog = choose_elog(drows, dent, max_elog);
+    std::vector<LevelProg> dfp, dbp;
+    pack_levels(L1, T1, D.elog, dfp, dbp);
+    for (size_t q = 0; q < dfp.size(); ++q) {
+      const int32_t b0 = (int32_t)D.fslots.size();
+      for (TailStep st : dfp[q].steps) {
+        st.off += b0;
+        D.fsteps.push_back(st);
+      }
+      D.fslots.insert(D.fslots.end(), dfp[q].slots.begin(), dfp[q].slots.end());
+    }
+    for (size_t q = dbp.size(); q-- > 0;) {
+      const int32_t b0 = (int32_t)D.bslots.size();
+      for (TailStep st : dbp[q].steps) {
+        st.off += b0;
+        D.bsteps.push_back(st);
+      }
+      D.bslots.insert(D.bslots.end(), dbp[q].slots.begin(), dbp[q].slots.end());
+    }
+  }
+  // forward sweep: the sparse levels ascending (then the block's inputs);
+  // backward: the sparse levels descending
+  const int64_t ns = L1 - t;
   auto append = [](const LevelProg& G, std::vector<TailStep>& steps,
                    std::vector<TailSlot>& slots) {
     const int32_t base = (int32_t)slots.size();
@@ -394,8 +497,8 @@ bool build_tail(Plan& P, int64_t max_steps, int64_t cap, int max_elog, int lanes
     }
     slots.insert(slots.end(), G.slots.begin(), G.slots.end());
   };
-  size_t fs = 0, fl = 0, bs = 0, bl = 0;
-  for (int64_t q = 0; q < nl; ++q) {
+  size_t fs = bnd.steps.size(), fl = bnd.slots.size(), bs = 0, bl = 0;
+  for (int64_t q = 0; q < ns; ++q) {
     fs += fp[q].steps.size();
     fl += fp[q].slots.size();
     bs += bp[q].steps.size();
@@ -405,8 +508,9 @@ bool build_tail(Plan& P, int64_t max_steps, int64_t cap, int max_elog, int lanes
   T.fslots.reserve(fl);
   T.bsteps.reserve(bs);
   T.bslots.reserve(bl);
-  for (int64_t q = 0; q < nl; ++q) append(fp[q], T.fsteps, T.fslots);
-  for (int64_t q = nl - 1; q >= 0; --q) append(bp[q], T.bsteps, T.bslots);
+  for (int64_t q = 0; q < ns; ++q) append(fp[q], T.fsteps, T.fslots);
+  append(bnd, T.fsteps, T.fslots);
+  for (int64_t q = ns - 1; q >= 0; --q) append(bp[q], T.bsteps, T.bslots);
   return ok.load();
 }
 
@@ -667,7 +771,8 @@ void build_blocks(Plan& P, int max_depth, double growth, int64_t max_extra) {
 bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
                 const int64_t* colptr, const int64_t* rowval,
                 const double* fval, const double* d, int base,
-                int64_t max_tail_steps, Plan& P, std::string& err) {
+                int64_t max_tail_steps, Plan& P, std::string& err,
+                const DenseModel& dense) {
   const int64_t n = L.n;
   if (n < 1) {
     err = "upload_ldli: upload the Laplacian first";
@@ -898,7 +1003,7 @@ bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
       }
     });
   }
-  if (!build_tail(P, max_tail_steps, kTailCap)) {
+  if (!build_tail(P, max_tail_steps, kTailCap, kTailMaxElog, kTailLanes, dense)) {
     err = "upload_ldli: internal error packing the tail program";
     return false;
   }

@@ -63,14 +63,49 @@ constexpr int kExtentLanes = 1024;
 #endif
 constexpr int64_t kTailCap = LAPGPU_TAIL_CAP;  // tail vertices held in LDS (147 KiB)
 
+// The tail [T0, n) is run by one workgroup.  With a dense block, its last
+// n - T1 vertices (T1 = T0 + NT1, a level boundary) are applied as one
+// matrix instead (DenseBlock): fsteps then run the levels of [T0, T1) and one
+// more level that gathers the block's inputs from [T0, T1), and bsteps the
+// levels of [T0, T1), whose rows may read the block's results.
 struct TailProgram {
   int64_t T0 = 0, NT = 0;
-  int elog = 0;  // every step of both sweeps has 2^elog entries per lane
+  int64_t NT1 = 0;  // vertices before the dense block (NT without one)
+  int elog = 0;     // every step of both sweeps has 2^elog entries per lane
   std::vector<TailStep> fsteps, bsteps;
   std::vector<TailSlot> fslots, bslots;
   // forward in-edges of tail rows from outside the tail (CSR over tail rows)
   std::vector<int32_t> Eptr, Eidx;
   std::vector<double> Ew;
+};
+
+// The dense block [T1, n) of the tail: its map from the forward inputs u
+// (r plus every forward in-edge from before T1) to yF is the symmetric
+// matrix D = (I - W^T)^-1 S (I - W)^-1 of the block's own levels, stored
+// less its constant part s_root 1 1^T (which only shifts yF by a constant
+// that the mean removal cancels).  Its programs (indices local to T1) are
+// run once per unit vector to build D.
+struct DenseBlock {
+  int64_t T1 = 0, K = 0;  // K = 0: no dense block
+  int64_t level = 0;      // first level of the block
+  int elog = 0;
+  std::vector<TailStep> fsteps, bsteps;
+  std::vector<TailSlot> fslots, bslots;
+};
+
+// Choice of the dense block (build_tail): the split level minimising
+// (steps left to the slot programs) x step_us + 8 K^2 bytes / gbps (+ two
+// launches when a sparse part remains), with K <= max_k (0: never dense).
+struct DenseModel {
+  int64_t max_k = 0;
+  // D u adds terms of size s_max |u| that cancel down to the result, while
+  // the sweeps difference first and scale after: a block whose diagonal
+  // scales s span more than max_spread (a rounding error of about eps x
+  // spread) is left to the sweeps
+  double max_spread = 1e12;
+  double step_us = 0.45;   // one dependent 512-lane step of the tail (FP64, measured)
+  double gbps = 5000.0;    // dense matrix-vector product streaming rate
+  double launch_us = 3.0;  // one more dependent launch
 };
 
 // Groups of consecutive wide levels computed by one launch each.  Row v of
@@ -112,6 +147,7 @@ struct Plan {
   int64_t nlev = 0;
   int64_t tail_level = 0;  // first level of the single-workgroup tail
   TailProgram tail;
+  DenseBlock dense;
   LevelBlocks blocks;
 };
 
@@ -124,14 +160,16 @@ bool csc_to_csr(int64_t n, const int64_t* colptr, const int64_t* rowval,
 bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
                 const int64_t* colptr, const int64_t* rowval,
                 const double* fval, const double* d, int base,
-                int64_t max_tail_steps, Plan& P, std::string& err);
+                int64_t max_tail_steps, Plan& P, std::string& err,
+                const DenseModel& dense = DenseModel());
 
 // (Re)build P.tail as the longest suffix of levels with at most `cap`
-// vertices whose levels need at most `max_steps` steps each.  With both
-// limits at INT64_MAX the program covers the whole sweep (tail_level 0).
-// Entries per lane are chosen for the whole program, at most 2^max_elog.
+// vertices whose levels need at most `max_steps` steps each, and P.dense by
+// `dense`.  With both limits at INT64_MAX the program covers the whole sweep
+// (tail_level 0).  Entries per lane are chosen for the whole program, at
+// most 2^max_elog.
 bool build_tail(Plan& P, int64_t max_steps, int64_t cap, int max_elog = kTailMaxElog,
-                int lanes = kTailLanes);
+                int lanes = kTailLanes, const DenseModel& dense = DenseModel());
 
 // Group the levels before the tail into launches of up to `max_depth`
 // levels while a group's composite rows stay within `growth` times the

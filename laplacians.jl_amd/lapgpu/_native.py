@@ -59,6 +59,8 @@ EXPORTS = (
     "lapgpu_autotune",
     "lapgpu_last_error",
     "lapgpu_destroy",
+    "lapgpu_set_dense_tail",
+    "lapgpu_dense_info",
     "lapgpu_batch_create",
     "lapgpu_batch_add_graph",
     "lapgpu_batch_remove_graph",
@@ -116,6 +118,8 @@ def load() -> ctypes.CDLL:
     L.lapgpu_destroy.restype = None
     pi32 = ctypes.POINTER(ctypes.c_int32)
     L.lapgpu_batch_create.argtypes = [ctypes.POINTER(P), ci]
+    L.lapgpu_set_dense_tail.argtypes = [P, i64]
+    L.lapgpu_dense_info.argtypes = [P, pi64, pi64, pi64]
     L.lapgpu_batch_add_graph.argtypes = [P, i64, P, P, P, i64, P, P, P, P, P, ci, pi32]
     L.lapgpu_batch_remove_graph.argtypes = [P, ctypes.c_int32]
     L.lapgpu_batch_graph_info.argtypes = [P, ctypes.c_int32, pi64, pi64]
@@ -364,6 +368,16 @@ class Context:
         upload_ldli; regroups an uploaded factor at once)."""
         self._check(self._lib.lapgpu_set_group_budget(self._h, int(max_depth), float(max_growth), int(max_extra)),
                     "lapgpu_set_group_budget")
+
+    def set_dense_tail(self, max_k: int) -> None:
+        """Largest dense tail block the planner may choose (0: none); takes
+        effect at the next upload_ldli."""
+        self._check(self._lib.lapgpu_set_dense_tail(self._h, int(max_k)), "lapgpu_set_dense_tail")
+
+    def dense_info(self) -> dict:
+        v = [ctypes.c_int64() for _ in range(3)]
+        self._check(self._lib.lapgpu_dense_info(self._h, *(ctypes.byref(t) for t in v)), "lapgpu_dense_info")
+        return {"k": v[0].value, "sparse_vertices": v[1].value, "bytes": v[2].value}
 
     def set_wait_mode(self, blocking: bool) -> None:
         """Wait for the device by sleeping (blocking=True) or spinning."""

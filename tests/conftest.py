@@ -34,8 +34,10 @@ def gpu_ctx_factory():
 
     made = []
 
-    def make(la, ldli):
+    def make(la, ldli, dense_max=None):
         ctx = lapgpu.Context(0)
+        if dense_max is not None:
+            ctx.set_dense_tail(dense_max)
         ctx.upload_lap(la)
         if ldli is not None:
             ctx.upload_ldli(ldli)

@@ -4,7 +4,9 @@ lane, as tests/test_plan_host.py checks on the host, so most rows are split
 into segments added with LDS atomics).  Bars as in the parity tests:
 one LDLinv apply to 1e-12 relative of the oracle's, solves with the same
 stop reason, iterations within 2 and true relres <= tol (tests/_bars.py), single and
-multi-RHS, FP64 and FP32-inner."""
+multi-RHS, FP64 and FP32-inner.  The dense tail block is switched off
+here, so the slot programs run every level (tests/test_gpu_dense.py covers
+the block on this graph)."""
 
 import numpy as np
 import pytest
@@ -30,7 +32,7 @@ def _rhs(n, seed):
 def test_apply_and_solve(case, gpu_ctx_factory, oracle_mod):
     la, f = case
     n = la.shape[0]
-    ctx = gpu_ctx_factory(la, f)
+    ctx = gpu_ctx_factory(la, f, dense_max=0)
     assert ctx.tail_info()["nt"] == n  # the whole sweep is the tail program
     r = _rhs(n, 1)
     z = ctx.apply_ldli(r)
@@ -46,7 +48,7 @@ def test_apply_and_solve(case, gpu_ctx_factory, oracle_mod):
 def test_multi_and_fp32(case, gpu_ctx_factory, oracle_mod):
     la, f = case
     n = la.shape[0]
-    ctx = gpu_ctx_factory(la, f)
+    ctx = gpu_ctx_factory(la, f, dense_max=0)
     B = np.stack([_rhs(n, 10 + j) for j in range(8)], axis=1)
     X, info = ctx.solve_multi(B, tol=1e-8, maxits=1000)
     _, io = oracle_mod.approxchol_solve(la, f, B[:, 3], tol=1e-8, maxits=1000, stag_test=5)
