@@ -1,23 +1,33 @@
 #!/usr/bin/env python3
-"""Benchmarks of the MI355X solve phase of approxchol_lap.
+"""Benchmarks of the MI355X solve phase of approxchol_lap.  Each mode prints
+ONE JSON line.
 
-Default (--mode plain): the plain FP64 GPU drop-in solve on a 3-D 128^3 grid
-Laplacian, n = 2,097,152, m = 6,242,304, edge weights exp(U(-2,2)) (numpy
-PCG64 seed 2), one RHS ~ N(0,1) from the same stream projected to sum zero,
-tol = 1e-8, maxits = 1000; the LDLinv is built once on the host (liblapfactor,
-seed 1) and uploaded.  A step is one full solve f(b) with b already resident
-in HBM.  `value` is solves/s.  Prints ONE JSON line.
+Default (--mode serve): the serving workload, a Poisson stream of solve
+requests at 200 QPS (seed 5) over 32 device-resident graphs with n
+log-uniform in [1e4, 4e6] (2-D / 3-D grids with exp(U(-2,2)) weights, path +
+4 matchings graphs with log-uniform weights), 20 % of the requests bringing
+a new graph of the same families (n up to 2e5) that is factored on the host
+and uploaded while the service runs; tol 1e-8, each right-hand side N(0,1)
+projected to sum zero.  A step is 500 requests of the stream: --warmup W
+steps of another stream (seed 55) over the same resident graphs run first,
+untimed, then --steps K steps (K x 500 requests; K = 20 is the 10,000-request
+stream) are timed.  `value` is served solves/s over the timed stream, with
+p50/p99 latency beside it, against plain one-at-a-time FP64 GPU PCG on the
+same GPU (a FIFO replay of the same arrivals with the measured plain solve
+time of every resident graph and a fit for the new graphs: modelled).  The
+same line carries the plain FP64 solve of a 3-D 128^3 grid (n = 2,097,152,
+exp(U(-2,2)) weights, seed 2): its per-kernel HBM roofline from HIP events
+on the library's stream and its CPU baseline (one full solve of the C
+restatement on one host core).
 
-Other modes (one JSON line each, same keys):
+Other modes (same keys):
+  plain      the 3-D 128^3 grid solve alone (--steps solves with b resident
+             in HBM); roofline and CPU baseline.
   multi      2-D 1024x1024 grid, exp(N(0,1)) weights, a 64-column RHS block
              (FP32-inner and FP64), next to 256 small unit-weight 2-D grids
-             (n log-uniform in [1e3, 1e5], aspect ratio in [1, 8]) solved by
-             the batched engine at the same time; seed 4.
+             solved by the batched engine at the same time; seed 4.
   precision  union of a path and 4 random matchings, n = 1e6, weights
              log-uniform in [1e-3, 1e3], seed 3: FP32-inner vs FP64.
-  serve      Poisson stream of requests over resident graphs (2-D/3-D grids
-             with exp(U(-2,2)) weights and path+matchings graphs, n
-             log-uniform) with a share that brings new graphs; seed 5.
 
 --gpus N > 1 exits: this build measures one GPU.
 """
@@ -39,55 +49,50 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+STEP_REQUESTS = 500
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def make_workload(name: str):
-    from lapgpu import graphs
+def emit(out: dict) -> None:
+    print(json.dumps(out), flush=True)
 
-    if name == "grid3_128":
-        rng = np.random.default_rng(2)
-        a = graphs.weighted_grid3(128, 128, 128, rng, "expU2")
-        b = rng.standard_normal(a.shape[0])
-    elif name == "grid3_64":
-        rng = np.random.default_rng(2)
-        a = graphs.weighted_grid3(64, 64, 64, rng, "expU2")
-        b = rng.standard_normal(a.shape[0])
-    elif name == "grid2_100":
-        a = graphs.grid2(100)
-        b = np.random.default_rng(0).standard_normal(a.shape[0])
-    else:
-        raise SystemExit(f"unknown workload {name}")
-    b = b - b.mean()
-    return a, b
+
+def host_info() -> dict:
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"nproc": os.cpu_count(), "usable_cpus": usable, "cpu_model": model}
 
 
 def algorithmic_bytes(n: int, nnzL: int, nnzF: int, sv: int = 8, si: int = 4) -> dict:
-    """Bytes each kernel class must move per PCG iteration (one RHS)."""
+    """Bytes each part of one PCG iteration must move (one RHS): the SpMV
+    (values and indices of la, row pointers, p read and q written), the
+    LDLinv apply (both sweeps over the factor's index and value, their row
+    pointers, the diagonal, one read and one write of y per sweep plus r)
+    and the vector updates."""
     spmv = nnzL * (sv + si) + (n + 1) * si + 2 * n * sv
-    # two sweeps over (index, weight) and the row pointers, the diagonal
-    # scale s, and one read + one write of y per sweep (plus r in forward)
-    ldl = 2 * (nnzF * (sv + si) + (n + 1) * si) + n * sv + 5 * n * sv
+    ldl = 2 * (nnzF * (sv + si) + (2 * n - 1) * si) + n * sv + 4 * n * sv
     vec = 10 * n * 8
     return {"spmv": spmv, "ldl": ldl, "vec": vec}
 
 
-def group_bytes(gi: dict, sv: int = 8, si: int = 4) -> int:
-    """Algorithmic bytes of the level-group launches of one iteration (both
-    sweeps): per composite-row entry its index and weight, per row its
-    pointer, one read of its input value (r or y0) and its write, and the
-    gathered values counted once per row they produce."""
-    rows = gi["rows"]
-    return (gi["fentries"] + gi["bentries"]) * (sv + si) + 2 * rows * (si + 3 * sv)
-
-
-def tail_bytes(ti: dict, sv: int = 8) -> int:
-    """Algorithmic bytes of one k_tail launch: its slot programs (16 B per
-    slot, 8 B per step), the tail inputs and scales read, yF written."""
-    return 16 * ti["nslots"] + 8 * ti["nsteps"] + 3 * ti["nt"] * sv
+def rows_bytes(entries: int, rows: int, sv: int = 8, si: int = 4) -> int:
+    """The same count for a part of the apply: its factor entries (index and
+    value) and, per row of both sweeps, its row pointers, its diagonal and
+    its y traffic (the apply's 56 bytes per vertex)."""
+    return entries * (sv + si) + rows * (4 * si + sv + 4 * sv)
 
 
 def read_traffic(kernel: str, workload: str):
@@ -101,197 +106,351 @@ def read_traffic(kernel: str, workload: str):
     return float(m.group(1)) if m else None
 
 
-def emit(out: dict) -> None:
-    print(json.dumps(out), flush=True)
+def _gbs(nbytes, ms):
+    return nbytes / (ms * 1e-3) / 1e9 if ms and ms > 0 else None
+
+
+def _frac(gbs):
+    return gbs / HBM_PEAK_GBS if gbs else None
 
 
 # ----------------------------------------------------------------- plain ---
-def mode_plain(args) -> int:
+def plain_leg(args, nsolves: int, warmup: int, cpu: bool) -> dict:
+    """The plain FP64 solve of grid3(128): solves/s, the per-kernel roofline
+    of one iteration (HIP events on the library's stream) and the CPU
+    baseline.  Returns {"line": ..., "roofline": ..., "cpu_baseline": ...}."""
     import lapgpu
-    from lapgpu import _native
+    from lapgpu import _native, workloads
     from lapgpu._native import DeviceArray, synchronize
 
     t0 = time.time()
-    a, b = make_workload(args.workload)
+    a, b = workloads.config2()
     la = lapgpu.lap(a)
     n = a.shape[0]
-    log(f"workload {args.workload}: n={n} nnz(L)={la.nnz} built in {time.time() - t0:.1f}s")
+    log(f"plain: grid3(128) n={n} nnz(L)={la.nnz} built in {time.time() - t0:.1f}s")
     t0 = time.time()
     f = lapgpu.build_ldli(a, seed=1)
     t_factor = time.time() - t0
-    log(f"factor nnzF={f.nnz} in {t_factor:.1f}s")
-
     ctx = _native.Context(0)
+    t0 = time.time()
     ctx.upload_lap(la)
     ctx.upload_ldli(f)
+    t_upload = time.time() - t0
     tune = None
     if args.autotune:
         t0 = time.time()
         tune = ctx.autotune(k=0, groups=True)
         tune["seconds"] = time.time() - t0
-        log(f"autotune: {tune}")
-    info = ctx.factor_info()
-    ti = ctx.tail_info()
-    gi = ctx.group_info()
-    log(f"schedule: {info} tail {ti} groups {gi}")
-
+    info, ti, gi = ctx.factor_info(), ctx.tail_info(), ctx.group_info()
+    se, di = ctx.sweep_entries(), ctx.dense_info()
+    log(f"plain: factor nnzF={f.nnz} in {t_factor:.1f}s, upload+plan {t_upload:.1f}s; schedule {info} tail {ti} "
+        f"groups {gi} dense {di}")
     db = DeviceArray.from_host(b)
     dx = DeviceArray(n)
     synchronize()
 
     def step():
-        return ctx.solve_device(db.address, dx.address, n, tol=args.tol, maxits=1000, stag_test=5)
+        return ctx.solve_device(db.address, dx.address, n, tol=1e-8, maxits=1000, stag_test=5)
 
-    for _ in range(args.warmup):
-        r = step()
-        log(f"warmup solve: {r}")
+    for _ in range(warmup):
+        step()
     synchronize()
     t0 = time.perf_counter()
-    its = []
-    for _ in range(args.steps):
-        r = step()
-        its.append(r["its"])
+    its = [step()["its"] for _ in range(nsolves)]
     synchronize()
     elapsed = time.perf_counter() - t0
     x = dx.download()
     relres = float(np.linalg.norm(la @ x - b) / np.linalg.norm(b))
-    log(f"timed {args.steps} solves in {elapsed:.3f}s its={its} true relres={relres:.3e}")
+    log(f"plain: {nsolves} solves in {elapsed:.3f}s its={its} true relres={relres:.3e}")
 
-    # per-kernel device times: HIP events recorded on the library's stream
-    # around each kernel class of every iteration of one more solve
-    prof = ctx.solve_device(db.address, dx.address, n, tol=args.tol, maxits=1000, stag_test=5,
-                            flags=_native.PROFILE)
-    tm = ctx.last_timings()
-    st = ctx.last_sweep_timings()
+    # per-kernel device times of one more solve: HIP events recorded on the
+    # library's stream around each kernel class of every iteration
+    ctx.solve_device(db.address, dx.address, n, tol=1e-8, maxits=1000, stag_test=5, flags=_native.PROFILE)
+    tm, st, dt = ctx.last_timings(), ctx.last_sweep_timings(), ctx.last_dense_timings()
+    log(f"plain: per iteration {tm} {st} {dt}")
     byts = algorithmic_bytes(n, la.nnz, f.nnz)
-    spmv_gbs = byts["spmv"] / (tm["spmv_ms"] * 1e-3) / 1e9 if tm["spmv_ms"] > 0 else None
-    ldl_gbs = byts["ldl"] / (tm["ldl_ms"] * 1e-3) / 1e9 if tm["ldl_ms"] > 0 else None
-    tb = tail_bytes(ti)
-    tail_gbs = tb / (st["tail_ms"] * 1e-3) / 1e9 if st["tail_ms"] > 0 else None
-    gb = group_bytes(gi)
-    g_ms = st["fgroups_ms"] + st["bgroups_ms"]
-    group_gbs = gb / (g_ms * 1e-3) / 1e9 if g_ms > 0 else None
-    log(f"profile solve {prof}: {tm} {st} spmv {spmv_gbs} GB/s ldl {ldl_gbs} GB/s tail {tail_gbs} GB/s")
-
-    cpu = None
-    if not args.no_cpu:
+    T0, NT, K = info["tail_start"], ti["nt"], di["k"]
+    ldd = (K + 255) // 256 * 256
+    groups_ms = st["fgroups_ms"] + st["bgroups_ms"]
+    g_alg = rows_bytes(se["groups_f"] + se["groups_b"], T0)
+    g_comp = rows_bytes(gi["fentries"] + gi["bentries"], T0)
+    tail_ms = st["tail_ext_ms"] + st["tail_ms"]
+    t_alg = rows_bytes(se["tail_f"] + se["tail_b"], NT)
+    dense_bytes = 8 * (K * ldd + 2 * ldd)
+    classes = {
+        "k_group": {"ms_per_iteration": groups_ms, "launches_per_iteration": 2 * gi["ngroups"],
+                    "algorithmic_bytes": g_alg, "achieved": _gbs(g_alg, groups_ms),
+                    "composite_bytes": g_comp, "composite_overhead": g_comp / g_alg,
+                    "traffic": read_traffic("k_group", "grid3_128")},
+        "tail (k_tail_ext + k_tail_fwd + k_dense + k_tail_bwd)": {
+            "ms_per_iteration": tail_ms, "algorithmic_bytes": t_alg, "achieved": _gbs(t_alg, tail_ms),
+            "tail_vertices": NT, "sparse_steps": ti["nsteps"]},
+        "k_dense": {"ms_per_iteration": dt["dense_ms"], "bytes_per_launch": dense_bytes, "K": K,
+                    "achieved": _gbs(dense_bytes, dt["dense_ms"]),
+                    "traffic": read_traffic("k_dense", "grid3_128")},
+        "k_spmv_dot": {"ms_per_iteration": tm["spmv_ms"], "algorithmic_bytes": byts["spmv"],
+                       "achieved": _gbs(byts["spmv"], tm["spmv_ms"]),
+                       "traffic": read_traffic("k_spmv_dot", "grid3_128")},
+    }
+    for v in classes.values():
+        v["frac"] = _frac(v["achieved"])
+    dom = max(("k_group", "k_dense", "k_spmv_dot"), key=lambda k: classes[k]["ms_per_iteration"] or 0.0)
+    apply_gbs = _gbs(byts["ldl"], tm["ldl_ms"])
+    both_gbs = _gbs(byts["ldl"] + byts["spmv"], tm["ldl_ms"] + tm["spmv_ms"])
+    d = classes[dom]
+    roofline = {
+        "bound": "hbm",
+        "kernel": dom,
+        "achieved": d["achieved"],
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": d["frac"],
+        "traffic": d.get("traffic"),
+        "workload": "plain FP64 solve of grid3(128), one PCG iteration",
+        "classes": classes,
+        "ldl_apply": {"ms_per_iteration": tm["ldl_ms"], "algorithmic_bytes": byts["ldl"], "achieved": apply_gbs,
+                      "frac": _frac(apply_gbs)},
+        "spmv_plus_ldl_apply": {"achieved": both_gbs, "frac": _frac(both_gbs)},
+    }
+    cpu_b = None
+    if cpu:
         from oracle import oracle
 
         oracle.build()
-        cpu_its = args.cpu_its or (20 if n > 500000 else 200)
         t0 = time.perf_counter()
-        _, io = oracle.approxchol_solve(la, f, b, tol=args.tol, maxits=cpu_its, stag_test=5)
+        xo, io = oracle.approxchol_solve(la, f, b, tol=1e-8, maxits=1000, stag_test=5)
         t_cpu = time.perf_counter() - t0
-        per_it = t_cpu / max(io["its"], 1)
-        its_full = int(np.median(its))
-        cpu = {
-            "value": 1.0 / (per_it * its_full),
+        cpu_b = {
+            "value": 1.0 / t_cpu,
             "unit": "solves/s",
             "cores": 1,
             "kind": "port",
-            "sample": f"oracle pcg+LDLsolver (C, 1 thread) for {io['its']} iterations on the same graph, factor and b "
-                      f"({t_cpu:.1f}s, {per_it * 1e3:.1f} ms/iteration), scaled to the GPU's {its_full} iterations",
+            "sample": f"one full solve of the same grid3(128) workload (graph, factor, b, tol 1e-8) by the oracle, "
+                      f"the C restatement of pcg + LDLsolver, on one host core: {io['its']} iterations in "
+                      f"{t_cpu:.2f} s ({t_cpu / max(io['its'], 1) * 1e3:.1f} ms/iteration)",
+            "host": host_info(),
         }
-        log(f"cpu baseline: {cpu}")
+        log(f"plain: cpu baseline {cpu_b}")
+    ms = elapsed / nsolves * 1e3
+    line = {
+        "solves_per_s": nsolves / elapsed,
+        "ms_per_solve": ms,
+        "iterations": int(np.median(its)),
+        "ms_per_iteration": ms / max(np.median(its), 1),
+        "true_relres": relres,
+        "n": n, "nnz_L": int(la.nnz), "nnzF": int(f.nnz),
+        "levels": info["nlevels"], "level_groups": info["nwide"], "tail_vertices": NT,
+        "dense_block_vertices": K, "tail_sparse_steps": ti["nsteps"],
+        "per_iteration_ms": {"spmv": tm["spmv_ms"], "ldl_apply": tm["ldl_ms"], "vectors": tm["vec_ms"], **st, **dt},
+        "factor_host_s": t_factor, "upload_plan_s": t_upload, "autotune": tune,
+    }
+    ctx.close()
+    return {"line": line, "roofline": roofline, "cpu_baseline": cpu_b}
 
-    ms = elapsed / args.steps * 1e3
+
+def mode_plain(args) -> int:
+    out = plain_leg(args, args.steps, args.warmup, cpu=not args.no_cpu)
+    ln = out["line"]
     emit({
         "metric": "solves/s (plain FP64 GPU PCG, relres<=1e-8)",
-        "value": args.steps / elapsed,
+        "value": ln["solves_per_s"],
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": ln["ms_per_solve"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "grid3(128): exp(U(-2,2)) weights seed 2, 1 RHS N(0,1) sum-zero, tol 1e-8", **ln},
+        "roofline": out["roofline"],
+        "cpu_baseline": out["cpu_baseline"],
+    })
+    if args.calib:
+        from lapgpu._native import DeviceArray, synchronize
+
+        src = DeviceArray(1 << 26)
+        dst = DeviceArray(1 << 26)
+        dst.copy_from(src)
+        synchronize()
+        log("calibration copy: 536870912 bytes read, 536870912 bytes written")
+    return 0
+
+
+# ----------------------------------------------------------------- serve ---
+def serve_leg(args) -> dict:
+    from concurrent.futures import ThreadPoolExecutor
+
+    import lapgpu
+    from lapgpu import _native, workloads
+    from lapgpu.serve import Request, Server, summary
+
+    nreq = args.steps * args.step_requests
+    t0 = time.time()
+    G, reqs = workloads.config5(nreq=nreq, qps=args.qps, ngraphs=args.serve_graphs, min_n=args.serve_min_n,
+                                max_n=args.serve_max_n, new_frac=args.new_frac, new_max_n=args.serve_new_max_n)
+    warm = workloads.serve_requests(G, args.warmup * args.step_requests, qps=args.qps, min_n=args.serve_min_n,
+                                    new_max_n=args.serve_new_max_n, new_frac=args.new_frac, rid0=10 ** 7)
+    log(f"serve: {len(G)} resident graphs n {min(g.shape[0] for g in G)}..{max(g.shape[0] for g in G)} "
+        f"(sum {sum(g.shape[0] for g in G)}); {nreq} timed requests ({sum(r.new_graph is not None for r in reqs)} "
+        f"with new graphs) and {len(warm)} warm-up requests at {args.qps} QPS; generated in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    with ThreadPoolExecutor(8) as ex:
+        F = list(ex.map(lambda g: lapgpu.build_ldli(g, seed=1), G))
+    log(f"serve: resident graphs factored in {time.time() - t0:.1f}s")
+    # solutions kept for the true-residual check: an even sample of the stream
+    sample = set(r.rid for r in reqs[:: max(1, nreq // args.check)])
+    srv = Server(0, hbm_budget=200 * 2**30, plain_workers=args.serve_workers,
+                 factor_workers=args.factor_workers, batch_window=args.batch_window,
+                 max_wait=args.max_wait, quick_workers=args.quick_workers,
+                 blocking_waits=not args.spin_waits, wg_max_n=args.wg_max_n, keep_ids=sample)
+    t0 = time.time()
+    for i, (g, f) in enumerate(zip(G, F)):
+        assert srv.register(i, g, ldli=f)
+    log(f"serve: resident graphs uploaded in {time.time() - t0:.1f}s, HBM reserved {srv.hbm_used / 2**30:.1f} GiB")
+    # warm-up: one solve per resident graph (captures, workspaces), then the
+    # warm-up stream
+    wrng = np.random.default_rng(55)
+    srv.run([Request(-1 - i, 0.0, wrng.standard_normal(G[i].shape[0]), graph=i) for i in range(len(G))],
+            realtime=False)
+    if warm:
+        t0 = time.perf_counter()
+        wres = srv.run(warm, progress=lambda d, t, e: log(f"  warm-up: {d} of {t} requests at {e:.0f} s"))
+        log(f"serve: warm-up stream {len(warm)} requests in {time.perf_counter() - t0:.1f}s: "
+            f"{summary(wres, time.perf_counter() - t0)}")
+    import resource
+
+    ru0 = resource.getrusage(resource.RUSAGE_SELF)
+    t0 = time.perf_counter()
+    res = srv.run(reqs, progress=lambda d, t, e: log(f"  served {d} of {t} requests at {e:.0f} s"))
+    wall = time.perf_counter() - t0
+    ru1 = resource.getrusage(resource.RUSAGE_SELF)
+    cpu_s = (ru1.ru_utime - ru0.ru_utime) + (ru1.ru_stime - ru0.ru_stime)
+    s = summary(res, wall)
+    paths = {p: sum(o.path == p for o in res) for p in sorted({o.path for o in res})}
+    log(f"serve: {s} paths {paths}; host CPU {cpu_s:.0f} s over {wall:.1f} s")
+    for pth in paths:
+        sel = [(r, o) for r, o in zip(reqs, res) if o.path == pth]
+        lat = np.array([o.latency for _, o in sel])
+        log(f"  path {pth}: {len(sel)} ({sum(r.new_graph is not None for r, _ in sel)} new graphs), latency p50 "
+            f"{np.percentile(lat, 50):.2f} s p99 {np.percentile(lat, 99):.2f} s")
+    nf = np.array(srv.factor_log)
+    up = np.array(srv.upload_log).reshape(-1, 4)
+    new_n = [r.new_graph.shape[0] for r in reqs if r.new_graph is not None]
+    pipeline = {"new_graphs": len(new_n), "new_graph_vertices": int(sum(new_n)),
+                "factor_s_median": float(np.median(nf)) if len(nf) else None,
+                "factor_s_sum": float(nf.sum()) if len(nf) else None,
+                "context_upload_s_median": float(np.median(up[:, 0])) if len(up) else None,
+                "context_upload_s_sum": float(up[:, 0].sum()) if len(up) else None,
+                "host_cpu_s": cpu_s, "factor_workers": args.factor_workers}
+    log(f"serve: new-graph pipeline {pipeline}")
+    # true residual ||L x - b|| / ||b|| on the kept sample
+    byr = {r.rid: r for r in reqs}
+    rr = []
+    for o in res:
+        if o.x is None:
+            continue
+        r = byr[o.rid]
+        la = lapgpu.lap(G[r.graph] if r.graph is not None else r.new_graph)
+        rr.append(float(np.linalg.norm(la @ o.x - r.b) / np.linalg.norm(r.b)))
+    check = {"checked": len(rr), "max_true_relres": max(rr) if rr else None,
+             "above_tol": int(sum(v > 1e-8 for v in rr)), "max_recursive_relres": s["max_nr"]}
+    log(f"serve: true residuals of the sample: {check}")
+    # plain one-at-a-time FP64 GPU PCG on the same GPU: the measured plain
+    # solve time of every resident graph, the same arrival stream through
+    # one FIFO server (new graphs: a log-log fit of solve time against n,
+    # factoring excluded)
+    svc = {}
+    for i, (g, f) in enumerate(zip(G, F)):
+        c = _native.Context(0)
+        c.upload_lap(lapgpu.lap(g))
+        c.upload_ldli(f)
+        bb = np.random.default_rng(i).standard_normal(g.shape[0])
+        bb -= bb.mean()
+        c.solve(bb, tol=1e-8, maxits=1000)
+        t1 = time.perf_counter()
+        c.solve(bb, tol=1e-8, maxits=1000)
+        svc[i] = time.perf_counter() - t1
+        c.close()
+    sizes_arr = np.array([g.shape[0] for g in G], dtype=float)
+    svc_arr = np.array([svc[i] for i in range(len(G))])
+    coef = np.polyfit(np.log(sizes_arr), np.log(svc_arr), 1)
+    t_free, busy, lat_plain = 0.0, 0.0, []
+    for r in reqs:
+        stt = svc[r.graph] if r.graph is not None else float(np.exp(np.polyval(coef, np.log(len(r.b)))))
+        start = max(t_free, r.arrival)
+        t_free = start + stt
+        busy += stt
+        lat_plain.append(t_free - r.arrival)
+    plain_sps = len(reqs) / busy
+    srv.close()
+    return {
+        "wall": wall, "nreq": nreq, "summary": s, "paths": paths, "check": check, "pipeline": pipeline,
+        "plain": {"solves_per_s": plain_sps, "p50_ms": float(np.percentile(lat_plain, 50) * 1e3),
+                  "p99_ms": float(np.percentile(lat_plain, 99) * 1e3),
+                  "how": "modelled: FIFO replay of the same arrivals, service time = the measured plain FP64 "
+                         "solve (lapgpu_solve, one at a time) of each resident graph; new graphs by a log-log "
+                         "fit of those times against n, factoring excluded"},
+        "resident_n": [int(g.shape[0]) for g in G],
+    }
+
+
+def mode_serve(args) -> int:
+    # every resident graph owns a stream; HIP maps streams round-robin onto
+    # its hardware queues (4 unless GPU_MAX_HW_QUEUES says otherwise), and two
+    # busy streams on one queue run one after the other.  Read at HIP start-up,
+    # so it is set before the library loads.
+    if args.hw_queues:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
+    sv = serve_leg(args)
+    pl = None if args.no_plain else plain_leg(args, 5, 1, cpu=not args.no_cpu)
+    s = sv["summary"]
+    emit({
+        "metric": "served solves/s at relres<=1e-8 (p50/p99 latency) vs plain one-at-a-time FP64 GPU PCG",
+        "value": s["solves_per_s"],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sv["wall"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"{args.workload}: exp(U(-2,2)) weights seed 2, 1 RHS N(0,1) sum-zero, tol {args.tol}",
-            "autotune": tune,
-            "n": n,
-            "nnz_L": int(la.nnz),
-            "nnzF": int(f.nnz),
-            "iterations": int(np.median(its)),
-            "ms_per_iteration": ms / max(np.median(its), 1),
-            "true_relres": relres,
-            "levels": info["nlevels"],
-            "level_groups": info["nwide"],
-            "tail_vertices": ti["nt"],
-            "tail_steps": ti["nsteps"],
-            "spmv_ms": tm["spmv_ms"],
-            "ldl_ms": tm["ldl_ms"],
-            "vec_ms": tm["vec_ms"],
-            "ldl_split_ms": st,
-            "spmv_GBps": spmv_gbs,
-            "ldl_apply_GBps": ldl_gbs,
-            "factor_host_s": t_factor,
+            "workload": f"serving: {sv['nreq']} requests ({args.steps} steps of {args.step_requests}), Poisson "
+                        f"{args.qps:g} QPS, {args.serve_graphs} resident graphs n log-uniform "
+                        f"[{args.serve_min_n:g},{args.serve_max_n:g}] (2-D/3-D grids exp(U(-2,2)), path + 4 "
+                        f"matchings log-uniform weights), {args.new_frac:.0%} with a new graph of the same "
+                        f"families n <= {args.serve_new_max_n:g}, seed 5, tol 1e-8",
+            "served": s,
+            "paths": sv["paths"],
+            "true_relres_sample": sv["check"],
+            "new_graph_pipeline": sv["pipeline"],
+            "plain": sv["plain"],
+            "speedup_solves_per_s": s["solves_per_s"] / sv["plain"]["solves_per_s"],
+            "p99_ratio_plain_over_served": sv["plain"]["p99_ms"] / s["p99_ms"],
+            "plain_grid3_128": pl["line"] if pl else None,
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "k_group (the level-group launches of both LDLinv sweeps, one iteration; "
-                      "latency-bound by dependent launches)",
-            "achieved": group_gbs,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": (group_gbs / HBM_PEAK_GBS) if group_gbs else None,
-            "traffic": read_traffic("k_group", args.workload),
-            "bytes_per_iteration": gb,
-            "launches_per_iteration": 2 * gi["ngroups"],
-            "ms_per_iteration": g_ms,
-            "tail": {"kernel": "k_tail", "achieved": tail_gbs,
-                     "frac": (tail_gbs / HBM_PEAK_GBS) if tail_gbs else None,
-                     "traffic": read_traffic("k_tail", args.workload), "bytes_per_launch": tb,
-                     "avg_launch_ms": st["tail_ms"]},
-            "spmv": {"kernel": "k_spmv_dot", "achieved": spmv_gbs,
-                     "frac": (spmv_gbs / HBM_PEAK_GBS) if spmv_gbs else None,
-                     "traffic": read_traffic("k_spmv_dot", args.workload)},
-        },
-        "cpu_baseline": cpu,
+        "roofline": pl["roofline"] if pl else None,
+        "cpu_baseline": pl["cpu_baseline"] if pl else None,
     })
-    if args.calib:
-        src = DeviceArray(1 << 26)
-        dst = DeviceArray(1 << 26)
-        dst.copy_from(src)
-        synchronize()
-        log("calibration copy: 536870912 bytes read, 536870912 bytes written")
-    ctx.close()
     return 0
 
 
 # ----------------------------------------------------------------- multi ---
-def _small_grids(rng, count: int):
-    from lapgpu import graphs
-
-    out = []
-    for _ in range(count):
-        nv = float(np.exp(rng.uniform(np.log(1e3), np.log(1e5))))
-        ar = rng.uniform(1.0, 8.0)
-        ny = max(2, int(round(np.sqrt(nv / ar))))
-        nx = max(2, int(round(nv / ny)))
-        out.append(graphs.grid2(nx, ny))
-    return out
-
-
 def mode_multi(args) -> int:
     import lapgpu
-    from lapgpu import _native, graphs
+    from lapgpu import _native, workloads
     from lapgpu._native import DeviceArray, synchronize
 
-    rng = np.random.default_rng(4)
-    a = graphs.weighted_grid2(1024, 1024, rng, "expN")
+    a, B, smalls, sb = workloads.config4(rhs=args.rhs, small=args.small)
     la = lapgpu.lap(a)
     n = a.shape[0]
     k = args.rhs
-    B = rng.standard_normal((k, n))
-    B -= B.mean(axis=1, keepdims=True)
-    smalls = _small_grids(rng, args.small)
-    sb = []
-    for g in smalls:
-        v = rng.standard_normal(g.shape[0])
-        sb.append(v - v.mean())
     t0 = time.time()
     f = lapgpu.build_ldli(a, seed=1)
     from concurrent.futures import ThreadPoolExecutor
@@ -338,8 +497,10 @@ def mode_multi(args) -> int:
         run_batch()
         synchronize()
         t_batch = time.perf_counter() - t0
+        ctx.solve_multi_device(dB.address, dX.address, n, k, tol=1e-8, maxits=1000, flags=_native.PROFILE)
         log(f"{pname}: 64-RHS block alone {t_multi * 1e3:.1f} ms ({max(results['multi']['its'])} its), "
-            f"256 small grids alone {t_batch * 1e3:.1f} ms")
+            f"256 small grids alone {t_batch * 1e3:.1f} ms; per iteration {ctx.last_timings()} "
+            f"{ctx.last_sweep_timings()} {ctx.last_dense_timings()}")
         times = []
         for _ in range(args.steps):
             t0 = time.perf_counter()
@@ -362,7 +523,7 @@ def mode_multi(args) -> int:
             "small_true_relres_max": srel,
             "multi_alone_ms": t_multi * 1e3,
             "small_alone_ms": t_batch * 1e3,
-            "X": X if pname == "fp64" else X.copy(),
+            "X": X,
         }
         log(f"{pname}: {out_modes[pname]['solves_per_s']:.1f} solves/s wall {out_modes[pname]['wall_s']*1e3:.1f} ms "
             f"relres {rel.max():.2e} small relres {srel:.2e}")
@@ -397,10 +558,7 @@ def mode_multi(args) -> int:
         c1.close()
     plain_wall = k * t_big + len(smalls) * float(np.mean(t_small))
     plain_sps = (k + len(smalls)) / plain_wall
-    # the workload names FP64 outer / FP32 inner; the FP64 leg is reported beside it
-    best_name = "fp32_inner"
-    best = out_modes[best_name]
-    faster = max(out_modes, key=lambda m: out_modes[m]["solves_per_s"])
+    best = out_modes["fp32_inner"]  # the workload names FP64 outer / FP32 inner
     emit({
         "metric": "solves/s (64-RHS block + 256 small grids, relres<=1e-8) vs plain FP64 GPU PCG",
         "value": best["solves_per_s"],
@@ -416,9 +574,9 @@ def mode_multi(args) -> int:
         "data": "synthetic",
         "config": {
             "workload": "grid2 1024x1024 exp(N(0,1)) weights, 64 RHS N(0,1) sum-zero + 256 unit 2-D grids "
-                        "(n log-uniform [1e3,1e5], aspect [1,8]), seed 4, tol 1e-8",
-            "reported_mode": best_name,
-            "faster_mode": faster,
+                        "(n log-uniform [1e3,1e5], aspect [1,8]), seed 4, tol 1e-8; the block and the small "
+                        "grids run as two concurrent launches streams (not one launch)",
+            "reported_mode": "fp32_inner",
             "autotune": tune,
             "fp32_inner": out_modes["fp32_inner"],
             "fp64": out_modes["fp64"],
@@ -439,15 +597,12 @@ def mode_multi(args) -> int:
 # ------------------------------------------------------------- precision ---
 def mode_precision(args) -> int:
     import lapgpu
-    from lapgpu import _native, graphs
+    from lapgpu import _native, workloads
     from lapgpu._native import DeviceArray, synchronize
 
-    rng = np.random.default_rng(3)
-    a = graphs.path_matchings(1_000_000, 4, rng, "logU3")
+    a, b = workloads.config3()
     la = lapgpu.lap(a)
     n = a.shape[0]
-    b = rng.standard_normal(n)
-    b -= b.mean()
     t0 = time.time()
     f = lapgpu.build_ldli(a, seed=1)
     log(f"path+4 matchings n={n} nnzL={la.nnz} nnzF={f.nnz} factor {time.time() - t0:.1f}s")
@@ -469,8 +624,8 @@ def mode_precision(args) -> int:
             ts.append(time.perf_counter() - t0)
         x = dx.download()
         ctx.solve_device(db.address, dx.address, n, tol=1e-8, maxits=1000, flags=_native.PROFILE)
-        log(f"{name} schedule {ctx.factor_info()} tail {ctx.tail_info()} per-iteration ms "
-            f"{ctx.last_timings()} {ctx.last_sweep_timings()}")
+        log(f"{name} schedule {ctx.factor_info()} tail {ctx.tail_info()} dense {ctx.dense_info()} per-iteration ms "
+            f"{ctx.last_timings()} {ctx.last_sweep_timings()} {ctx.last_dense_timings()}")
         res[name] = {"its": r["its"], "stop": r["stop"], "solve_ms": float(np.median(ts)) * 1e3,
                      "true_relres": float(np.linalg.norm(la @ x - b) / np.linalg.norm(b)), "x": x}
         log(f"{name}: {res[name]['its']} its {res[name]['solve_ms']:.1f} ms relres {res[name]['true_relres']:.2e}")
@@ -505,184 +660,19 @@ def mode_precision(args) -> int:
     return 0
 
 
-# ----------------------------------------------------------------- serve ---
-def _serve_graph(rng, nv: float):
-    from lapgpu import graphs
-
-    kind = rng.integers(3)
-    if kind == 0:
-        ar = rng.uniform(1.0, 4.0)
-        ny = max(2, int(round(np.sqrt(nv / ar))))
-        return graphs.weighted_grid2(max(2, int(round(nv / ny))), ny, rng, "expU2")
-    if kind == 1:
-        s = max(2, int(round(nv ** (1 / 3))))
-        return graphs.weighted_grid3(s, s, max(2, int(round(nv / (s * s)))), rng, "expU2")
-    return graphs.path_matchings(max(4, int(nv) // 2 * 2), 4, rng, "logU3")
-
-
-def mode_serve(args) -> int:
-    from concurrent.futures import ThreadPoolExecutor
-
-    # every resident graph owns a stream; HIP maps streams round-robin onto
-    # its hardware queues (4 unless GPU_MAX_HW_QUEUES says otherwise), and two
-    # busy streams on one queue run one after the other.  Read at HIP start-up,
-    # so it is set before the library loads.
-    if args.hw_queues:
-        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
-
-    import lapgpu
-    from lapgpu import _native
-    from lapgpu.serve import Request, Server, summary
-
-    rng = np.random.default_rng(5)
-    lo, hi = np.log(args.serve_min_n), np.log(args.serve_max_n)
-    sizes = np.exp(rng.uniform(lo, hi, args.serve_graphs))
-    G = [_serve_graph(rng, nv) for nv in sizes]
-    t0 = time.time()
-    with ThreadPoolExecutor(8) as ex:
-        F = list(ex.map(lambda g: lapgpu.build_ldli(g, seed=1), G))
-    log(f"{len(G)} resident graphs, n {min(g.shape[0] for g in G)}..{max(g.shape[0] for g in G)} "
-        f"(sum {sum(g.shape[0] for g in G)}), factored in {time.time() - t0:.1f}s")
-    srv = Server(0, hbm_budget=200 * 2**30, plain_workers=args.serve_workers,
-                 factor_workers=args.factor_workers, batch_window=args.batch_window,
-                 max_wait=args.max_wait, quick_workers=args.quick_workers,
-                 blocking_waits=not args.spin_waits, wg_max_n=args.wg_max_n)
-    for i, (g, f) in enumerate(zip(G, F)):
-        assert srv.register(i, g, ldli=f)
-    # request stream
-    nreq = args.requests
-    arr = np.cumsum(rng.exponential(1.0 / args.qps, nreq))
-    reqs = []
-    new_sizes = np.exp(rng.uniform(lo, np.log(args.serve_new_max_n), nreq))
-    for i in range(nreq):
-        if rng.uniform() < args.new_frac:
-            g = _serve_graph(rng, new_sizes[i])
-            b = rng.standard_normal(g.shape[0])
-            reqs.append(Request(i, float(arr[i]), b - b.mean(), new_graph=g, tol=1e-8))
-        else:
-            gi = int(rng.integers(len(G)))
-            b = rng.standard_normal(G[gi].shape[0])
-            reqs.append(Request(i, float(arr[i]), b - b.mean(), graph=gi, tol=1e-8))
-    log(f"{nreq} requests at {args.qps} QPS ({sum(r.new_graph is not None for r in reqs)} with new graphs)")
-    # warm every resident graph once (graph capture, workspaces)
-    wrng = np.random.default_rng(55)
-    srv.run([Request(-1 - i, 0.0, wrng.standard_normal(G[i].shape[0]), graph=i) for i in range(len(G))],
-            realtime=False)
-    import resource
-
-    ru0 = resource.getrusage(resource.RUSAGE_SELF)
-    t0 = time.perf_counter()
-    res = srv.run(reqs, progress=lambda d, t, e: log(f"  served {d} of {t} requests at {e:.0f} s"))
-    wall = time.perf_counter() - t0
-    ru1 = resource.getrusage(resource.RUSAGE_SELF)
-    cpu_s = (ru1.ru_utime - ru0.ru_utime) + (ru1.ru_stime - ru0.ru_stime)
-    log(f"host CPU during the run: {cpu_s:.1f} s over {wall:.1f} s wall ({cpu_s / wall:.1f} cores busy on average)")
-    s = summary(res, wall)
-    paths = {p: sum(o.path == p for o in res) for p in {o.path for o in res}}
-    log(f"served: {s} paths {paths}")
-    for pth in sorted(paths):
-        sel = [(r, o) for r, o in zip(reqs, res) if o.path == pth]
-        lat = np.array([o.latency for _, o in sel])
-        done = max(r.arrival + o.latency for r, o in sel)
-        newg = sum(r.new_graph is not None for r, _ in sel)
-        log(f"  path {pth}: {len(sel)} ({newg} new graphs), latency p50 {np.percentile(lat, 50):.2f} s "
-            f"p99 {np.percentile(lat, 99):.2f} s, last done at {done:.1f} s")
-    log(f"  dispatch loop: {srv.stats}")
-    up = np.array(srv.upload_log).reshape(-1, 4)
-    for name, lg in (("factor", np.array(srv.factor_log)), ("context upload", up[:, 0]),
-                     ("  create", up[:, 1]), ("  upload la", up[:, 2]), ("  upload ldli", up[:, 3])):
-        if len(lg):
-            log(f"  new graphs, {name}: {len(lg)} x, median {np.median(lg) * 1e3:.1f} ms, "
-                f"max {lg.max() * 1e3:.1f} ms, sum {lg.sum():.1f} s")
-    fc = np.array(srv.factor_cpu_log)
-    if len(fc):
-        log(f"  new graphs, CPU of the calling thread: factor {fc.sum():.1f} s, "
-            f"context upload {np.sum(srv.upload_cpu_log):.1f} s (planner helper threads not counted)")
-    sl = np.array(srv.solve_log[len(G):], dtype=float) if len(srv.solve_log) > len(G) else np.zeros((0, 5))
-    if len(sl):
-        log(f"  solve workers: {sl[:, 2].sum():.1f} s waiting in solves, {sl[:, 4].sum():.1f} s of it on the CPU")
-    for lo_n, hi_n in ((0, 1e5), (1e5, 1e6), (1e6, 1e8)):
-        m = (sl[:, 0] >= lo_n) & (sl[:, 0] < hi_n) if len(sl) else np.zeros(0, bool)
-        if m.any():
-            f1 = m & (sl[:, 3] > 0)
-            log(f"  solves n in [{lo_n:g},{hi_n:g}): {m.sum()} solves, {sl[m, 1].sum():.0f} columns, "
-                f"mean columns {sl[m, 1].mean():.1f}, {sl[m, 2].sum():.1f} s; first on a context: "
-                f"{f1.sum()}, {sl[f1, 2].sum():.1f} s")
-    # plain one-at-a-time FP64 GPU PCG on the same GPU: the service time of
-    # one plain solve per resident graph, then the same arrival stream
-    # through a single FIFO server (new graphs: their own plain solve time,
-    # factoring excluded)
-    svc = {}
-    for i, (g, f) in enumerate(zip(G, F)):
-        c = _native.Context(0)
-        c.upload_lap(lapgpu.lap(g))
-        c.upload_ldli(f)
-        bb = np.random.default_rng(i).standard_normal(g.shape[0])
-        bb -= bb.mean()
-        c.solve(bb, tol=1e-8, maxits=1000)
-        t1 = time.perf_counter()
-        c.solve(bb, tol=1e-8, maxits=1000)
-        svc[i] = time.perf_counter() - t1
-        c.close()
-    sizes_arr = np.array([g.shape[0] for g in G], dtype=float)
-    svc_arr = np.array([svc[i] for i in range(len(G))])
-    coef = np.polyfit(np.log(sizes_arr), np.log(svc_arr), 1)
-    t_free = 0.0
-    lat_plain = []
-    busy = 0.0
-    for r in reqs:
-        st = svc[r.graph] if r.graph is not None else float(np.exp(np.polyval(coef, np.log(len(r.b)))))
-        start = max(t_free, r.arrival)
-        t_free = start + st
-        busy += st
-        lat_plain.append(t_free - r.arrival)
-    plain_sps = len(reqs) / busy
-    emit({
-        "metric": "served solves/s + p50/p99 latency (relres<=1e-8) vs plain one-at-a-time FP64 GPU PCG",
-        "value": s["solves_per_s"],
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": nreq,
-        "warmup": len(G),
-        "ms_per_step": wall / nreq * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {
-            "workload": f"{nreq} requests, Poisson {args.qps} QPS, {len(G)} resident graphs n log-uniform "
-                        f"[{args.serve_min_n:g},{args.serve_max_n:g}] (2-D/3-D grids exp(U(-2,2)), path+4 matchings "
-                        f"log-uniform weights), {args.new_frac:.0%} with a new graph n <= {args.serve_new_max_n:g}, "
-                        "seed 5, tol 1e-8",
-            "served": s,
-            "paths": paths,
-            "plain_solves_per_s": plain_sps,
-            "plain_p50_ms": float(np.percentile(lat_plain, 50) * 1e3),
-            "plain_p99_ms": float(np.percentile(lat_plain, 99) * 1e3),
-            "speedup_solves_per_s": s["solves_per_s"] / plain_sps,
-            "p99_ratio_plain_over_served": float(np.percentile(lat_plain, 99) * 1e3) / s["p99_ms"],
-        },
-        "roofline": None,
-        "cpu_baseline": None,
-    })
-    srv.close()
-    return 0
-
-
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", default="plain", choices=["plain", "multi", "precision", "serve"])
-    ap.add_argument("--workload", default="grid3_128")
-    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--steps", type=int, default=20,
+                    help="serve: timed steps of --step-requests requests; plain/multi/precision: timed solves")
+    ap.add_argument("--warmup", type=int, default=2, help="serve: untimed steps of another stream first")
+    ap.add_argument("--mode", default="serve", choices=["serve", "plain", "multi", "precision"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-its", type=int, default=0, help="oracle iterations for the CPU baseline sample")
+    ap.add_argument("--no-plain", action="store_true", help="serve: skip the plain grid3(128) leg")
+    ap.add_argument("--step-requests", type=int, default=STEP_REQUESTS)
+    ap.add_argument("--check", type=int, default=200, help="serve: requests whose true residual is checked")
     ap.add_argument("--rhs", type=int, default=64)
     ap.add_argument("--small", type=int, default=256)
-    ap.add_argument("--requests", type=int, default=10000)
     ap.add_argument("--qps", type=float, default=200.0)
     ap.add_argument("--serve-graphs", type=int, default=32)
     ap.add_argument("--serve-min-n", type=float, default=1e4)

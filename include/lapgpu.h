@@ -173,11 +173,23 @@ int lapgpu_set_dense_tail(lapgpu_ctx* ctx, int64_t max_k);
  * before it, and the device bytes of D. */
 int lapgpu_dense_info(lapgpu_ctx* ctx, int64_t* k, int64_t* sparse_vertices, int64_t* bytes);
 
+/* Factor entries (LDLinv nonzeros) each part of the schedule applies: the
+ * forward and backward rows before the tail (the level groups), those of the
+ * tail, and K of the dense block (measurement only). */
+int lapgpu_sweep_entries(lapgpu_ctx* ctx, int64_t* groups_f, int64_t* groups_b, int64_t* tail_f,
+                         int64_t* tail_b, int64_t* dense_k);
+
 /* Schedule facts of the uploaded factor: number of levels, number of levels
  * run one launch each, the first vertex position of the single-workgroup
  * tail, nnzF. Any pointer may be NULL. */
 int lapgpu_factor_info(lapgpu_ctx* ctx, int64_t* nlevels, int64_t* nwide,
                        int64_t* tail_start, int64_t* nnzF);
+
+/* Per-iteration device times (ms) of the last LAPGPU_PROFILE solve's tail
+ * with a dense block: the sparse forward, the dense product, the sparse
+ * backward (0 without a block). */
+int lapgpu_last_dense_timings(lapgpu_ctx* ctx, double* tail_fwd_ms, double* dense_ms,
+                              double* tail_bwd_ms);
 
 /* Average device time per call of each kernel class over the last solve
  * (ms), measured with HIP events when LAPGPU_PROFILE is set in flags. */

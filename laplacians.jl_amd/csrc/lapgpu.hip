@@ -206,11 +206,13 @@ struct lapgpu_ctx {
 
   // profiling (LAPGPU_PROFILE): events around the kernel classes of one
   // iteration; ev[5..7] split the sweeps into forward groups, tail inputs,
-  // tail and backward groups
-  hipEvent_t ev[8] = {};
+  // tail and backward groups, ev[8..9] the tail into its sparse forward, the
+  // dense block's product and its sparse backward
+  hipEvent_t ev[10] = {};
   bool prof_sweeps = false;
   double t_spmv = 0, t_ldl = 0, t_vec = 0, t_total = 0;
   double t_fgroups = 0, t_tail_ext = 0, t_tail = 0, t_bgroups = 0;
+  double t_tail_fwd = 0, t_dense = 0, t_tail_bwd = 0;
   int64_t t_its = 0;
 };
 
@@ -411,9 +413,11 @@ void launch_sweeps(lapgpu_ctx* c, const double* rin, T* y, T* z, const T* gFw, c
                           dim3(kTailThreads), 0, c->stream, (int)TP.T0, (int)TP.NT, NT1, ext, sv, z,
                           da.dvec, c->fsteps, (int)TP.fsteps.size(), fslots, c->st);
       const int waves = (int)((DB.K + kDenseRows - 1) / kDenseRows);
+      if (c->prof_sweeps) (void)hipEventRecord(c->ev[8], c->stream);
       hipLaunchKernelGGL(k_dense<T>, dim3((waves + kBlock / kWave - 1) / (kBlock / kWave)),
                          dim3(kBlock), 0, c->stream, (int)DB.K, (long)c->ldd, da.D, u, z + DB.T1,
                          c->st);
+      if (c->prof_sweeps) (void)hipEventRecord(c->ev[9], c->stream);
       if (NT1 > 0)
         LAPGPU_DISPATCH_E(TP.elog, hipLaunchKernelGGL, (k_tail_bwd<T, S, E_>), dim3(1),
                           dim3(kTailThreads), 0, c->stream, (int)TP.T0, (int)TP.NT, NT1, z,
@@ -870,7 +874,7 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
 
   // ---- iterations: poll the stop flag one iteration behind ---------------
   double acc_spmv = 0, acc_ldl = 0, acc_vec = 0, acc_fg = 0, acc_ext = 0, acc_tail = 0,
-         acc_bg = 0;
+         acc_bg = 0, acc_tf = 0, acc_dn = 0, acc_tb = 0;
   int64_t enq = 0, polled = 0;
   bool done = c->hst->stopped != 0;
   const int rep_every = f32 ? c->replace_every : 0;
@@ -921,6 +925,15 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
       acc_ext += f2;
       acc_tail += f3;
       acc_bg += f4;
+      if (c->plan.dense.K > 0) {
+        float g1 = 0, g2 = 0, g3 = 0;
+        (void)hipEventElapsedTime(&g1, c->ev[6], c->ev[8]);
+        (void)hipEventElapsedTime(&g2, c->ev[8], c->ev[9]);
+        (void)hipEventElapsedTime(&g3, c->ev[9], c->ev[7]);
+        acc_tf += g1;
+        acc_dn += g2;
+        acc_tb += g3;
+      }
     }
     // keep one iteration in flight: wait for the flags of all older ones
     while (!done && polled < enq - 1) {
@@ -951,6 +964,9 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
     c->t_tail_ext = acc_ext / (double)enq;
     c->t_tail = acc_tail / (double)enq;
     c->t_bgroups = acc_bg / (double)enq;
+    c->t_tail_fwd = acc_tf / (double)enq;
+    c->t_dense = acc_dn / (double)enq;
+    c->t_tail_bwd = acc_tb / (double)enq;
   }
   c->t_total = secs * 1e3;
   c->t_its = h.itcnt;
@@ -1122,10 +1138,12 @@ void msweeps_t(lapgpu_ctx* c, const double* rin, T* y, T* z, T* ext, const T* gF
       const int tiles = (int)((DB.K + kMDenseRows - 1) / kMDenseRows);
       const int nsplit = dense_splits(c);
       const long len = (long)((c->ldd / nsplit + kMDenseSlab - 1) / kMDenseSlab * kMDenseSlab);
+      if (c->prof_sweeps) (void)hipEventRecord(c->ev[8], c->stream);
       hipLaunchKernelGGL((k_mdense<K, T>), dim3(tiles, nsplit), dim3(kBlock), 0, c->stream,
                          (int)DB.K, (long)c->ldd, da.D, u, da.part, len, m.flags);
       hipLaunchKernelGGL((k_mdense_sum<K, T>), dim3(kMRed), dim3(kBlock), 0, c->stream, (int)DB.K,
                          nsplit, da.part, z + (size_t)DB.T1 * K, m.flags);
+      if (c->prof_sweeps) (void)hipEventRecord(c->ev[9], c->stream);
       if (NT1 > 0)
         LAPGPU_DISPATCH_E(TP.elog, hipLaunchKernelGGL, (k_mtail_bwd<K, T, S, E_>), dim3(K),
                           dim3(kTailThreads), 0, c->stream, (int)TP.T0, (int)TP.NT, NT1, z,
@@ -1327,6 +1345,15 @@ int solve_multi_chunk(lapgpu_ctx* c, const double* d_B, double* d_X, int k, doub
       acc[5] += f2;
       acc[6] += f3;
       acc[7] += f4;
+      if (c->plan.dense.K > 0) {
+        float g1 = 0, g2 = 0, g3 = 0;
+        (void)hipEventElapsedTime(&g1, c->ev[6], c->ev[8]);
+        (void)hipEventElapsedTime(&g2, c->ev[8], c->ev[9]);
+        (void)hipEventElapsedTime(&g3, c->ev[9], c->ev[7]);
+        acc[8] += g1;
+        acc[9] += g2;
+        acc[10] += g3;
+      }
     }
     while (!done && polled < enq - 1) {
       const int ps = (int)(polled % lapgpu_ctx::kRing);
@@ -1360,8 +1387,9 @@ int solve_multi_device(lapgpu_ctx* c, const double* d_B, double* d_X, int64_t k,
   if (!c->has_F) return set_err(c, -2, "solve_multi: upload the Laplacian and LDLinv first");
   const auto t_start = std::chrono::steady_clock::now();
   const int64_t n = c->n;
-  // spmv, ldl, vec, iterations; forward groups, tail inputs, tail, backward groups
-  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // spmv, ldl, vec, iterations; forward groups, tail inputs, tail, backward
+  // groups; sparse tail forward, dense product, sparse tail backward
+  double acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   int64_t maxit = 0;
   for (int64_t c0 = 0; c0 < k; c0 += kMultiMax) {
     const int kc = (int)std::min<int64_t>(kMultiMax, k - c0);
@@ -1383,6 +1411,9 @@ int solve_multi_device(lapgpu_ctx* c, const double* d_B, double* d_X, int64_t k,
     c->t_tail_ext = acc[5] / acc[3];
     c->t_tail = acc[6] / acc[3];
     c->t_bgroups = acc[7] / acc[3];
+    c->t_tail_fwd = acc[8] / acc[3];
+    c->t_dense = acc[9] / acc[3];
+    c->t_tail_bwd = acc[10] / acc[3];
   }
   c->t_total =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count() * 1e3;
@@ -1936,6 +1967,20 @@ int lapgpu_dense_info(lapgpu_ctx* c, int64_t* k, int64_t* sparse_vertices, int64
   return 0;
 }
 
+int lapgpu_sweep_entries(lapgpu_ctx* c, int64_t* groups_f, int64_t* groups_b, int64_t* tail_f,
+                         int64_t* tail_b, int64_t* dense_k) {
+  if (!c) return -1;
+  if (!c->has_F) return set_err(c, -2, "sweep_entries: no LDLinv uploaded");
+  const Plan& P = c->plan;
+  const int64_t T0 = P.tail.T0, n = P.n;
+  if (groups_f) *groups_f = P.Fptr[T0];
+  if (groups_b) *groups_b = P.Bptr[T0];
+  if (tail_f) *tail_f = P.Fptr[n] - P.Fptr[T0];
+  if (tail_b) *tail_b = P.Bptr[n] - P.Bptr[T0];
+  if (dense_k) *dense_k = P.dense.K;
+  return 0;
+}
+
 int lapgpu_factor_info(lapgpu_ctx* c, int64_t* nlevels, int64_t* nwide,
                        int64_t* tail_start, int64_t* nnzF) {
   if (!c) return -1;
@@ -1978,6 +2023,15 @@ int lapgpu_last_sweep_timings(lapgpu_ctx* c, double* fgroups_ms, double* tail_ex
   if (tail_ext_ms) *tail_ext_ms = c->t_tail_ext;
   if (tail_ms) *tail_ms = c->t_tail;
   if (bgroups_ms) *bgroups_ms = c->t_bgroups;
+  return 0;
+}
+
+int lapgpu_last_dense_timings(lapgpu_ctx* c, double* tail_fwd_ms, double* dense_ms,
+                              double* tail_bwd_ms) {
+  if (!c) return -1;
+  if (tail_fwd_ms) *tail_fwd_ms = c->t_tail_fwd;
+  if (dense_ms) *dense_ms = c->t_dense;
+  if (tail_bwd_ms) *tail_bwd_ms = c->t_tail_bwd;
   return 0;
 }
 

@@ -60,6 +60,8 @@ EXPORTS = (
     "lapgpu_last_error",
     "lapgpu_destroy",
     "lapgpu_set_dense_tail",
+    "lapgpu_sweep_entries",
+    "lapgpu_last_dense_timings",
     "lapgpu_dense_info",
     "lapgpu_batch_create",
     "lapgpu_batch_add_graph",
@@ -119,6 +121,8 @@ def load() -> ctypes.CDLL:
     pi32 = ctypes.POINTER(ctypes.c_int32)
     L.lapgpu_batch_create.argtypes = [ctypes.POINTER(P), ci]
     L.lapgpu_set_dense_tail.argtypes = [P, i64]
+    L.lapgpu_sweep_entries.argtypes = [P, pi64, pi64, pi64, pi64, pi64]
+    L.lapgpu_last_dense_timings.argtypes = [P, ctypes.POINTER(dbl), ctypes.POINTER(dbl), ctypes.POINTER(dbl)]
     L.lapgpu_dense_info.argtypes = [P, pi64, pi64, pi64]
     L.lapgpu_batch_add_graph.argtypes = [P, i64, P, P, P, i64, P, P, P, P, P, ci, pi32]
     L.lapgpu_batch_remove_graph.argtypes = [P, ctypes.c_int32]
@@ -373,6 +377,18 @@ class Context:
         """Largest dense tail block the planner may choose (0: none); takes
         effect at the next upload_ldli."""
         self._check(self._lib.lapgpu_set_dense_tail(self._h, int(max_k)), "lapgpu_set_dense_tail")
+
+    def last_dense_timings(self) -> dict:
+        v = [ctypes.c_double() for _ in range(3)]
+        self._check(self._lib.lapgpu_last_dense_timings(self._h, *(ctypes.byref(t) for t in v)),
+                    "lapgpu_last_dense_timings")
+        return {"tail_fwd_ms": v[0].value, "dense_ms": v[1].value, "tail_bwd_ms": v[2].value}
+
+    def sweep_entries(self) -> dict:
+        v = [ctypes.c_int64() for _ in range(5)]
+        self._check(self._lib.lapgpu_sweep_entries(self._h, *(ctypes.byref(t) for t in v)), "lapgpu_sweep_entries")
+        return {"groups_f": v[0].value, "groups_b": v[1].value, "tail_f": v[2].value, "tail_b": v[3].value,
+                "dense_k": v[4].value}
 
     def dense_info(self) -> dict:
         v = [ctypes.c_int64() for _ in range(3)]

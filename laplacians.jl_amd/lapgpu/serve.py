@@ -25,7 +25,9 @@ over time; each is answered by the same solve closure as approxchol_lap's
   contexts, so the dispatch loop itself never waits on an upload or a free.
   A graph that comes with its request is planned with a small level-group
   budget (`new_graph_budget`): fewer composite rows to build on the host, a
-  few more launches on the device for its one solve.
+  few more launches on the device for its one solve; and without a dense
+  tail block (lapgpu_set_dense_tail), whose matrix costs more to build than
+  one solve saves.
 
 Every result carries its latency (arrival to completion); `summary` gives
 solves/s and p50/p99 latency.
@@ -90,11 +92,28 @@ class _Graph:
     solves: int = 0  # plain / multi-RHS solves run on the graph's context
 
 
+DENSE_MAX = 18432  # the largest dense tail block a context may hold (its LDS tail)
+
+
+def dense_bytes_estimate(n: int) -> int:
+    """The largest dense tail block a context of n vertices can hold (8 K^2
+    bytes, K <= min(n, 18432)); the planner usually chooses less, and the
+    reservation is corrected once the context reports its block."""
+    k = min(n, DENSE_MAX)
+    return 8 * k * ((k + 255) // 256 * 256)
+
+
 def graph_bytes_estimate(n: int, nnz_l: int, nnzf: int) -> int:
     """Device bytes of a graph for admission: la (12 B/entry), the level-
-    ordered factor in both sweep directions (about 16 B/entry each), and
-    8 workspace vectors."""
+    ordered factor in both sweep directions (about 16 B/entry each) and 8
+    workspace vectors (a context adds its dense tail block,
+    dense_bytes_estimate)."""
     return 12 * nnz_l + 32 * nnzf + 8 * 8 * n + 16 * n
+
+
+def dense_correction(ctx, n: int) -> int:
+    """Actual minus reserved bytes of a context's dense tail block."""
+    return ctx.dense_info()["bytes"] - dense_bytes_estimate(n)
 
 
 def multi_bytes_estimate(n: int, k: int) -> int:
@@ -112,8 +131,10 @@ class Server:
                  keep_x: bool = False, max_rhs: int = 64, batch_window: float = 0.5,
                  max_wait: float = 1.0, new_graph_budget: tuple | None = (8, 2.0, 16384),
                  eager_new: bool = True, quick_workers: int = 4, quick_max_n: int = 300_000,
-                 blocking_waits: bool = True):
+                 blocking_waits: bool = True, keep_ids: set | None = None):
         self.device = device
+        # the solutions kept in the results: all (keep_x) or those of keep_ids
+        self.keep_ids = keep_ids
         self.batch = _native.Batch(device)
         self.hbm_budget = hbm_budget if hbm_budget is not None else 200 * 2**30
         self.wg_max_n = wg_max_n
@@ -173,6 +194,8 @@ class Server:
         non-resident graphs when needed; None when it does not fit."""
         n = la.shape[0]
         need = graph_bytes_estimate(n, la.nnz, ldli.nnz)
+        if n > self.wg_max_n:
+            need += dense_bytes_estimate(n)  # contexts only (not the batched engine)
         if self.hbm_used + need > self.hbm_budget:
             self._evict(need)
             if self.hbm_used + need > self.hbm_budget:
@@ -187,7 +210,7 @@ class Server:
                 need += multi_bytes_estimate(n, k)
         return need, k
 
-    def _upload_context(self, la, ldli, budget=None):
+    def _upload_context(self, la, ldli, budget=None, dense_max=None):
         c0 = time.thread_time()
         t0 = time.perf_counter()
         ctx = _native.Context(self.device)
@@ -195,6 +218,8 @@ class Server:
             ctx.set_wait_mode(True)
         if budget is not None:
             ctx.set_group_budget(*budget)
+        if dense_max is not None:
+            ctx.set_dense_tail(dense_max)
         t1 = time.perf_counter()
         ctx.upload_lap(la)
         t2 = time.perf_counter()
@@ -221,7 +246,8 @@ class Server:
         if n <= self.wg_max_n:
             self._admit_wg(key, la, ldli, resident)
         else:
-            g = _Graph(key, n, "plain", need, resident, self._upload_context(la, ldli), max_rhs=k)
+            ctx = self._upload_context(la, ldli)
+            g = _Graph(key, n, "plain", need + dense_correction(ctx, n), resident, ctx, max_rhs=k)
             self.graphs[key] = g
             self.hbm_used += g.bytes
         return True
@@ -269,8 +295,8 @@ class Server:
 
         def finish(r: Request, info: dict, x, path: str):
             lat = time.perf_counter() - arrival_abs[r.rid]
-            results[r.rid] = Result(r.rid, lat, info["its"], info["nr"], info["stop"], path,
-                                    x if self.keep_x else None)
+            keep = self.keep_x or (self.keep_ids is not None and r.rid in self.keep_ids)
+            results[r.rid] = Result(r.rid, lat, info["its"], info["nr"], info["stop"], path, x if keep else None)
 
         def route(g: _Graph, r: Request):
             if g.kind == "wg":
@@ -356,14 +382,17 @@ class Server:
                 else:
                     # reserved now, uploaded on a worker thread
                     self.hbm_used += need
-                    upload_futs[self._factor_pool.submit(self._upload_context, la, ldli, self.new_graph_budget)] = (
+                    upload_futs[self._factor_pool.submit(self._upload_context, la, ldli, self.new_graph_budget, 0)] = (
                         key, r, la.shape[0], need, k)
                 self.stats["admit_s"] += time.perf_counter() - ta
                 self.stats["admits"] += 1
                 progressed = True
             for fut in [f for f in upload_futs if f.done()]:
                 key, r, n, need, k = upload_futs.pop(fut)
-                g = _Graph(key, n, "plain", need, False, fut.result(), max_rhs=k)
+                ctx = fut.result()
+                corr = dense_correction(ctx, n)
+                g = _Graph(key, n, "plain", need + corr, False, ctx, max_rhs=k)
+                self.hbm_used += corr
                 self.graphs[key] = g
                 route(g, r)
                 progressed = True

@@ -96,20 +96,33 @@ def config5(nreq: int = 10000, qps: float = 200.0, ngraphs: int = 32, min_n: flo
     times at `qps`; each brings a new graph (n log-uniform in [min_n,
     new_max_n], default max_n) with probability new_frac, else names a
     resident graph uniformly.  Returns (resident graphs, requests)."""
-    from .serve import Request
-
     rng = np.random.default_rng(seed)
     lo, hi = np.log(min_n), np.log(max_n)
     sizes = np.exp(rng.uniform(lo, hi, ngraphs))
     G = [serve_graph(rng, nv) for nv in sizes]
+    return G, _requests(rng, G, nreq, qps, min_n, new_max_n if new_max_n else max_n, new_frac, tol, 0)
+
+
+def serve_requests(G, nreq: int, qps: float = 200.0, min_n: float = 1e4, new_max_n: float = 4e6,
+                   new_frac: float = 0.2, tol: float = 1e-8, seed: int = 55, rid0: int = 0):
+    """Another stream of the serving kind over the resident graphs G (its own
+    seed; request ids from rid0)."""
+    return _requests(np.random.default_rng(seed), G, nreq, qps, min_n, new_max_n, new_frac, tol, rid0)
+
+
+def _requests(rng, G, nreq, qps, min_n, new_max_n, new_frac, tol, rid0):
+    from .serve import Request
+
     arr = np.cumsum(rng.exponential(1.0 / qps, nreq))
-    new_sizes = np.exp(rng.uniform(lo, np.log(new_max_n if new_max_n else max_n), nreq))
+    new_sizes = np.exp(rng.uniform(np.log(min_n), np.log(new_max_n), nreq))
     reqs = []
     for i in range(nreq):
         if rng.uniform() < new_frac:
             g = serve_graph(rng, new_sizes[i])
-            reqs.append(Request(i, float(arr[i]), _center(rng.standard_normal(g.shape[0])), new_graph=g, tol=tol))
+            reqs.append(Request(rid0 + i, float(arr[i]), _center(rng.standard_normal(g.shape[0])), new_graph=g,
+                                tol=tol))
         else:
             gi = int(rng.integers(len(G)))
-            reqs.append(Request(i, float(arr[i]), _center(rng.standard_normal(G[gi].shape[0])), graph=gi, tol=tol))
-    return G, reqs
+            reqs.append(Request(rid0 + i, float(arr[i]), _center(rng.standard_normal(G[gi].shape[0])), graph=gi,
+                                tol=tol))
+    return reqs
