@@ -305,7 +305,8 @@ def serve_leg(args) -> dict:
     srv = Server(0, hbm_budget=200 * 2**30, plain_workers=args.serve_workers,
                  factor_workers=args.factor_workers, batch_window=args.batch_window,
                  max_wait=args.max_wait, quick_workers=args.quick_workers,
-                 blocking_waits=not args.spin_waits, wg_max_n=args.wg_max_n, keep_ids=sample)
+                 blocking_waits=not args.spin_waits, wg_max_n=args.wg_max_n, keep_ids=sample,
+                 new_wg_max_n=args.new_wg_max_n)
     t0 = time.time()
     for i, (g, f) in enumerate(zip(G, F)):
         assert srv.register(i, g, ldli=f)
@@ -322,6 +323,10 @@ def serve_leg(args) -> dict:
             f"{summary(wres, time.perf_counter() - t0)}")
     import resource
 
+    # the pipeline figures below cover the timed stream only
+    for lst in (srv.factor_log, srv.factor_cpu_log, srv.upload_log, srv.upload_cpu_log, srv.wg_upload_log,
+                srv.solve_log):
+        lst.clear()
     ru0 = resource.getrusage(resource.RUSAGE_SELF)
     t0 = time.perf_counter()
     res = srv.run(reqs, progress=lambda d, t, e: log(f"  served {d} of {t} requests at {e:.0f} s"))
@@ -338,14 +343,30 @@ def serve_leg(args) -> dict:
             f"{np.percentile(lat, 50):.2f} s p99 {np.percentile(lat, 99):.2f} s")
     nf = np.array(srv.factor_log)
     up = np.array(srv.upload_log).reshape(-1, 4)
+    wgu = np.array(srv.wg_upload_log).reshape(-1, 2)
     new_n = [r.new_graph.shape[0] for r in reqs if r.new_graph is not None]
     pipeline = {"new_graphs": len(new_n), "new_graph_vertices": int(sum(new_n)),
                 "factor_s_median": float(np.median(nf)) if len(nf) else None,
                 "factor_s_sum": float(nf.sum()) if len(nf) else None,
                 "context_upload_s_median": float(np.median(up[:, 0])) if len(up) else None,
                 "context_upload_s_sum": float(up[:, 0].sum()) if len(up) else None,
+                "contexts_uploaded": int(len(up)),
+                "batched_upload_s_median": float(np.median(wgu[:, 0])) if len(wgu) else None,
+                "batched_upload_s_sum": float(wgu[:, 0].sum()) if len(wgu) else None,
+                "batched_uploads": int(len(wgu)),
                 "host_cpu_s": cpu_s, "factor_workers": args.factor_workers}
     log(f"serve: new-graph pipeline {pipeline}")
+    # context solves (plain and multi-RHS) of the timed stream: count,
+    # columns, and the worker-seconds they held
+    sl = np.array([(n, k, dt) for n, k, dt, _, _ in srv.solve_log]).reshape(-1, 3)
+    if len(sl):
+        log(f"serve: context solves {len(sl)}, columns mean {sl[:, 1].mean():.1f} max {int(sl[:, 1].max())}, "
+            f"worker-seconds {sl[:, 2].sum():.0f} over {wall:.1f} s")
+        for lo, hi in ((5e4, 3e5), (3e5, 1e6), (1e6, 5e6)):
+            m = (sl[:, 0] > lo) & (sl[:, 0] <= hi)
+            if m.any():
+                log(f"  n in ({lo:g},{hi:g}]: {int(m.sum())} solves, {int(sl[m, 1].sum())} columns, "
+                    f"{sl[m, 2].sum():.0f} worker-s, median {np.median(sl[m, 2]):.2f} s")
     # true residual ||L x - b|| / ||b|| on the kept sample
     byr = {r.rid: r for r in reqs}
     rr = []
@@ -681,6 +702,8 @@ def main() -> int:
     ap.add_argument("--new-frac", type=float, default=0.2)
     ap.add_argument("--serve-workers", type=int, default=10, help="serving: graphs solved concurrently")
     ap.add_argument("--factor-workers", type=int, default=12, help="serving: host threads factoring new graphs")
+    ap.add_argument("--new-wg-max-n", type=int, default=200000,
+                    help="serving: largest graph that comes with its request served by the batched engine")
     ap.add_argument("--wg-max-n", type=int, default=50000,
                     help="serving: graphs up to this size are solved by the batched engine")
     ap.add_argument("--spin-waits", action="store_true",

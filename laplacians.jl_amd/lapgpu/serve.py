@@ -5,8 +5,10 @@ over time; each is answered by the same solve closure as approxchol_lap's
 (src/approxChol.jl:1624) on the device.  The loop
 
 * admits a request as soon as it arrives: graphs of at most `wg_max_n`
-  vertices are queued for the next batched launch (one workgroup per
-  request, lapgpu_batch_submit_host); larger graphs queue per graph, and a
+  vertices (`new_wg_max_n` for a graph that comes with its request) are
+  queued for the next batched launch (one workgroup per request,
+  lapgpu_batch_submit_host, largest graphs first in the launch so that the
+  longest solves start first); larger graphs queue per graph, and a
   worker solves everything queued for that graph in one multi-RHS solve
   (lapgpu_solve_multi, up to the graph's `max_rhs` columns sharing every
   launch; lapgpu_solve when only one is queued) on the graph's own stream;
@@ -20,8 +22,9 @@ over time; each is answered by the same solve closure as approxchol_lap's
   idle non-resident graphs are evicted (least recently used first), and a
   request waits while its graph cannot be admitted;
 * factors new graphs on host threads (liblapfactor, re-entrant) while the
-  device keeps solving; the same threads create a large graph's context and
-  upload it (its HBM is reserved first), and one more thread closes dropped
+  device keeps solving; the same threads plan and upload it (to the batched
+  engine, or to a context of its own when it is larger; its HBM is reserved
+  first), and one more thread closes dropped
   contexts, so the dispatch loop itself never waits on an upload or a free.
   A graph that comes with its request is planned with a small level-group
   budget (`new_graph_budget`): fewer composite rows to build on the host, a
@@ -131,13 +134,18 @@ class Server:
                  keep_x: bool = False, max_rhs: int = 64, batch_window: float = 0.5,
                  max_wait: float = 1.0, new_graph_budget: tuple | None = (8, 2.0, 16384),
                  eager_new: bool = True, quick_workers: int = 4, quick_max_n: int = 300_000,
-                 blocking_waits: bool = True, keep_ids: set | None = None):
+                 blocking_waits: bool = True, keep_ids: set | None = None,
+                 new_wg_max_n: int | None = 200_000):
         self.device = device
         # the solutions kept in the results: all (keep_x) or those of keep_ids
         self.keep_ids = keep_ids
         self.batch = _native.Batch(device)
         self.hbm_budget = hbm_budget if hbm_budget is not None else 200 * 2**30
         self.wg_max_n = wg_max_n
+        # a graph that comes with its request is solved once: on the batched
+        # engine up to new_wg_max_n vertices (one CU for the solve, and no
+        # context, stream or level groups to set up on the host)
+        self.new_wg_max_n = max(wg_max_n, new_wg_max_n or 0)
         self.max_launch = max_launch
         self.keep_x = keep_x
         self.max_rhs = max_rhs
@@ -166,6 +174,8 @@ class Server:
         self.factor_cpu_log: list = []
         self.upload_cpu_log: list = []
         self.upload_log: list = []
+        # (seconds, CPU seconds) of each batched-engine graph upload
+        self.wg_upload_log: list = []
         self.graphs: "OrderedDict[object, _Graph]" = OrderedDict()
         self.hbm_used = 0
         self._plain_pool = ThreadPoolExecutor(max_workers=plain_workers)
@@ -189,19 +199,20 @@ class Server:
             ldli = build_ldli(a, seed=seed)
         return self._admit(key, la, ldli, resident)
 
-    def _reserve(self, la, ldli):
-        """HBM footprint and multi-RHS width of a graph, evicting idle
-        non-resident graphs when needed; None when it does not fit."""
+    def _reserve(self, la, ldli, wg: bool):
+        """HBM footprint and multi-RHS width of a graph on the batched engine
+        (wg) or its own context, evicting idle non-resident graphs when
+        needed; None when it does not fit."""
         n = la.shape[0]
         need = graph_bytes_estimate(n, la.nnz, ldli.nnz)
-        if n > self.wg_max_n:
+        if not wg:
             need += dense_bytes_estimate(n)  # contexts only (not the batched engine)
         if self.hbm_used + need > self.hbm_budget:
             self._evict(need)
             if self.hbm_used + need > self.hbm_budget:
                 return None
         k = 1
-        if n > self.wg_max_n:
+        if not wg:
             # the widest multi-RHS workspace that still fits the budget
             k = self.max_rhs
             while k > 1 and self.hbm_used + need + multi_bytes_estimate(n, k) > self.hbm_budget:
@@ -229,22 +240,30 @@ class Server:
         self.upload_cpu_log.append(time.thread_time() - c0)
         return ctx
 
-    def _admit_wg(self, key, la, ldli, resident) -> _Graph:
-        n = la.shape[0]
+    def _upload_wg(self, la, ldli):
+        """Plan and upload a graph to the batched engine (thread-safe:
+        add_graph plans outside the batch's lock); returns its id."""
+        c0 = time.thread_time()
+        t0 = time.perf_counter()
         gid = self.batch.add_graph(la, ldli)
+        self.wg_upload_log.append((time.perf_counter() - t0, time.thread_time() - c0))
+        return gid
+
+    def _wg_graph(self, key, n, gid, resident) -> _Graph:
         g = _Graph(key, n, "wg", self.batch.graph_info(gid)["device_bytes"] + 64 * n, resident, gid)
         self.graphs[key] = g
         self.hbm_used += g.bytes
         return g
 
     def _admit(self, key, la, ldli, resident) -> bool:
-        res = self._reserve(la, ldli)
+        n = la.shape[0]
+        wg = n <= self.wg_max_n
+        res = self._reserve(la, ldli, wg)
         if res is None:
             return False
         need, k = res
-        n = la.shape[0]
-        if n <= self.wg_max_n:
-            self._admit_wg(key, la, ldli, resident)
+        if wg:
+            self._wg_graph(key, n, self._upload_wg(la, ldli), resident)
         else:
             ctx = self._upload_context(la, ldli)
             g = _Graph(key, n, "plain", need + dense_correction(ctx, n), resident, ctx, max_rhs=k)
@@ -370,30 +389,36 @@ class Server:
             for _ in range(len(waiting_admission)):
                 r, (la, ldli) = waiting_admission.popleft()
                 ta = time.perf_counter()
-                res = self._reserve(la, ldli)
+                wg = la.shape[0] <= self.new_wg_max_n
+                res = self._reserve(la, ldli, wg)
                 if res is None:
                     waiting_admission.append((r, (la, ldli)))
                     continue
                 self._anon += 1
                 key = ("new", self._anon)
                 need, k = res
-                if la.shape[0] <= self.wg_max_n:
-                    route(self._admit_wg(key, la, ldli, resident=False), r)
+                # reserved now (a batched-engine graph by the estimate until
+                # it reports its own bytes), uploaded on a worker thread
+                self.hbm_used += need
+                if wg:
+                    fut = self._factor_pool.submit(self._upload_wg, la, ldli)
                 else:
-                    # reserved now, uploaded on a worker thread
-                    self.hbm_used += need
-                    upload_futs[self._factor_pool.submit(self._upload_context, la, ldli, self.new_graph_budget, 0)] = (
-                        key, r, la.shape[0], need, k)
+                    fut = self._factor_pool.submit(self._upload_context, la, ldli, self.new_graph_budget, 0)
+                upload_futs[fut] = (key, r, la.shape[0], need, k, wg)
                 self.stats["admit_s"] += time.perf_counter() - ta
                 self.stats["admits"] += 1
                 progressed = True
             for fut in [f for f in upload_futs if f.done()]:
-                key, r, n, need, k = upload_futs.pop(fut)
-                ctx = fut.result()
-                corr = dense_correction(ctx, n)
-                g = _Graph(key, n, "plain", need + corr, False, ctx, max_rhs=k)
-                self.hbm_used += corr
-                self.graphs[key] = g
+                key, r, n, need, k, wg = upload_futs.pop(fut)
+                if wg:
+                    self.hbm_used -= need
+                    g = self._wg_graph(key, n, fut.result(), False)
+                else:
+                    ctx = fut.result()
+                    corr = dense_correction(ctx, n)
+                    g = _Graph(key, n, "plain", need + corr, False, ctx, max_rhs=k)
+                    self.hbm_used += corr
+                    self.graphs[key] = g
                 route(g, r)
                 progressed = True
             # idle graphs whose batching window has ended
@@ -440,6 +465,8 @@ class Server:
                 rest.extend(small)
                 small.clear()
                 small.extend(rest)
+                # workgroups start in request order: the longest solves first
+                take.sort(key=lambda it: -it[0].n)
                 h = self.batch.submit_host([g.handle for g, _ in take], [r.b for _, r in take],
                                            tol=key[0], maxits=key[1])
                 if h is None:

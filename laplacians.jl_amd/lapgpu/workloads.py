@@ -111,18 +111,27 @@ def serve_requests(G, nreq: int, qps: float = 200.0, min_n: float = 1e4, new_max
 
 
 def _requests(rng, G, nreq, qps, min_n, new_max_n, new_frac, tol, rid0):
+    """Arrival times, kinds and sizes from `rng` in request order; each
+    request's new graph and right-hand side from its own stream
+    default_rng([entropy, request id]) (entropy drawn from `rng`), built on
+    host threads (numpy fills arrays without holding the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
+
     from .serve import Request
 
     arr = np.cumsum(rng.exponential(1.0 / qps, nreq))
     new_sizes = np.exp(rng.uniform(np.log(min_n), np.log(new_max_n), nreq))
-    reqs = []
-    for i in range(nreq):
-        if rng.uniform() < new_frac:
-            g = serve_graph(rng, new_sizes[i])
-            reqs.append(Request(rid0 + i, float(arr[i]), _center(rng.standard_normal(g.shape[0])), new_graph=g,
-                                tol=tol))
-        else:
-            gi = int(rng.integers(len(G)))
-            reqs.append(Request(rid0 + i, float(arr[i]), _center(rng.standard_normal(G[gi].shape[0])), graph=gi,
-                                tol=tol))
-    return reqs
+    is_new = rng.uniform(size=nreq) < new_frac
+    which = rng.integers(len(G), size=nreq)
+    entropy = int(rng.integers(2**62))
+
+    def make(i):
+        r = np.random.default_rng([entropy, rid0 + i])
+        if is_new[i]:
+            g = serve_graph(r, new_sizes[i])
+            return Request(rid0 + i, float(arr[i]), _center(r.standard_normal(g.shape[0])), new_graph=g, tol=tol)
+        gi = int(which[i])
+        return Request(rid0 + i, float(arr[i]), _center(r.standard_normal(G[gi].shape[0])), graph=gi, tol=tol)
+
+    with ThreadPoolExecutor(8) as ex:
+        return list(ex.map(make, range(nreq)))

@@ -140,3 +140,33 @@ def test_serve_mixed_tolerances_each_meet_their_own():
             assert alone.its == out.its and np.array_equal(alone.x, out.x), (r.tol, alone.its, out.its)
     finally:
         srv.close()
+
+
+def test_serve_new_graphs_take_both_engines():
+    """Graphs that come with their request: up to new_wg_max_n vertices on
+    the batched engine (above the resident threshold wg_max_n), larger ones
+    on a context of their own; both are factored and uploaded on worker
+    threads, and each result meets its tol on the true residual."""
+    rng = np.random.default_rng(21)
+    mid = graphs.weighted_grid2(60, 50, rng)  # 3,000: batched as a new graph, a context if resident
+    big = graphs.weighted_grid3(20, 20, 20, rng)  # 8,000: a context either way
+    srv = Server(0, wg_max_n=2000, new_wg_max_n=5000, keep_x=True)
+    try:
+        reqs = [Request(0, 0.0, _rhs(3000, rng), new_graph=mid, tol=1e-8),
+                Request(1, 0.0, _rhs(8000, rng), new_graph=big, tol=1e-8),
+                Request(2, 0.001, _rhs(3000, rng), new_graph=mid, tol=1e-8)]
+        res = srv.run(reqs)
+        assert [o.path for o in res] == ["batched", "plain", "batched"], [o.path for o in res]
+        for r, out in zip(reqs, res):
+            assert out.stop == "converged", out
+            a = r.new_graph
+            assert_relres(lapgpu.lap(a), out.x, r.b, 1e-8, lambda: lapgpu.build_ldli(a, seed=1))
+        # the one-off graphs are dropped after their solves
+        assert not srv.graphs and srv.hbm_used == 0
+        # the batched result is the engine's own answer for that factor
+        g = srv.batch.add_graph(lapgpu.lap(mid), lapgpu.build_ldli(mid, seed=1))
+        xs, infos = srv.batch.solve([g], [reqs[0].b], tol=1e-8)
+        assert infos[0]["its"] == res[0].its and np.array_equal(xs[0], res[0].x)
+        srv.batch.remove_graph(g)
+    finally:
+        srv.close()
