@@ -45,7 +45,11 @@ enum lapgpu_stop {
   LAPGPU_STAG_BETA = 6,   /* beta*sum|p| < eps*sum|z|          pcg.jl:406  */
   LAPGPU_INF_RHO = 7,     /* isinf(rho)                        pcg.jl:385  */
   LAPGPU_INF_BETA = 8,    /* isinf(beta)                       pcg.jl:397  */
-  LAPGPU_ZERO_RHS = 9     /* nb == 0: returns zeros            pcg.jl:269  */
+  LAPGPU_ZERO_RHS = 9,    /* nb == 0: returns zeros            pcg.jl:269  */
+  LAPGPU_CANCELLED = 10   /* lapgpu_cancel / lapgpu_batch_cancel: stopped at
+                             an iteration boundary, best iterate returned (no
+                             reference counterpart; the reference's only
+                             interruption is its time limit, maxtime) */
 };
 
 /* lapgpu_solve flags */
@@ -123,6 +127,19 @@ int lapgpu_solve_multi_device(lapgpu_ctx* ctx, const double* d_B, double* d_X,
                               double maxtime, int stag_test, int flags,
                               int64_t* its_out, double* relres_out,
                               int* stop_reason_out);
+
+/* Cancel the solve running on ctx from another thread (no reference
+ * counterpart; for serving, where a client may withdraw a request): column
+ * j of the running lapgpu_solve_multi call, or every column when j == -1
+ * (lapgpu_solve has one column, 0).  The solve stops those columns at the
+ * next iteration boundary the host polls, with stop reason
+ * LAPGPU_CANCELLED, their best iterate so far and its iteration count; the
+ * other columns run on unchanged.  Returns 0 when a solve was running,
+ * LAPGPU_IDLE when none was (nothing is cancelled), -1 for a column out of
+ * range.  Thread-safe; the only entry point that may run concurrently with
+ * a solve on the same context. */
+#define LAPGPU_IDLE 2
+int lapgpu_cancel(lapgpu_ctx* ctx, int64_t column);
 
 /* Apply the preconditioner once: z = LDLsolver(ldli, r)
  * (src/approxChol.jl:1347-1366), host buffers. */
@@ -318,11 +335,22 @@ int lapgpu_batch_query(lapgpu_batch* B, int32_t slot, int* done);
 int lapgpu_batch_collect(lapgpu_batch* B, int32_t slot, double* x_concat, int64_t* its_out,
                          double* relres_out, int32_t* stop_out);
 
+/* Cancel request `index` (its position in the submit call) of the launch
+ * on `slot` (no reference counterpart; for serving): its workgroup stops at
+ * its next iteration boundary with stop reason LAPGPU_CANCELLED and its
+ * best iterate so far; every other request of the launch runs on unchanged
+ * (each has its own workgroup and nothing is shared).  The flag is read by
+ * the device from host memory while the launch runs; a request that has
+ * already finished keeps its result.  The slot must hold the launch the
+ * caller means (between its submit and its collect). */
+int lapgpu_batch_cancel(lapgpu_batch* B, int32_t slot, int64_t index);
+
 /* Allocation-free form of lapgpu_batch_solve_device (no counterpart in the
  * reference): for callers that cannot allocate device memory, such as code
  * under HIP graph capture, tight serving loops and harnesses that forbid
  * hipMalloc.  It calls no hipMalloc, hipFree or pinned allocation: the
- * requests' work vectors (8 n doubles each), request table and graph
+ * requests' work vectors (8 n doubles each, plus up to 1024 partial
+ * slots), request table and graph
  * descriptors live in a static device array of 64 MiB per device (about
  * 1,000,000 vertices over all requests of one call), and at most 1024
  * requests go in one call; larger calls fail with a message.  Same kernel

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <thread>
 #include <cmath>
@@ -30,6 +31,7 @@ using namespace lapgpu;
 
 static_assert(kTailThreads == kTailLanes, "tail workgroup width = lanes of a tail step");
 static_assert(kTailCapLds == kTailCap, "tail capacity: planner and kernels agree");
+static_assert(kPartSlots == kPartCap, "partial slots: planner and kernels agree");
 
 namespace lapgpu {
 
@@ -115,6 +117,15 @@ struct lapgpu_ctx {
   // 2-D grid with 64 right-hand sides the slowest column needed 62
   // iterations at 1e-4 against 47 at 1e-2 (FP64: 45)
   double switch_tol = 1e-2;
+
+  // cancellation (lapgpu_cancel, from another thread): one flag per column
+  // of the running solve call; cancel_any is polled by the solve loop
+  std::mutex cancel_mu;
+  std::atomic<bool> cancel_any{false};
+  std::vector<char> cancel_cols;
+  bool running = false;
+  // global column index of column 0 of the multi-RHS chunk being solved
+  int64_t col_base = 0, chunk0 = 0;
 
   // device graph
   int *perm = nullptr, *Lptr = nullptr, *Lidx = nullptr;
@@ -833,6 +844,45 @@ void enqueue_iteration(lapgpu_ctx* c, bool prof, bool replace) {
   if (prof) (void)hipEventRecord(c->ev[4], c->stream);
 }
 
+// The running-solve window of lapgpu_cancel: k columns, flags cleared.
+struct RunGuard {
+  lapgpu_ctx* c;
+  RunGuard(lapgpu_ctx* c_, int64_t k) : c(c_) {
+    std::lock_guard<std::mutex> g(c->cancel_mu);
+    c->cancel_cols.assign((size_t)k, 0);
+    c->cancel_any.store(false);
+    c->running = true;
+  }
+  ~RunGuard() {
+    std::lock_guard<std::mutex> g(c->cancel_mu);
+    c->running = false;
+    c->cancel_any.store(false);
+    c->cancel_cols.clear();
+  }
+};
+
+// Columns [base, base + k) cancelled since the last call, as a bit mask
+// (cleared as they are taken; flags of earlier columns, already solved,
+// are dropped).
+uint64_t take_cancel(lapgpu_ctx* c, int64_t base, int k) {
+  if (!c->cancel_any.load(std::memory_order_acquire)) return 0;
+  std::lock_guard<std::mutex> g(c->cancel_mu);
+  uint64_t mask = 0;
+  bool left = false;
+  for (size_t j = 0; j < c->cancel_cols.size(); ++j) {
+    if (!c->cancel_cols[j]) continue;
+    const int64_t r = (int64_t)j - base;
+    if (r < k) {
+      if (r >= 0) mask |= 1ull << r;
+      c->cancel_cols[j] = 0;
+    } else {
+      left = true;
+    }
+  }
+  c->cancel_any.store(left);
+  return mask;
+}
+
 int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
                  int64_t maxits, double maxtime, int stag_test, int flags,
                  int64_t* its_out, double* relres_out, int* stop_out) {
@@ -941,6 +991,11 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
       HIPC(c, wait_event(c, c->ring_ev[ps]));
       ++polled;
       if (c->hflags[ps]) done = true;
+    }
+    // cancelled: stop after the iterations already enqueued
+    if (!done && take_cancel(c, 0, 1)) {
+      hipLaunchKernelGGL(k_cancel, dim3(1), dim3(kWave), 0, c->stream, c->st, 1, 1ull);
+      done = true;
     }
   }
   hipLaunchKernelGGL(k_final_copy, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
@@ -1361,6 +1416,12 @@ int solve_multi_chunk(lapgpu_ctx* c, const double* d_B, double* d_X, int k, doub
       ++polled;
       if (c->hflags[ps]) done = true;
     }
+    // cancelled columns stop after the iterations already enqueued; the
+    // others run on (all stopped: the next iteration raises flags[0])
+    if (!done) {
+      if (const uint64_t mk = take_cancel(c, c->chunk0, k))
+        hipLaunchKernelGGL(k_cancel, dim3(1), dim3(kWave), 0, c->stream, m.st, k, mk);
+    }
   }
   hipLaunchKernelGGL(k_mfinal<K>, dim3(kMRed), dim3(kBlock), 0, c->stream, n, k, c->perm, m.x,
                      m.bestx, d_X, m.st);
@@ -1396,6 +1457,7 @@ int solve_multi_device(lapgpu_ctx* c, const double* d_B, double* d_X, int64_t k,
     int K = 8;
     while (K < kc) K *= 2;
     int rc = 0;
+    c->chunk0 = c->col_base + c0;
     LAPGPU_DISPATCH_K(K, rc = solve_multi_chunk<K_>, c, d_B + c0 * n, d_X + c0 * n, kc, tol,
                       maxits, maxtime, stag_test, flags, its_out ? its_out + c0 : nullptr,
                       relres_out ? relres_out + c0 : nullptr, stop_out ? stop_out + c0 : nullptr,
@@ -1782,8 +1844,24 @@ int lapgpu_solve_device(lapgpu_ctx* c, const double* d_b, double* d_x,
   if (n != c->n) return set_err(c, -1, "solve: length of b differs from n");
   if (!d_b || !d_x) return set_err(c, -1, "solve: null vector");
   HIPC(c, hipSetDevice(c->device));
+  RunGuard run(c, 1);
   return solve_device(c, d_b, d_x, tol, maxits, maxtime, stag_test, flags, its_out,
                       relres_out, stop_out);
+}
+
+int lapgpu_cancel(lapgpu_ctx* c, int64_t column) {
+  if (!c || column < -1) return -1;
+  // c->err belongs to the solving thread: errors are return codes only
+  std::lock_guard<std::mutex> g(c->cancel_mu);
+  if (!c->running) return LAPGPU_IDLE;
+  const int64_t k = (int64_t)c->cancel_cols.size();
+  if (column < -1 || column >= k) return -1;
+  if (column == -1)
+    std::fill(c->cancel_cols.begin(), c->cancel_cols.end(), (char)1);
+  else
+    c->cancel_cols[(size_t)column] = 1;
+  c->cancel_any.store(true, std::memory_order_release);
+  return 0;
 }
 
 int lapgpu_solve(lapgpu_ctx* c, const double* b, double* x_out, int64_t n,
@@ -1795,6 +1873,7 @@ int lapgpu_solve(lapgpu_ctx* c, const double* b, double* x_out, int64_t n,
   if (!b || !x_out) return set_err(c, -1, "solve: null vector");
   if (!c->has_F) return set_err(c, -2, "solve: upload the Laplacian and LDLinv first");
   HIPC(c, hipSetDevice(c->device));
+  RunGuard run(c, 1);
   HIPC(c, hipMemcpyAsync(c->io, b, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
   int rc = solve_device(c, c->io, c->io, tol, maxits, maxtime, stag_test, flags, its_out,
                         relres_out, stop_out);
@@ -1813,6 +1892,8 @@ int lapgpu_solve_multi_device(lapgpu_ctx* c, const double* d_B, double* d_X, int
   if (k < 0 || (k > 0 && (!d_B || !d_X))) return set_err(c, -1, "solve_multi: bad arguments");
   if (k == 0) return 0;
   HIPC(c, hipSetDevice(c->device));
+  RunGuard run(c, k);
+  c->col_base = 0;
   return solve_multi_device(c, d_B, d_X, k, tol, maxits, maxtime, stag_test, flags, its_out,
                             relres_out, stop_out);
 }
@@ -1826,9 +1907,11 @@ int lapgpu_solve_multi(lapgpu_ctx* c, const double* B, double* X_out, int64_t n,
   if (!c->has_F) return set_err(c, -2, "solve_multi: upload the Laplacian and LDLinv first");
   if (k == 0) return 0;
   HIPC(c, hipSetDevice(c->device));
+  RunGuard run(c, k);
   // staging: one chunk of up to 64 columns at a time through m.io
   for (int64_t c0 = 0; c0 < k; c0 += kMultiMax) {
     const int64_t kc = std::min<int64_t>(kMultiMax, k - c0);
+    c->col_base = c0;
     int K = 8;
     while (K < kc) K *= 2;
     int rc = multi_prepare(c, K);

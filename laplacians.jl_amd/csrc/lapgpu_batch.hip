@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -27,7 +28,8 @@ using namespace lapgpu;
 
 // ---- scratch of lapgpu_batch_solve_device_noalloc -------------------------
 // One array per device, holding the requests' graph descriptors, the request
-// table and the per-request work vectors (8 n doubles each) of one call.  In
+// table and the per-request work vectors (8 n doubles each, plus up to 1024
+// partial slots) of one call.  In
 // host code `scratch` names a host-side stand-in: kernels and copies use the
 // device address that hipGetSymbolAddress returns for the current device,
 // looked up once per device when a batch is created there (scratch_setup).
@@ -38,6 +40,7 @@ namespace {
 
 struct GraphStore {
   int n = 0, nf = 0, nb = 0;
+  int npart = 0;  // partial slots of the programs (work per request: 8 n + npart)
   int64_t bytes = 0;
   int *perm = nullptr, *Lptr = nullptr, *Lidx = nullptr;
   double *Lval = nullptr, *s = nullptr;
@@ -45,6 +48,10 @@ struct GraphStore {
   TailSlotD *fslots = nullptr, *bslots = nullptr;
   bool live = false;
 };
+
+// work vectors of one request: b, r, x, bestx, p, q, z and y (with the
+// programs' partial slots past y[n))
+size_t work_doubles(const GraphStore& G) { return 8 * (size_t)G.n + (size_t)G.npart; }
 
 std::string g_batch_err;
 constexpr int kBatchSlots = 4;
@@ -97,7 +104,15 @@ struct BatchSlot {
   size_t h_cap = 0;
   double* d_work = nullptr;
   size_t work_cap = 0;
-  std::vector<int64_t> order;
+  std::vector<int64_t> order;  // request table position -> request index
+  std::vector<int64_t> pos;    // request index -> request table position
+  // cancel flags, one per request table position, in coherent (uncached)
+  // pinned host memory that the running kernel reads; earlier, smaller
+  // arrays are kept until the batch is destroyed
+  int* h_cancel = nullptr;
+  const int* d_cancel = nullptr;
+  size_t cancel_cap = 0;
+  std::vector<int*> old_cancel;
   int64_t nreq = 0;
   bool busy = false;
   // host-buffer launches: device io and pinned staging
@@ -184,6 +199,28 @@ int grow_pinned(lapgpu_batch* B, T** p, size_t* cap, size_t count) {
   return 0;
 }
 
+// A slot's cancel flags for at least `count` requests (the slot is idle).
+int grow_cancel(lapgpu_batch* B, BatchSlot& S, size_t count) {
+  if (count <= S.cancel_cap) return 0;
+  const size_t c = std::max<size_t>(count, std::max<size_t>(1024, 2 * S.cancel_cap));
+  int* p = nullptr;
+  {
+    LAPGPU_LOCK();
+    BHIP(B, hipHostMalloc((void**)&p, sizeof(int) * c, hipHostMallocMapped | hipHostMallocCoherent));
+  }
+  void* d = nullptr;
+  hipError_t e = hipHostGetDevicePointer(&d, p, 0);
+  if (e != hipSuccess) {
+    S.old_cancel.push_back(p);
+    return berr(B, -10, std::string("hipHostGetDevicePointer: ") + hipGetErrorString(e));
+  }
+  if (S.h_cancel) S.old_cancel.push_back(S.h_cancel);
+  S.h_cancel = p;
+  S.d_cancel = (const int*)d;
+  S.cancel_cap = c;
+  return 0;
+}
+
 // Frees are ordered on `stream` (which must already wait for every launch
 // that reads the graph).
 void free_store(GraphStore& G, hipStream_t stream) {
@@ -249,6 +286,10 @@ void lapgpu_batch_destroy(lapgpu_batch* B) {
     }
     lapgpu::pinned_put(S.h_reqs, sizeof(WgReqD) * S.h_cap);
     lapgpu::pinned_put(S.h_io, sizeof(double) * S.hio_cap);
+    if (S.h_cancel) S.old_cancel.push_back(S.h_cancel);
+    for (int* p : S.old_cancel) (void)hipHostFree(p);
+    S.old_cancel.clear();
+    S.h_cancel = nullptr;
     if (S.done) (void)hipEventDestroy(S.done);
     if (S.stream) (void)hipStreamDestroy(S.stream);
   }
@@ -287,6 +328,7 @@ int lapgpu_batch_add_graph(lapgpu_batch* B, int64_t n, const int64_t* colptr,
   G.n = (int)n;
   G.nf = (int)P.tail.fsteps.size();
   G.nb = (int)P.tail.bsteps.size();
+  G.npart = (int)P.tail.npart;
   static_assert(sizeof(TailStepD) == sizeof(TailStep), "step layout");
   static_assert(sizeof(TailSlotD) == sizeof(TailSlot), "slot layout");
   BHIP(B, hipSetDevice(B->device));
@@ -369,7 +411,7 @@ int launch_on(lapgpu_batch* B, int slot, int64_t nreq, const int32_t* graph_ids,
     S.h_graphs.resize(ng);
     for (int i = 0; i < ng; ++i) {
       const GraphStore& G = B->graphs[i];
-      S.h_graphs[i] = WgGraphD{G.n, G.nf, G.nb, 0, G.perm, G.Lptr, G.Lidx, G.Lval, G.s,
+      S.h_graphs[i] = WgGraphD{G.n, G.nf, G.nb, G.npart, G.perm, G.Lptr, G.Lidx, G.Lval, G.s,
                                G.fsteps, G.fslots, G.bsteps, G.bslots};
     }
     if ((rc = grow(B, S.stream, &S.d_graphs, &S.d_graphs_cap, (size_t)ng))) return rc;
@@ -385,10 +427,18 @@ int launch_on(lapgpu_batch* B, int slot, int64_t nreq, const int32_t* graph_ids,
     return B->graphs[graph_ids[a]].n > B->graphs[graph_ids[b]].n;
   });
   size_t work = 0;
-  for (int64_t i = 0; i < nreq; ++i) work += 8 * (size_t)B->graphs[graph_ids[i]].n;
+  for (int64_t i = 0; i < nreq; ++i) work += work_doubles(B->graphs[graph_ids[i]]);
   if ((rc = grow(B, S.stream, &S.d_work, &S.work_cap, work))) return rc;
   if ((rc = grow(B, S.stream, &S.d_reqs, &S.reqs_cap, (size_t)nreq))) return rc;
   if ((rc = grow_pinned(B, &S.h_reqs, &S.h_cap, (size_t)nreq))) return rc;
+  if ((rc = grow_cancel(B, S, (size_t)nreq))) return rc;
+  S.pos.resize(nreq);
+  for (int64_t k = 0; k < nreq; ++k) {
+    S.pos[S.order[k]] = k;
+    __atomic_store_n(&S.h_cancel[k], 0, __ATOMIC_RELAXED);
+  }
+  // the cleared flags are visible before the launch that reads them
+  std::atomic_thread_fence(std::memory_order_seq_cst);
   const bool has_mt = maxtime > 0 && std::isfinite(maxtime);
   size_t off = 0;
   for (int64_t k = 0; k < nreq; ++k) {
@@ -397,7 +447,7 @@ int launch_on(lapgpu_batch* B, int slot, int64_t nreq, const int32_t* graph_ids,
     R.b = d_b[i];
     R.x = d_x[i];
     R.work = S.d_work + off;
-    off += 8 * (size_t)B->graphs[graph_ids[i]].n;
+    off += work_doubles(B->graphs[graph_ids[i]]);
     R.tol = tol;
     R.maxits = maxits;
     R.maxticks = has_mt ? (unsigned long long)(maxtime * B->wallclock_khz * 1e3) : 0ull;
@@ -405,6 +455,7 @@ int launch_on(lapgpu_batch* B, int slot, int64_t nreq, const int32_t* graph_ids,
     R.stag_test = stag_test;
     R.center = (flags & LAPGPU_NO_CENTER) ? 0 : 1;
     R.has_maxtime = has_mt ? 1 : 0;
+    R.cancel = S.d_cancel + k;
     S.h_reqs[k] = R;
   }
   BHIP(B, hipMemcpyAsync(S.d_reqs, S.h_reqs, sizeof(WgReqD) * nreq, hipMemcpyHostToDevice,
@@ -488,6 +539,18 @@ int lapgpu_batch_submit_host(lapgpu_batch* B, int64_t nreq, const int32_t* graph
   BHIP(B, hipEventRecord(S.done, S.stream));
   S.io_len = total;
   *slot_out = slot;
+  B->err.clear();
+  return 0;
+}
+
+int lapgpu_batch_cancel(lapgpu_batch* B, int32_t slot, int64_t index) {
+  BATCH_LOCK(B);
+  if (!B || slot < 0 || slot >= (int32_t)B->slots.size())
+    return berr(B, -1, "lapgpu_batch_cancel: bad slot");
+  BatchSlot& S = B->slots[slot];
+  if (!S.busy) return berr(B, -1, "lapgpu_batch_cancel: slot has no launch");
+  if (index < 0 || index >= S.nreq) return berr(B, -1, "lapgpu_batch_cancel: request index out of range");
+  __atomic_store_n(&S.h_cancel[S.pos[index]], 1, __ATOMIC_RELEASE);
   B->err.clear();
   return 0;
 }
@@ -603,11 +666,12 @@ int lapgpu_batch_solve_device_noalloc(lapgpu_batch* B, int64_t nreq, const int32
     return berr(B, -1, "lapgpu_batch_solve_device_noalloc: more than 1024 requests in one call");
   int rc;
   if ((rc = check_ids(B, nreq, graph_ids))) return rc;
-  // scratch layout: descriptors | requests | work vectors (8 n doubles each)
+  // scratch layout: descriptors | requests | work vectors (8 n + npart
+  // doubles each)
   const size_t off_reqs = align256(sizeof(WgGraphD) * (size_t)nreq);
   const size_t off_work = off_reqs + align256(sizeof(WgReqD) * (size_t)nreq);
   size_t need = off_work;
-  for (int64_t i = 0; i < nreq; ++i) need += sizeof(double) * 8 * (size_t)B->graphs[graph_ids[i]].n;
+  for (int64_t i = 0; i < nreq; ++i) need += sizeof(double) * work_doubles(B->graphs[graph_ids[i]]);
   if (need > kScratchBytes)
     return berr(B, -1, "lapgpu_batch_solve_device_noalloc: the requests need " + std::to_string(need) +
                            " bytes of scratch, more than its " + std::to_string(kScratchBytes) +
@@ -632,13 +696,13 @@ int lapgpu_batch_solve_device_noalloc(lapgpu_batch* B, int64_t nreq, const int32
     const int64_t i = g_na_order[k];
     const GraphStore& G = B->graphs[graph_ids[i]];
     // one descriptor per request (no table of every graph to upload)
-    g_na_graphs[k] = WgGraphD{G.n, G.nf, G.nb, 0, G.perm, G.Lptr, G.Lidx, G.Lval, G.s,
+    g_na_graphs[k] = WgGraphD{G.n, G.nf, G.nb, G.npart, G.perm, G.Lptr, G.Lidx, G.Lval, G.s,
                               G.fsteps, G.fslots, G.bsteps, G.bslots};
     WgReqD R{};
     R.b = d_b[i];
     R.x = d_x[i];
     R.work = work + off;
-    off += 8 * (size_t)G.n;
+    off += work_doubles(G);
     R.tol = tol;
     R.maxits = maxits;
     R.maxticks = has_mt ? (unsigned long long)(maxtime * B->wallclock_khz * 1e3) : 0ull;

@@ -198,6 +198,9 @@ constexpr int kTailCapV = 18432;
 #define LAPGPU_TAIL_CAP 18432
 #endif
 constexpr int kTailCapLds = LAPGPU_TAIL_CAP;
+// partial slots of a slot program, past its own indices (= the planner's
+// kPartCap): the y arrays of the programs hold kTailCapLds + kPartSlots
+constexpr int kPartSlots = 1024;
 
 struct alignas(16) TailSlotD {
   double w;
@@ -292,12 +295,15 @@ __device__ __forceinline__ void run_step(const S (&r)[EM], int meta, Y* ly) {
     v = seg_sum_last(v, (int)((r[0].meta >> 26) & 7u), (meta >> 19) & 7);
     // in LDS the lead lanes add without waiting for a reply: a single-
     // segment row has exactly one add per level (the same result as a
-    // read-modify-write), the chunks of a split row add in any order.  In
-    // global memory a single-segment row is read-modified-written, so the
-    // workgroup's later loads see it through its L1.
+    // read-modify-write), the chunks of a split row either store partial
+    // sums (lead 3, summed in order by a later fix-up step) or add in any
+    // order (lead 2).  In global memory a single-segment row is read-
+    // modified-written, so the workgroup's later loads see it through its L1.
     const unsigned lead = r[0].meta >> 29;
     const unsigned row = r[0].meta & ((1u << 26) - 1);
-    if (LDSY) {
+    if (lead == 3) {
+      ly[row] = v;
+    } else if (LDSY) {
       if (lead != 0) atomicAdd(&ly[row], v);
     } else if (lead == 1) {
       ly[row] += v;

@@ -273,7 +273,7 @@ k_tail(int T0, int NT, const T* __restrict__ ext, const T* __restrict__ s,
        const S* __restrict__ fslots, const TailStepD* __restrict__ bsteps,
        int nbsteps, const S* __restrict__ bslots,
        const DevState* __restrict__ st) {
-  __shared__ T ly[kTailCapLds];
+  __shared__ T ly[kTailCapLds + kPartSlots];
   if (st->stopped) return;
   for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] = ext[i];
   __syncthreads();
@@ -309,7 +309,7 @@ k_dense_basis(int K, int T1, int j0, const double* __restrict__ s,
               const TailSlotD* __restrict__ fslots, const TailStepD* __restrict__ bsteps,
               int nbsteps, const TailSlotD* __restrict__ bslots, double* __restrict__ D,
               long ldd) {
-  __shared__ double ly[kTailCapLds];
+  __shared__ double ly[kTailCapLds + kPartSlots];
   const int j = j0 + blockIdx.x;
   if (j >= K) return;  // the whole workgroup
   for (int i = threadIdx.x; i < K; i += kTailThreads) ly[i] = i == j ? 1.0 : 0.0;
@@ -338,7 +338,7 @@ __global__ void __launch_bounds__(kTailThreads)
 k_tail_fwd(int T0, int NT, int NT1, const T* __restrict__ ext, const T* __restrict__ s,
            T* __restrict__ yF, T* __restrict__ u, const TailStepD* __restrict__ fsteps,
            int nfsteps, const S* __restrict__ fslots, const DevState* __restrict__ st) {
-  __shared__ T ly[kTailCapLds];
+  __shared__ T ly[kTailCapLds + kPartSlots];
   if (st->stopped) return;
   for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] = ext[i];
   __syncthreads();
@@ -354,7 +354,7 @@ template <typename T, typename S, int E>
 __global__ void __launch_bounds__(kTailThreads)
 k_tail_bwd(int T0, int NT, int NT1, T* __restrict__ yF, const TailStepD* __restrict__ bsteps,
            int nbsteps, const S* __restrict__ bslots, const DevState* __restrict__ st) {
-  __shared__ T ly[kTailCapLds];
+  __shared__ T ly[kTailCapLds + kPartSlots];
   if (st->stopped) return;
   for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] = yF[T0 + i];
   __syncthreads();
@@ -692,6 +692,17 @@ __global__ void k_pupdate(int n, const double* __restrict__ z,
 }
 
 // bestx <- x when the last evaluated iteration improved (runs after stop)
+// Stop (reason 10, cancelled) the states j < k whose bit is set in mask
+// and that have not stopped: every later kernel of the solve skips them,
+// and the final copy returns their best iterate.
+__global__ void k_cancel(DevState* st, int k, unsigned long long mask) {
+  const int j = threadIdx.x;
+  if (j < k && ((mask >> j) & 1ull) && !st[j].stopped) {
+    st[j].stopped = 1;
+    st[j].stop_reason = 10;
+  }
+}
+
 __global__ void k_final_copy(int n, const double* __restrict__ x,
                              double* __restrict__ bestx,
                              const DevState* __restrict__ st) {

@@ -408,7 +408,7 @@ k_mtail(int T0, int NT, const T* __restrict__ ext, const T* __restrict__ s, T* _
         const TailStepD* __restrict__ fsteps, int nfsteps, const S* __restrict__ fslots,
         const TailStepD* __restrict__ bsteps, int nbsteps, const S* __restrict__ bslots,
         const DevState* __restrict__ mst, const int* __restrict__ flags) {
-  __shared__ T ly[kTailCapLds];
+  __shared__ T ly[kTailCapLds + kPartSlots];
   const int j = blockIdx.x;
   if (flags[0] || mst[j].stopped) return;
   for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] = ext[(int64_t)i * K + j];
@@ -431,7 +431,7 @@ k_mtail_fwd(int T0, int NT, int NT1, const T* __restrict__ ext, const T* __restr
             T* __restrict__ z, T* __restrict__ u, const TailStepD* __restrict__ fsteps,
             int nfsteps, const S* __restrict__ fslots, const DevState* __restrict__ mst,
             const int* __restrict__ flags) {
-  __shared__ T ly[kTailCapLds];
+  __shared__ T ly[kTailCapLds + kPartSlots];
   const int j = blockIdx.x;
   if (flags[0] || mst[j].stopped) return;
   for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] = ext[(int64_t)i * K + j];
@@ -449,7 +449,7 @@ __global__ void __launch_bounds__(kTailThreads)
 k_mtail_bwd(int T0, int NT, int NT1, T* __restrict__ z, const TailStepD* __restrict__ bsteps,
             int nbsteps, const S* __restrict__ bslots, const DevState* __restrict__ mst,
             const int* __restrict__ flags) {
-  __shared__ T ly[kTailCapLds];
+  __shared__ T ly[kTailCapLds + kPartSlots];
   const int j = blockIdx.x;
   if (flags[0] || mst[j].stopped) return;
   for (int i = threadIdx.x; i < NT; i += kTailThreads) ly[i] = z[(int64_t)(T0 + i) * K + j];

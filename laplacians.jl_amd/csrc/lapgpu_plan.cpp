@@ -227,31 +227,54 @@ struct RowEntries {
   std::vector<std::pair<int32_t, double>> e;  // (local src, weight)
 };
 
-// Pack one level's rows into steps of `lanes` lanes (see TailSlot), E =
-// 2^elog entries per lane.  A row of at most 64 E entries is one segment
-// whose lead lane adds its sum to y[row] (lead = 1); a longer row is split
-// into chunks whose leads add with an LDS atomic (lead = 2).  Lane j of a
-// segment takes the chunk's entries j E .. j E + E - 1.  Segments are
-// placed largest first with a bump pointer per step, which keeps every
-// segment aligned to its size.
-void pack_level(std::vector<RowEntries>& rows, int elog, int64_t lanes,
-                std::vector<TailStep>& steps, std::vector<TailSlot>& slots) {
+// Entries [k0, k1) of one row, packed as chunks of at most 64 E entries.
+// A piece of several chunks whose leads store partial sums (split) gets a
+// fix-up row that adds them in chunk order.
+struct Piece {
+  const RowEntries* R;
+  int64_t k0, k1;
+  bool split;
+};
+
+// One pass: the pieces' chunks in steps of `lanes` lanes, the last step
+// ending with a barrier.  Chunk leads add to y[row] (lead = 1) when the
+// piece is one chunk, store to the partial slot part_base + i (lead = 3,
+// i counting the split pieces' chunks in piece order) when it is split,
+// and add atomically (lead = 2) for the chunks of an unsplit multi-chunk
+// piece (the order-free form).  Lane j of a chunk takes its entries
+// j E .. j E + E - 1.  Chunks are placed largest first with a bump
+// pointer per step, which keeps every chunk aligned to its size.  The
+// split pieces' fix-up rows are appended to *fix.
+void pack_pass(const std::vector<Piece>& pieces, int elog, int64_t lanes,
+               std::vector<TailStep>& steps, std::vector<TailSlot>& slots, int64_t part_base,
+               std::vector<RowEntries>* fix) {
   const int64_t E = int64_t(1) << elog;
   struct Chunk {
-    int32_t row;
+    uint32_t dst;  // row, or partial slot
+    uint32_t lead;
     int64_t size;  // segment lanes
     int64_t k0, k1;
     const RowEntries* R;
-    bool atomic;
   };
   std::vector<Chunk> chunks;
-  for (const RowEntries& R : rows) {
-    const int64_t dg = (int64_t)R.e.size();
-    if (dg == 0) continue;
-    const bool multi = dg > 64 * E;
-    for (int64_t k0 = 0; k0 < dg; k0 += 64 * E) {
-      const int64_t k1 = std::min(dg, k0 + 64 * E);
-      chunks.push_back({R.row, pow2_at_least((k1 - k0 + E - 1) / E), k0, k1, &R, multi});
+  int64_t npart = 0;
+  for (const Piece& Q : pieces) {
+    const bool multi = Q.k1 - Q.k0 > 64 * E;
+    RowEntries* F = nullptr;
+    if (Q.split) {
+      fix->push_back(RowEntries{Q.R->row, {}});
+      F = &fix->back();
+    }
+    for (int64_t k0 = Q.k0; k0 < Q.k1; k0 += 64 * E) {
+      const int64_t k1 = std::min(Q.k1, k0 + 64 * E);
+      uint32_t dst = (uint32_t)Q.R->row, lead = multi ? 2u : 1u;
+      if (Q.split) {
+        dst = (uint32_t)(part_base + npart);
+        lead = 3u;
+        F->e.emplace_back((int32_t)(part_base + npart), 1.0);
+        ++npart;
+      }
+      chunks.push_back({dst, lead, pow2_at_least((k1 - k0 + E - 1) / E), k0, k1, Q.R});
     }
   }
   std::stable_sort(chunks.begin(), chunks.end(),
@@ -289,8 +312,7 @@ void pack_level(std::vector<RowEntries>& rows, int elog, int64_t lanes,
     const int lg = log2_exact(C.size);
     TailSlot* S0 = slots.data() + base[p];
     for (int64_t t = 0; t < C.size; ++t) {
-      S0[off + t].meta = slot_meta((uint32_t)C.row, (uint32_t)lg,
-                                   t == C.size - 1 ? (C.atomic ? 2u : 1u) : 0u);
+      S0[off + t].meta = slot_meta(C.dst, (uint32_t)lg, t == C.size - 1 ? C.lead : 0u);
       for (int64_t e = 0; e < E; ++e) {
         const int64_t k = C.k0 + t * E + e;
         if (k < C.k1) {
@@ -303,10 +325,69 @@ void pack_level(std::vector<RowEntries>& rows, int elog, int64_t lanes,
   }
 }
 
+// Pack one level's rows (see TailSlot), E = 2^elog entries per lane; every
+// pass ends with a barrier.  Without det, a row of at most 64 E entries is
+// one chunk whose lead adds its sum to y[row], and a longer row's chunks
+// add with atomics, in whatever order they land.  With det every sum is
+// formed in a fixed order: a longer row's chunks store partial sums in
+// slots part_base + i (i < kPartCap, past the program's own indices), and
+// a fix-up pass (packed the same way, recursively) adds each row's
+// partials in chunk order; rows whose chunks do not fit the slots of one
+// pass go on in later passes (each pass adds to y[row] once).  Returns the
+// partial slots used.
+int64_t pack_level(std::vector<RowEntries>& rows, int elog, int64_t lanes,
+                   std::vector<TailStep>& steps, std::vector<TailSlot>& slots, bool det,
+                   int64_t part_base) {
+  const int64_t E = int64_t(1) << elog, CH = 64 * E;
+  if (!det) {
+    std::vector<Piece> pieces;
+    for (const RowEntries& R : rows)
+      if (!R.e.empty()) pieces.push_back({&R, 0, (int64_t)R.e.size(), false});
+    pack_pass(pieces, elog, lanes, steps, slots, 0, nullptr);
+    return 0;
+  }
+  // chunks with partial slots per pass, leaving room for the fix-up
+  // passes' own partials (a fix-up row longer than 64 E entries)
+  const int64_t cap = kPartCap - kPartCap / 4;
+  std::vector<std::vector<Piece>> passes(1);
+  std::vector<int64_t> used(1, 0);
+  for (const RowEntries& R : rows) {
+    const int64_t dg = (int64_t)R.e.size();
+    if (dg == 0) continue;
+    if (dg <= CH) {
+      passes[0].push_back({&R, 0, dg, false});
+      continue;
+    }
+    size_t p = 0;  // first pass this row's next piece may use
+    for (int64_t k0 = 0; k0 < dg;) {
+      while (p < passes.size() && used[p] >= cap) ++p;
+      if (p == passes.size()) {
+        passes.emplace_back();
+        used.push_back(0);
+      }
+      const int64_t k1 = std::min(dg, k0 + (cap - used[p]) * CH);
+      const int64_t nch = (k1 - k0 + CH - 1) / CH;
+      passes[p].push_back({&R, k0, k1, nch > 1});
+      if (nch > 1) used[p] += nch;
+      k0 = k1;
+      ++p;
+    }
+  }
+  int64_t npart = 0;
+  for (size_t p = 0; p < passes.size(); ++p) {
+    std::vector<RowEntries> fix;
+    pack_pass(passes[p], elog, lanes, steps, slots, part_base, &fix);
+    int64_t nested = 0;
+    if (!fix.empty()) nested = pack_level(fix, elog, lanes, steps, slots, true, part_base + used[p]);
+    npart = std::max(npart, used[p] + nested);
+  }
+  return npart;
+}
+
 }  // namespace
 
 bool build_tail(Plan& P, int64_t max_steps, int64_t cap, int max_elog, int lanes,
-                const DenseModel& dm) {
+                const DenseModel& dm, bool det) {
   const int64_t n = P.n;
   // extent: the longest suffix of levels within cap whose levels need at
   // most max_steps steps of kExtentLanes lanes at one entry per lane
@@ -356,6 +437,7 @@ bool build_tail(Plan& P, int64_t max_steps, int64_t cap, int max_elog, int lanes
   struct LevelProg {
     std::vector<TailStep> steps;
     std::vector<TailSlot> slots;
+    int64_t npart = 0;  // partial slots (pack_level)
   };
   PlanThreads threads;
   Team team(threads.n);
@@ -363,6 +445,7 @@ bool build_tail(Plan& P, int64_t max_steps, int64_t cap, int max_elog, int lanes
   // gather their in-edges from [base, n), backward rows all their entries
   // (every level packs on its own, offsets local)
   std::atomic<bool> ok{true};
+  // partial slots follow the program's own indices [0, n - base)
   auto pack_levels = [&](int64_t l0, int64_t base, int elog, std::vector<LevelProg>& fp,
                          std::vector<LevelProg>& bp) {
     const int64_t nl = P.nlev - l0;
@@ -389,7 +472,7 @@ bool build_tail(Plan& P, int64_t max_steps, int64_t cap, int max_elog, int lanes
           rws.push_back(std::move(R));
         }
         LevelProg& G = fwd ? fp[l - l0] : bp[l - l0];
-        pack_level(rws, elog, lanes, G.steps, G.slots);
+        G.npart = pack_level(rws, elog, lanes, G.steps, G.slots, det, n - base);
       }
     });
   };
@@ -451,7 +534,7 @@ bool build_tail(Plan& P, int64_t max_steps, int64_t cap, int max_elog, int lanes
         if (P.Fidx[k] >= T0 && P.Fidx[k] < T1) R.e.emplace_back((int32_t)(P.Fidx[k] - T0), P.Fw[k]);
       if (!R.e.empty()) rws.push_back(std::move(R));
     }
-    if (!rws.empty()) pack_level(rws, T.elog, lanes, bnd.steps, bnd.slots);
+    if (!rws.empty()) bnd.npart = pack_level(rws, T.elog, lanes, bnd.steps, bnd.slots, det, T.NT);
     // the block's own programs, local to T1
     DenseBlock& D = P.dense;
     D.T1 = T1;
@@ -468,6 +551,7 @@ bool build_tail(Plan& P, int64_t max_steps, int64_t cap, int max_elog, int lanes
     D.elog = choose_elog(drows, dent, max_elog);
     std::vector<LevelProg> dfp, dbp;
     pack_levels(L1, T1, D.elog, dfp, dbp);
+    for (size_t q = 0; q < dfp.size(); ++q) D.npart = std::max({D.npart, dfp[q].npart, dbp[q].npart});
     for (size_t q = 0; q < dfp.size(); ++q) {
       const int32_t b0 = (int32_t)D.fslots.size();
       for (TailStep st : dfp[q].steps) {
@@ -504,6 +588,9 @@ bool build_tail(Plan& P, int64_t max_steps, int64_t cap, int max_elog, int lanes
     bs += bp[q].steps.size();
     bl += bp[q].slots.size();
   }
+  T.npart = bnd.npart;
+  for (int64_t q = 0; q < ns; ++q) T.npart = std::max({T.npart, fp[q].npart, bp[q].npart});
+  if (T.npart > kPartCap || P.dense.npart > kPartCap) return false;
   T.fsteps.reserve(fs);
   T.fslots.reserve(fl);
   T.bsteps.reserve(bs);

@@ -20,9 +20,11 @@ struct HostCsr {
 // E = 2^elog entries (the step's "planes"), then the lanes of an aligned
 // segment of 2^seglog lanes are summed and the segment's last lane adds the
 // sum to y[row] (lead = 1: plain add; lead = 2: atomic add, for rows split
-// in chunks).  meta packs row (bits 0-25), seglog (bits 26-28) and lead
-// (bits 29-30); it is read from plane 0 only.  Indices are local to the
-// program (position - T0).
+// in chunks and summed in any order) or stores it to the partial slot
+// y[row] (lead = 3: a chunk of a row that a later fix-up step sums in chunk
+// order).  meta packs row (bits 0-25), seglog (bits 26-28) and lead (bits
+// 29-31); it is read from plane 0 only.  Indices are local to the program
+// (position - T0); partial slots follow them, [NT, NT + npart).
 struct TailSlot {
   double w;
   uint32_t src;
@@ -45,6 +47,9 @@ struct TailStep {
   int32_t meta;
 };
 constexpr int kTailMaxElog = 2;
+// Partial slots a program may use (past its own indices; the kernels' y
+// arrays hold them).
+constexpr int64_t kPartCap = 1024;
 inline int32_t step_meta(int64_t count, bool end, int elog, int maxlog) {
   return (int32_t)(count | ((end ? 1 : 0) << 16) | (elog << 17) | (maxlog << 19));
 }
@@ -72,6 +77,7 @@ struct TailProgram {
   int64_t T0 = 0, NT = 0;
   int64_t NT1 = 0;  // vertices before the dense block (NT without one)
   int elog = 0;     // every step of both sweeps has 2^elog entries per lane
+  int64_t npart = 0;  // partial slots past [0, NT) (at most kPartCap)
   std::vector<TailStep> fsteps, bsteps;
   std::vector<TailSlot> fslots, bslots;
   // forward in-edges of tail rows from outside the tail (CSR over tail rows)
@@ -89,6 +95,7 @@ struct DenseBlock {
   int64_t T1 = 0, K = 0;  // K = 0: no dense block
   int64_t level = 0;      // first level of the block
   int elog = 0;
+  int64_t npart = 0;  // partial slots past [0, K)
   std::vector<TailStep> fsteps, bsteps;
   std::vector<TailSlot> fslots, bslots;
 };
@@ -168,8 +175,12 @@ bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
 // `dense`.  With both limits at INT64_MAX the program covers the whole sweep
 // (tail_level 0).  Entries per lane are chosen for the whole program, at
 // most 2^max_elog.
+// With det (the default) every row sum of the programs is formed in a
+// fixed order (partial slots and fix-up steps for rows split in chunks), so
+// that the sweeps give the same bits on every run; without, split rows add
+// their chunks atomically.
 bool build_tail(Plan& P, int64_t max_steps, int64_t cap, int max_elog = kTailMaxElog,
-                int lanes = kTailLanes, const DenseModel& dense = DenseModel());
+                int lanes = kTailLanes, const DenseModel& dense = DenseModel(), bool det = true);
 
 // Group the levels before the tail into launches of up to `max_depth`
 // levels while a group's composite rows stay within `growth` times the

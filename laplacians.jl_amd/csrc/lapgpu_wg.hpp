@@ -22,7 +22,7 @@ constexpr int kWgElog = 0;
 constexpr int kWgPlanes = 1 << kWgElog;
 
 struct WgGraphD {
-  int n, nf, nb, pad;
+  int n, nf, nb, npart;  // npart: partial slots of the programs, past y[n)
   const int* perm;
   const int* Lptr;
   const int* Lidx;
@@ -37,11 +37,14 @@ struct WgGraphD {
 struct WgReqD {
   const double* b;  // rhs, caller's vertex order
   double* x;        // solution, caller's vertex order
-  double* work;     // 8 n doubles
+  double* work;     // 8 n + npart doubles
   double tol;
   long long maxits;
   unsigned long long maxticks;
   int graph, stag_test, center, has_maxtime;
+  // cancel flag in coherent host memory (nullptr: not cancellable); read
+  // once per iteration
+  const int* cancel;
   // results
   long long its;
   double nr;
@@ -144,9 +147,15 @@ __device__ __forceinline__ void wg_pcg(const WgGraphD& G, WgReqD& R, Y* y, doubl
     int stop = 1;
     bool pending = false;
     const unsigned long long t1 = wall_clock64();
+    const bool poll = R.has_maxtime || R.cancel != nullptr;
     while (R.maxits < 0 || itcnt < R.maxits) {
       itcnt += 1;
       const bool chk = (itcnt % 10) == 0;
+      // the cancel flag, read at the start of the iteration (the read's
+      // latency hides behind the SpMV) and acted on at its end
+      int cflag = 0;
+      if (R.cancel != nullptr && t == 0)
+        cflag = __hip_atomic_load(R.cancel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
       // q = L p, p.q
       double pq[1] = {0.0};
       {
@@ -243,14 +252,22 @@ __device__ __forceinline__ void wg_pcg(const WgGraphD& G, WgReqD& R, Y* y, doubl
         break;
       }
       for (int i = t; i < n; i += kWgThreads) p[i] = z[i] + beta * p[i];
-      if (R.has_maxtime) {
-        // one clock reading, broadcast, so that every thread agrees
-        if (t == 0) red[0] = (wall_clock64() - t1) > R.maxticks ? 1.0 : 0.0;
+      if (poll) {
+        // one clock reading and one flag, broadcast, so that every thread
+        // agrees
+        if (t == 0)
+          red[0] = (R.has_maxtime && (wall_clock64() - t1) > R.maxticks) ? 1.0
+                   : cflag != 0                                          ? 2.0
+                                                                         : 0.0;
         __syncthreads();
-        const bool late = red[0] != 0.0;
+        const double why = red[0];
         __syncthreads();
-        if (late) {
+        if (why == 1.0) {
           stop = 2;
+          break;
+        }
+        if (why == 2.0) {
+          stop = 10;
           break;
         }
       } else {
@@ -270,7 +287,7 @@ __device__ __forceinline__ void wg_pcg(const WgGraphD& G, WgReqD& R, Y* y, doubl
 
 __global__ void __launch_bounds__(kWgThreads)
 k_wg_pcg(const WgGraphD* __restrict__ graphs, WgReqD* __restrict__ reqs) {
-  __shared__ double ly[kTailCapV];
+  __shared__ double ly[kTailCapV + kPartSlots];
   __shared__ double red[4 * (kWgThreads / kWave)];
   WgReqD R = reqs[blockIdx.x];
   const WgGraphD G = graphs[R.graph];

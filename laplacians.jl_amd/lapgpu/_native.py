@@ -25,6 +25,7 @@ STOP_NAMES = {
     7: "inf_rho",
     8: "inf_beta",
     9: "zero_rhs",
+    10: "cancelled",
 }
 
 VERBOSE = 1
@@ -63,6 +64,7 @@ EXPORTS = (
     "lapgpu_sweep_entries",
     "lapgpu_last_dense_timings",
     "lapgpu_dense_info",
+    "lapgpu_cancel",
     "lapgpu_batch_create",
     "lapgpu_batch_add_graph",
     "lapgpu_batch_remove_graph",
@@ -73,6 +75,7 @@ EXPORTS = (
     "lapgpu_batch_submit",
     "lapgpu_batch_submit_host",
     "lapgpu_batch_query",
+    "lapgpu_batch_cancel",
     "lapgpu_batch_collect",
     "lapgpu_batch_last_error",
     "lapgpu_batch_destroy",
@@ -132,6 +135,8 @@ def load() -> ctypes.CDLL:
     L.lapgpu_batch_solve_device_noalloc.argtypes = [P, i64, P, P, P, dbl, i64, dbl, ci, ci, P, P, P]
     L.lapgpu_batch_submit.argtypes = [P, i64, P, P, P, dbl, i64, dbl, ci, ci, pi32]
     L.lapgpu_batch_query.argtypes = [P, ctypes.c_int32, ctypes.POINTER(ci)]
+    L.lapgpu_batch_cancel.argtypes = [P, ctypes.c_int32, i64]
+    L.lapgpu_cancel.argtypes = [P, i64]
     L.lapgpu_batch_submit_host.argtypes = [P, i64, P, P, dbl, i64, dbl, ci, ci, pi32]
     L.lapgpu_batch_collect.argtypes = [P, ctypes.c_int32, P, P, P, P]
     L.lapgpu_batch_last_error.argtypes = [P]
@@ -292,6 +297,16 @@ class Context:
             self._lib.lapgpu_upload_ldli(self._h, len(d), len(fv), _p(col), _p(cp), _p(rv), _p(fv), _p(d), 0),
             "lapgpu_upload_ldli",
         )
+
+    def cancel(self, column: int = -1) -> bool:
+        """From another thread: cancel column `column` (-1: all) of the
+        solve running on this context; those columns stop at the next
+        iteration boundary with stop "cancelled" and their best iterate.
+        Returns False when no solve was running."""
+        rc = self._lib.lapgpu_cancel(self._h, int(column))
+        if rc < 0:
+            raise ValueError(f"lapgpu_cancel: column {column} out of range")
+        return rc == 0
 
     def solve(self, b, tol=1e-6, maxits=1000, maxtime=float("inf"), stag_test=5, flags=0):
         b = np.ascontiguousarray(b, dtype=np.float64)
@@ -582,6 +597,12 @@ class Batch:
             return None
         self._check(rc, "lapgpu_batch_submit_host")
         return (slot.value, len(ids), [len(b) for b in bs])
+
+    def cancel(self, handle, index: int) -> None:
+        """Cancel request `index` (its position in the submit call) of the
+        launch `handle`: it stops at its next iteration boundary with stop
+        "cancelled" (unless it has finished); the others run on unchanged."""
+        self._check(self._lib.lapgpu_batch_cancel(self._h, handle[0], int(index)), "lapgpu_batch_cancel")
 
     def query(self, handle) -> bool:
         done = ctypes.c_int(0)

@@ -7,8 +7,8 @@ over time; each is answered by the same solve closure as approxchol_lap's
 * admits a request as soon as it arrives: graphs of at most `wg_max_n`
   vertices (`new_wg_max_n` for a graph that comes with its request) are
   queued for the next batched launch (one workgroup per request,
-  lapgpu_batch_submit_host, largest graphs first in the launch so that the
-  longest solves start first); larger graphs queue per graph, and a
+  lapgpu_batch_submit_host, whose workgroups start largest graph first);
+  larger graphs queue per graph, and a
   worker solves everything queued for that graph in one multi-RHS solve
   (lapgpu_solve_multi, up to the graph's `max_rhs` columns sharing every
   launch; lapgpu_solve when only one is queued) on the graph's own stream;
@@ -33,7 +33,10 @@ over time; each is answered by the same solve closure as approxchol_lap's
   one solve saves.
 
 Every result carries its latency (arrival to completion); `summary` gives
-solves/s and p50/p99 latency.
+solves/s and p50/p99 latency.  `cancel(rid)` withdraws a request from any
+thread: answered at once while it is queued, stopped at its next iteration
+boundary once it is on the device, without changing the requests that share
+its launch or multi-RHS solve.
 
 Every graph context owns a stream, and HIP maps streams round-robin onto its
 hardware queues (4 unless GPU_MAX_HW_QUEUES says otherwise): a small solve on
@@ -44,6 +47,7 @@ first HIP call (more than 16 queues slowed plain solves on MI355X).
 
 from __future__ import annotations
 
+import threading
 import time
 from collections import OrderedDict, deque
 from concurrent.futures import ThreadPoolExecutor
@@ -187,6 +191,36 @@ class Server:
         self._factor_pool = ThreadPoolExecutor(max_workers=factor_workers)
         self._close_pool = ThreadPoolExecutor(max_workers=1)
         self._anon = 0
+        # cancellation (cancel(), from any thread): every rid ever cancelled,
+        # and those whose cancel the dispatch loop has not carried out yet
+        self._cancel_lock = threading.Lock()
+        self._cancelled: set = set()
+        self._cancel_todo: set = set()
+        # rid -> (graph, column) of the requests in a running context solve,
+        # rid -> (launch handle, index) of those in a batched launch
+        self._running: dict = {}
+        self._in_launch: dict = {}
+
+    def cancel(self, rid) -> None:
+        """Withdraw request `rid` (thread-safe; for a running run()).  A
+        request still queued, factoring or waiting for admission is answered
+        at once with stop "cancelled" and no solution; one already on the
+        device stops at its next iteration boundary (lapgpu_batch_cancel /
+        lapgpu_cancel) with stop "cancelled" and its best iterate, while the
+        requests sharing its launch or multi-RHS solve run on unchanged.  A
+        request that has finished keeps its result."""
+        with self._cancel_lock:
+            self._cancelled.add(rid)
+            self._cancel_todo.add(rid)
+
+    def on_device(self, rid) -> bool:
+        """True while request `rid` is in a running batched launch or
+        context solve (thread-safe)."""
+        return rid in self._in_launch or rid in self._running
+
+    def _is_cancelled(self, rid) -> bool:
+        with self._cancel_lock:
+            return rid in self._cancelled
 
     # ------------------------------------------------------------ graphs ---
     def register(self, key, a: sp.spmatrix, ldli: LDLinv | None = None, seed: int = 1,
@@ -306,6 +340,9 @@ class Server:
         upload_futs = {}  # context uploads of admitted large graphs
         waiting_admission: deque = deque()
         waiting: set = set()  # idle graphs collecting a batch
+        in_launch = self._in_launch  # rid -> (launch handle, index in the launch)
+        in_launch.clear()
+        placed = {}  # rid -> graph it was routed to
         t0 = time.perf_counter()
         c0 = time.thread_time()
         nxt = 0
@@ -317,7 +354,14 @@ class Server:
             keep = self.keep_x or (self.keep_ids is not None and r.rid in self.keep_ids)
             results[r.rid] = Result(r.rid, lat, info["its"], info["nr"], info["stop"], path, x if keep else None)
 
+        def finish_cancelled(r: Request, path: str):
+            finish(r, {"its": 0, "nr": 1.0, "stop": "cancelled"}, None, path)
+
         def route(g: _Graph, r: Request):
+            if self._is_cancelled(r.rid):
+                finish_cancelled(r, "queued")
+                return
+            placed[r.rid] = g
             if g.kind == "wg":
                 g.busy += 1
                 small.append((g, r))
@@ -333,10 +377,17 @@ class Server:
             # (not when the task was submitted, so requests that arrive while
             # the task waits for a worker join it), with the same tol and
             # maxits as the oldest, up to max_rhs columns
-            take = [g.pending.popleft()]
+            take = [g.pending.popleft()] if g.pending else []
             while (g.pending and len(take) < g.max_rhs and g.pending[0].tol == take[0].tol
                    and g.pending[0].maxits == take[0].maxits):
                 take.append(g.pending.popleft())
+            # requests cancelled while queued are answered without a solve
+            gone = [r for r in take if self._is_cancelled(r.rid)]
+            take = [r for r in take if not self._is_cancelled(r.rid)]
+            if not take:
+                return take, gone, None, None
+            for j, r in enumerate(take):
+                self._running[r.rid] = (g, j)
             r0 = take[0]
             # a graph that came with its request is solved without capturing
             # its iteration as a HIP graph (the capture would be used once)
@@ -349,13 +400,17 @@ class Server:
                 B = np.stack([r.b for r in take], axis=1)
                 x, info = g.handle.solve_multi(B, r0.tol, r0.maxits, float("inf"), 5, flags)
             dt = time.perf_counter() - t_start
+            for r in take:
+                self._running.pop(r.rid, None)
             self.solve_log.append((g.n, len(take), dt, g.solves == 0, time.thread_time() - c_start))
             g.solves += 1
             g.last_solve_s = dt
-            return take, x, info
+            return take, gone, x, info
 
         def pump_plain(g: _Graph):
             # one solve at a time per context (a context owns one stream)
+            if not g.pending:
+                waiting.discard(g.key)
             if g.busy or not g.pending:
                 return
             if len(g.pending) < g.max_rhs and time.perf_counter() < g.ready_at:
@@ -375,7 +430,9 @@ class Server:
                 r = reqs[nxt]
                 nxt += 1
                 arrival_abs[r.rid] = t0 + (r.arrival if realtime else 0.0)
-                if r.new_graph is not None:
+                if self._is_cancelled(r.rid):
+                    finish_cancelled(r, "queued")
+                elif r.new_graph is not None:
                     fut = self._factor_pool.submit(self._factor, r.new_graph)
                     factor_futs[fut] = r
                 else:
@@ -388,6 +445,10 @@ class Server:
                 progressed = True
             for _ in range(len(waiting_admission)):
                 r, (la, ldli) = waiting_admission.popleft()
+                if self._is_cancelled(r.rid):
+                    finish_cancelled(r, "queued")
+                    progressed = True
+                    continue
                 ta = time.perf_counter()
                 wg = la.shape[0] <= self.new_wg_max_n
                 res = self._reserve(la, ldli, wg)
@@ -436,15 +497,20 @@ class Server:
                 xs, infos = self.batch.collect(item[0])
                 for (g, r), x, info in zip(item[1], xs, infos):
                     g.busy -= 1
+                    in_launch.pop(r.rid, None)
                     finish(r, info, x, "batched")
                 progressed = True
             # finished plain solves
             for fut in [f for f in plain_futs if f.done()]:
                 g = plain_futs.pop(fut)
-                take, x, info = fut.result()
+                take, gone, x, info = fut.result()
                 g.busy = 0
                 g.ready_at = 0.0  # requests queued meanwhile go at once
-                if len(take) == 1:
+                for r in gone:
+                    finish_cancelled(r, "queued")
+                if not take:
+                    pass
+                elif len(take) == 1:
                     finish(take[0], info, x, "plain")
                 else:
                     for j, r in enumerate(take):
@@ -452,6 +518,39 @@ class Server:
                                x[:, j].copy(), "multi")
                 pump_plain(g)
                 progressed = True
+            # cancellations asked for since the last pass
+            with self._cancel_lock:
+                todo = list(self._cancel_todo)
+            for rid in todo:
+                done_now = rid in results
+                if not done_now and rid in in_launch:
+                    h, i = in_launch[rid]
+                    self.batch.cancel(h, i)
+                    done_now = True
+                elif not done_now and rid in self._running:
+                    g, j = self._running.get(rid, (None, 0))
+                    # False: the solve has not started or has just ended
+                    done_now = g is not None and g.handle.cancel(j)
+                elif not done_now and rid in placed:
+                    g = placed[rid]
+                    if g.kind == "wg":
+                        for item in list(small):
+                            if item[1].rid == rid:
+                                small.remove(item)
+                                g.busy -= 1
+                                finish_cancelled(item[1], "queued")
+                                done_now = True
+                    elif not g.busy:
+                        # no worker is taking from an idle graph's queue
+                        for r in list(g.pending):
+                            if r.rid == rid:
+                                g.pending.remove(r)
+                                finish_cancelled(r, "queued")
+                                done_now = True
+                if done_now:
+                    with self._cancel_lock:
+                        self._cancel_todo.discard(rid)
+                    progressed = True
             # start a batched launch with everything queued, if a slot is free
             while small:
                 # one launch runs every request with the same tol and maxits:
@@ -465,8 +564,6 @@ class Server:
                 rest.extend(small)
                 small.clear()
                 small.extend(rest)
-                # workgroups start in request order: the longest solves first
-                take.sort(key=lambda it: -it[0].n)
                 h = self.batch.submit_host([g.handle for g, _ in take], [r.b for _, r in take],
                                            tol=key[0], maxits=key[1])
                 if h is None:
@@ -474,6 +571,8 @@ class Server:
                         small.appendleft(item)
                     break
                 launches.append((h, take))
+                for i, (_, r) in enumerate(take):
+                    in_launch[r.rid] = (h, i)
                 progressed = True
             # drop finished one-off graphs
             for key in [k for k, g in self.graphs.items() if not g.resident and not g.busy and not g.pending]:
@@ -512,13 +611,17 @@ class Server:
 
 
 def summary(results: list[Result], wall: float) -> dict:
+    """solves/s and latency percentiles over the results (cancelled
+    requests included); max_nr over the requests that were not cancelled."""
     lat = np.array([r.latency for r in results])
+    nrs = [r.nr for r in results if r.stop != "cancelled"]
     return {
         "solves": len(results),
         "wall_s": wall,
         "solves_per_s": len(results) / wall if wall > 0 else None,
         "p50_ms": float(np.percentile(lat, 50) * 1e3),
         "p99_ms": float(np.percentile(lat, 99) * 1e3),
-        "max_nr": float(max(r.nr for r in results)),
+        "max_nr": float(max(nrs)) if nrs else None,
         "stops": sorted({r.stop for r in results}),
+        "cancelled": sum(r.stop == "cancelled" for r in results),
     }

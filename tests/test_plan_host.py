@@ -96,3 +96,53 @@ def test_root_scale_follows_its_d(driver, tmp_path):
         words = r.stdout.split()
         assert int(words[1]) == root
         assert float(words[3]) == want, (droot, r.stdout)
+
+
+@pytest.fixture(scope="module")
+def checker(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("prog") / "program_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", f"-I{ROOT}/include", f"-I{CSRC}", "-o", exe,
+                    os.path.join(ROOT, "tests", "native", "program_check.cpp"),
+                    os.path.join(CSRC, "lapgpu_plan.cpp")], check=True)
+    return exe
+
+
+def _star(m):
+    import scipy.sparse as sp
+
+    a = sp.coo_matrix((np.ones(m), (np.zeros(m, dtype=int), np.arange(1, m + 1))), shape=(m + 1, m + 1))
+    return sp.csc_matrix(a + a.T)
+
+
+@pytest.mark.parametrize("name", ["grid3", "path_matchings", "dense", "star"])
+def test_slot_programs_equal_the_sweeps(checker, tmp_path, name):
+    """Every slot program (context tail with and without its dense block,
+    the block's own programs, the batched engine's whole-graph programs),
+    run lane by lane on the host as the kernels run them, equals the sweeps
+    written out over the level-order layout, to rounding.  With det, rows
+    split in chunks store partial sums that a fix-up step adds in chunk
+    order (no atomic adds left: lead 2 count 0); partial slots start as NaN,
+    so a fix-up that read an unwritten slot would fail.  The star's centre
+    row (100,000 entries) needs several passes of partial slots."""
+    rng = np.random.default_rng(11)
+    a = {"grid3": lambda: graphs.weighted_grid3(40, 40, 40, rng, "expU2"),
+         "path_matchings": lambda: graphs.path_matchings(60000, 4, rng, "logU3"),
+         "dense": lambda: graphs.random_dense(1500, 120, rng),
+         "star": lambda: _star(100000)}[name]()
+    d = str(tmp_path / name)
+    _dump(a, d)
+    split_rows = 0
+    for mode, dm in (("ctx", 0), ("ctx", 18432), ("wg", 0)):
+        for det in ("0", "1"):
+            out = subprocess.run([checker, d, mode, str(dm), det], capture_output=True, text=True,
+                                 check=True).stdout.split()
+            f = dict(zip(out[0::2], out[1::2]))
+            for k in ("err_fwd", "err_bwd", "err_dense"):
+                assert float(f[k]) < 1e-13, (mode, dm, det, k, f[k])
+            npart = [int(f["npart"]), int(f["npart_dense"])]
+            if det == "1":
+                assert int(f["lead2"]) == 0 and max(npart) <= 1024, (mode, dm, out)
+                split_rows += int(f["lead3"])
+            else:
+                assert int(f["lead3"]) == 0 and max(npart) == 0
+    assert split_rows > 0, "expected rows split in chunks"
