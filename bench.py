@@ -418,6 +418,78 @@ def serve_leg(args) -> dict:
     }
 
 
+def cache_leg(args) -> list:
+    """Admission of a graph that comes with its request, with and without
+    the factor/plan cache (lapgpu.cache): host seconds from the adjacency to
+    a graph ready to solve, for graphs of the serving families at three
+    sizes — without a cache (factor + plan + upload), into an empty cache
+    (the same plus the writes) and from a warm one (read factor and plan,
+    upload).  Graphs up to the batched-engine bound take the batched engine
+    (the factor is cached, its whole-graph plan is rebuilt: tens of ms)."""
+    import shutil
+    import tempfile
+
+    import lapgpu
+    from lapgpu import _native, workloads
+    from lapgpu.cache import GraphCache
+
+    root = tempfile.mkdtemp(prefix="lapgpu_cache_")
+    out = []
+    batch = _native.Batch(0)
+    try:
+        cache = GraphCache(root)
+        for i, nv in enumerate((1e5, 5e5, 2e6)):
+            rng = np.random.default_rng([9, i])
+            a = workloads.serve_graph(rng, nv)
+            n = a.shape[0]
+            wg = n <= args.new_wg_max_n
+            settings = "new"
+
+            def admit(la, f, use_cache, key):
+                if wg:
+                    g = batch.add_graph(la, f)
+                    batch.remove_graph(g)
+                    return
+                ctx = _native.Context(0)
+                ctx.set_group_budget(8, 2.0, 16384)
+                ctx.set_dense_tail(0)
+                if use_cache:
+                    cache.upload(ctx, key, settings, la, f)
+                else:
+                    ctx.upload_lap(la)
+                    ctx.upload_ldli(f)
+                ctx.close()
+
+            t0 = time.perf_counter()
+            la = lapgpu.lap(a)
+            f = lapgpu.build_ldli(a, seed=1)
+            t1 = time.perf_counter()
+            admit(la, f, False, None)
+            t2 = time.perf_counter()
+            key, la, f, hit = cache.factor(a, seed=1)
+            t3 = time.perf_counter()
+            admit(la, f, True, key)
+            t4 = time.perf_counter()
+            key, la, f, hit2 = cache.factor(a, seed=1)
+            t5 = time.perf_counter()
+            admit(la, f, True, key)
+            t6 = time.perf_counter()
+            row = {"n": int(n), "engine": "batched" if wg else "context",
+                   "no_cache_s": {"factor": t1 - t0, "plan_upload": t2 - t1, "total": t2 - t0},
+                   "cache_cold_s": {"factor": t3 - t2, "plan_upload": t4 - t3, "total": t4 - t2},
+                   "cache_warm_s": {"factor": t5 - t4, "plan_upload": t6 - t5, "total": t6 - t4},
+                   "hits": [bool(hit), bool(hit2)]}
+            out.append(row)
+            log(f"serve: admission n={n} ({row['engine']}): no cache {t2 - t0:.3f}s (factor {t1 - t0:.3f}), "
+                f"cold cache {t4 - t2:.3f}s, warm cache {t6 - t4:.3f}s (factor {t5 - t4:.3f}, "
+                f"plan+upload {t6 - t5:.3f})")
+        out.append({"cache_bytes": cache.bytes()})
+    finally:
+        batch.close()
+        shutil.rmtree(root, ignore_errors=True)
+    return out
+
+
 def mode_serve(args) -> int:
     # every resident graph owns a stream; HIP maps streams round-robin onto
     # its hardware queues (4 unless GPU_MAX_HW_QUEUES says otherwise), and two
@@ -426,6 +498,7 @@ def mode_serve(args) -> int:
     if args.hw_queues:
         os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     sv = serve_leg(args)
+    cl = None if args.no_cache_leg else cache_leg(args)
     pl = None if args.no_plain else plain_leg(args, 5, 1, cpu=not args.no_cpu)
     s = sv["summary"]
     emit({
@@ -455,6 +528,7 @@ def mode_serve(args) -> int:
             "speedup_solves_per_s": s["solves_per_s"] / sv["plain"]["solves_per_s"],
             "p99_ratio_plain_over_served": sv["plain"]["p99_ms"] / s["p99_ms"],
             "plain_grid3_128": pl["line"] if pl else None,
+            "admission_cache": cl,
         },
         "roofline": pl["roofline"] if pl else None,
         "cpu_baseline": pl["cpu_baseline"] if pl else None,
@@ -690,6 +764,7 @@ def main() -> int:
     ap.add_argument("--mode", default="serve", choices=["serve", "plain", "multi", "precision"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="serve: skip the plain grid3(128) leg")
+    ap.add_argument("--no-cache-leg", action="store_true", help="serve: skip the admission-cache measurement")
     ap.add_argument("--step-requests", type=int, default=STEP_REQUESTS)
     ap.add_argument("--check", type=int, default=200, help="serve: requests whose true residual is checked")
     ap.add_argument("--rhs", type=int, default=64)

@@ -128,6 +128,23 @@ int lapgpu_solve_multi_device(lapgpu_ctx* ctx, const double* d_B, double* d_X,
                               int64_t* its_out, double* relres_out,
                               int* stop_reason_out);
 
+/* The plan file: everything lapgpu_upload_csc + lapgpu_upload_ldli build
+ * on the host (level order, the permuted Laplacian, both sweep layouts, the
+ * tail and dense-block slot programs, the level groups), so that a graph
+ * seen before is admitted without planning again (the reference keeps the
+ * factor as an LDLinv, src/approxCholTypes.jl:73-79, and saves solver
+ * results with JLD2, compare/st_grid3.jl:71; it has no plan).
+ * lapgpu_save_plan writes the context's plan (after its uploads or a
+ * load) to `path` (written to path.tmp, then renamed).  lapgpu_load_plan
+ * replaces both uploads: it reads a file written by lapgpu_save_plan,
+ * refuses one whose checksum, version, shapes or indices do not fit (-1,
+ * with a message), uploads it and writes the graph's n to *n_out (may be
+ * NULL); the dense block's matrix is rebuilt on the device.  The file does not record the context settings it was
+ * planned with: the loaded plan keeps them (group budget, dense block,
+ * tail). */
+int lapgpu_save_plan(lapgpu_ctx* ctx, const char* path);
+int lapgpu_load_plan(lapgpu_ctx* ctx, const char* path, int64_t* n_out);
+
 /* Cancel the solve running on ctx from another thread (no reference
  * counterpart; for serving, where a client may withdraw a request): column
  * j of the running lapgpu_solve_multi call, or every column when j == -1

@@ -64,6 +64,10 @@ def build_gpu(verbose: bool = False) -> str:
     plan_o = os.path.join(OUT, "_lapgpu_plan.o")
     _run(["g++", "-O3", "-std=c++17", "-fPIC", "-c", f"-I{INC}", f"-I{CSRC}", "-o", plan_o, os.path.join(CSRC, "lapgpu_plan.cpp")], verbose)
     objs.append(plan_o)
+    planio_o = os.path.join(OUT, "_lapgpu_planio.o")
+    _run(["g++", "-O3", "-std=c++17", "-fPIC", "-c", f"-I{INC}", f"-I{CSRC}", "-o", planio_o,
+          os.path.join(CSRC, "lapgpu_planio.cpp")], verbose)
+    objs.append(planio_o)
     srcs = ["lapgpu.hip", "lapgpu_batch.hip"]
     outs = [os.path.join(OUT, f"_{os.path.splitext(s)[0]}.o") for s in srcs]
     with ThreadPoolExecutor(max_workers=len(srcs)) as ex:

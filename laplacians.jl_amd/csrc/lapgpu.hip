@@ -632,9 +632,9 @@ int build_dense(lapgpu_ctx* c) {
 
 // The level groups before the tail: composite rows, the per-group lane
 // count and the heavy-row lists of both sweeps.
-int upload_groups(lapgpu_ctx* c) {
+int upload_groups(lapgpu_ctx* c, bool blocks_ready) {
   Plan& P = c->plan;
-  build_blocks(P, c->block_depth, c->block_growth, c->block_extra);
+  if (!blocks_ready) build_blocks(P, c->block_depth, c->block_growth, c->block_extra);
   const LevelBlocks& B = P.blocks;
   c->groups.clear();
   std::vector<int32_t> heavy, mheavy;
@@ -700,7 +700,8 @@ int upload_groups(lapgpu_ctx* c) {
   return 0;
 }
 
-int upload_plan(lapgpu_ctx* c) {
+// blocks_ready: P.blocks already holds the level groups (a loaded plan)
+int upload_plan(lapgpu_ctx* c, bool blocks_ready = false) {
   Plan& P = c->plan;
   const int64_t n = P.n;
   int rc;
@@ -712,7 +713,7 @@ int upload_plan(lapgpu_ctx* c) {
   if ((rc = dev_upload(c, &c->Lidx, P.Lidx))) return rc;
   if ((rc = dev_upload(c, &c->Lval, P.Lval))) return rc;
   if ((rc = dev_upload(c, &c->s, P.s))) return rc;
-  if ((rc = upload_groups(c))) return rc;
+  if ((rc = upload_groups(c, blocks_ready))) return rc;
   if ((rc = dev_upload(c, &c->lvl, P.lvl))) return rc;
   if ((rc = dev_upload(c, &c->Eptr, P.tail.Eptr))) return rc;
   if ((rc = dev_upload(c, &c->Eidx, P.tail.Eidx))) return rc;
@@ -1649,7 +1650,7 @@ float mapply_time(lapgpu_ctx* c) {
 // and the FP32 group weights follow)
 int regroup(lapgpu_ctx* c) {
   drop_graphs(c);
-  int rc = upload_groups(c);
+  int rc = upload_groups(c, false);
   if (rc) return rc;
   if (c->f32_ready) {
     if ((rc = dev_upload(c, &c->gFw32, narrow<float>(c->plan.blocks.fw)))) return rc;
@@ -1817,7 +1818,8 @@ int lapgpu_upload_ldli(lapgpu_ctx* c, int64_t n, int64_t nnzF,
                        const int64_t* rowval, const double* fval,
                        const double* d, int index_base) {
   if (!c) return -1;
-  if (!c->has_L) return set_err(c, -2, "upload_ldli: upload the Laplacian first");
+  if (!c->has_L || c->L.n == 0)
+    return set_err(c, -2, "upload_ldli: upload the Laplacian first (a loaded plan keeps only its own)");
   if (n != c->L.n) return set_err(c, -1, "upload_ldli: n differs from the Laplacian's");
   if (index_base != 0 && index_base != 1)
     return set_err(c, -1, "upload_ldli: index_base must be 0 or 1");
@@ -1832,6 +1834,35 @@ int lapgpu_upload_ldli(lapgpu_ctx* c, int64_t n, int64_t nnzF,
   int rc = upload_plan(c);
   if (rc) return rc;
   c->has_F = true;
+  c->err.clear();
+  return 0;
+}
+
+int lapgpu_save_plan(lapgpu_ctx* c, const char* path) {
+  if (!c || !path) return -1;
+  if (!c->has_F) return set_err(c, -2, "save_plan: upload the Laplacian and LDLinv (or load a plan) first");
+  std::string err;
+  if (!save_plan(c->plan, path, err)) return set_err(c, -1, err);
+  c->err.clear();
+  return 0;
+}
+
+int lapgpu_load_plan(lapgpu_ctx* c, const char* path, int64_t* n_out) {
+  if (!c || !path) return -1;
+  HIPC(c, hipSetDevice(c->device));
+  std::string err;
+  Plan P;
+  if (!load_plan(path, P, err)) return set_err(c, -1, err);
+  HIPC(c, hipStreamSynchronize(c->stream));
+  c->plan = std::move(P);
+  // la lives in the plan, permuted; the caller-order copy is not kept
+  c->L = HostCsr();
+  c->has_L = true;
+  c->has_F = false;
+  int rc = upload_plan(c, true);
+  if (rc) return rc;
+  c->has_F = true;
+  if (n_out) *n_out = c->n;
   c->err.clear();
   return 0;
 }

@@ -182,6 +182,12 @@ bool build_plan(const HostCsr& L, int64_t nnzF, const int64_t* col,
 bool build_tail(Plan& P, int64_t max_steps, int64_t cap, int max_elog = kTailMaxElog,
                 int lanes = kTailLanes, const DenseModel& dense = DenseModel(), bool det = true);
 
+// The plan file (lapgpu_planio.cpp): the whole plan, including the level
+// groups, with a checksum; load_plan refuses a file that fails it or whose
+// shapes or indices do not fit together.
+bool save_plan(const Plan& P, const std::string& path, std::string& err);
+bool load_plan(const std::string& path, Plan& P, std::string& err);
+
 // Group the levels before the tail into launches of up to `max_depth`
 // levels while a group's composite rows stay within `growth` times the
 // entries of its levels (and `max_extra` extra entries).

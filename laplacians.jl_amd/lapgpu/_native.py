@@ -65,6 +65,8 @@ EXPORTS = (
     "lapgpu_last_dense_timings",
     "lapgpu_dense_info",
     "lapgpu_cancel",
+    "lapgpu_save_plan",
+    "lapgpu_load_plan",
     "lapgpu_batch_create",
     "lapgpu_batch_add_graph",
     "lapgpu_batch_remove_graph",
@@ -137,6 +139,8 @@ def load() -> ctypes.CDLL:
     L.lapgpu_batch_query.argtypes = [P, ctypes.c_int32, ctypes.POINTER(ci)]
     L.lapgpu_batch_cancel.argtypes = [P, ctypes.c_int32, i64]
     L.lapgpu_cancel.argtypes = [P, i64]
+    L.lapgpu_save_plan.argtypes = [P, ctypes.c_char_p]
+    L.lapgpu_load_plan.argtypes = [P, ctypes.c_char_p, pi64]
     L.lapgpu_batch_submit_host.argtypes = [P, i64, P, P, dbl, i64, dbl, ci, ci, pi32]
     L.lapgpu_batch_collect.argtypes = [P, ctypes.c_int32, P, P, P, P]
     L.lapgpu_batch_last_error.argtypes = [P]
@@ -285,6 +289,16 @@ class Context:
         nz = np.ascontiguousarray(la.data, dtype=np.float64)
         self._check(self._lib.lapgpu_upload_csc(self._h, n, _p(cp), _p(rv), _p(nz), 0), "lapgpu_upload_csc")
         self.n = n
+
+    def save_plan(self, path) -> None:
+        """Write the context's plan (lapgpu_save_plan) to `path`."""
+        self._check(self._lib.lapgpu_save_plan(self._h, os.fsencode(path)), "lapgpu_save_plan")
+
+    def load_plan(self, path) -> None:
+        """Replace upload_lap + upload_ldli by a plan file (lapgpu_load_plan)."""
+        n = ctypes.c_int64(0)
+        self._check(self._lib.lapgpu_load_plan(self._h, os.fsencode(path), ctypes.byref(n)), "lapgpu_load_plan")
+        self.n = n.value
 
     def upload_ldli(self, f) -> None:
         """Upload a 0-based LDLinv (lapgpu.factor.LDLinv)."""

@@ -57,6 +57,7 @@ import numpy as np
 import scipy.sparse as sp
 
 from . import _native
+from .cache import GraphCache
 from .factor import LDLinv, build_ldli
 from .graphs import lap
 
@@ -139,8 +140,11 @@ class Server:
                  max_wait: float = 1.0, new_graph_budget: tuple | None = (8, 2.0, 16384),
                  eager_new: bool = True, quick_workers: int = 4, quick_max_n: int = 300_000,
                  blocking_waits: bool = True, keep_ids: set | None = None,
-                 new_wg_max_n: int | None = 200_000):
+                 new_wg_max_n: int | None = 200_000, cache_dir: str | None = None):
         self.device = device
+        # factors and context plans of graphs seen before (lapgpu.cache):
+        # a graph that comes back is admitted without factoring or planning
+        self.cache = GraphCache(cache_dir) if cache_dir else None
         # the solutions kept in the results: all (keep_x) or those of keep_ids
         self.keep_ids = keep_ids
         self.batch = _native.Batch(device)
@@ -228,10 +232,14 @@ class Server:
         """Upload a graph (factoring it on the host if no LDLinv is given).
         Returns False when it does not fit in the HBM budget."""
         a = sp.csc_matrix(a)
-        la = lap(a)
-        if ldli is None:
-            ldli = build_ldli(a, seed=seed)
-        return self._admit(key, la, ldli, resident)
+        gkey = None
+        if ldli is None and self.cache is not None:
+            gkey, la, ldli, _ = self.cache.factor(a, seed=seed)
+        else:
+            la = lap(a)
+            if ldli is None:
+                ldli = build_ldli(a, seed=seed)
+        return self._admit(key, la, ldli, resident, gkey)
 
     def _reserve(self, la, ldli, wg: bool):
         """HBM footprint and multi-RHS width of a graph on the batched engine
@@ -255,7 +263,7 @@ class Server:
                 need += multi_bytes_estimate(n, k)
         return need, k
 
-    def _upload_context(self, la, ldli, budget=None, dense_max=None):
+    def _upload_context(self, la, ldli, budget=None, dense_max=None, gkey=None):
         c0 = time.thread_time()
         t0 = time.perf_counter()
         ctx = _native.Context(self.device)
@@ -266,9 +274,15 @@ class Server:
         if dense_max is not None:
             ctx.set_dense_tail(dense_max)
         t1 = time.perf_counter()
-        ctx.upload_lap(la)
-        t2 = time.perf_counter()
-        ctx.upload_ldli(ldli)
+        if self.cache is not None and gkey is not None:
+            # the plan depends on the settings it was built with
+            settings = f"b{'-'.join(map(str, budget)) if budget else 'default'}_d{dense_max}"
+            self.cache.upload(ctx, gkey, settings, la, ldli)
+            t2 = time.perf_counter()
+        else:
+            ctx.upload_lap(la)
+            t2 = time.perf_counter()
+            ctx.upload_ldli(ldli)
         t3 = time.perf_counter()
         self.upload_log.append((t3 - t0, t1 - t0, t2 - t1, t3 - t2))
         self.upload_cpu_log.append(time.thread_time() - c0)
@@ -289,7 +303,7 @@ class Server:
         self.hbm_used += g.bytes
         return g
 
-    def _admit(self, key, la, ldli, resident) -> bool:
+    def _admit(self, key, la, ldli, resident, gkey=None) -> bool:
         n = la.shape[0]
         wg = n <= self.wg_max_n
         res = self._reserve(la, ldli, wg)
@@ -299,7 +313,7 @@ class Server:
         if wg:
             self._wg_graph(key, n, self._upload_wg(la, ldli), resident)
         else:
-            ctx = self._upload_context(la, ldli)
+            ctx = self._upload_context(la, ldli, gkey=gkey)
             g = _Graph(key, n, "plain", need + dense_correction(ctx, n), resident, ctx, max_rhs=k)
             self.graphs[key] = g
             self.hbm_used += g.bytes
@@ -444,7 +458,7 @@ class Server:
                 waiting_admission.append((r, fut.result()))
                 progressed = True
             for _ in range(len(waiting_admission)):
-                r, (la, ldli) = waiting_admission.popleft()
+                r, (la, ldli, gkey) = waiting_admission.popleft()
                 if self._is_cancelled(r.rid):
                     finish_cancelled(r, "queued")
                     progressed = True
@@ -453,7 +467,7 @@ class Server:
                 wg = la.shape[0] <= self.new_wg_max_n
                 res = self._reserve(la, ldli, wg)
                 if res is None:
-                    waiting_admission.append((r, (la, ldli)))
+                    waiting_admission.append((r, (la, ldli, gkey)))
                     continue
                 self._anon += 1
                 key = ("new", self._anon)
@@ -464,7 +478,8 @@ class Server:
                 if wg:
                     fut = self._factor_pool.submit(self._upload_wg, la, ldli)
                 else:
-                    fut = self._factor_pool.submit(self._upload_context, la, ldli, self.new_graph_budget, 0)
+                    fut = self._factor_pool.submit(self._upload_context, la, ldli, self.new_graph_budget, 0,
+                                                   gkey)
                 upload_futs[fut] = (key, r, la.shape[0], need, k, wg)
                 self.stats["admit_s"] += time.perf_counter() - ta
                 self.stats["admits"] += 1
@@ -589,10 +604,16 @@ class Server:
         return [results[r.rid] for r in requests]
 
     def _factor(self, a):
+        """(la, LDLinv, cache key or None) of a graph that came with its
+        request: from the cache when it has the graph, else factored."""
         c = time.thread_time()
         t = time.perf_counter()
         a = sp.csc_matrix(a)
-        out = lap(a), build_ldli(a, seed=1)
+        if self.cache is not None:
+            gkey, la, f, _ = self.cache.factor(a, seed=1)
+            out = la, f, gkey
+        else:
+            out = lap(a), build_ldli(a, seed=1), None
         self.factor_log.append(time.perf_counter() - t)
         self.factor_cpu_log.append(time.thread_time() - c)
         return out
