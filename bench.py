@@ -499,6 +499,21 @@ def mode_serve(args) -> int:
         os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     sv = serve_leg(args)
     cl = None if args.no_cache_leg else cache_leg(args)
+    # the precision and batching forms on their own workloads, a few solves
+    # each (their own modes run them at length)
+    forms = None
+    if not args.no_forms:
+        import copy
+
+        fa = copy.copy(args)
+        fa.steps = args.form_steps
+        forms = {}
+        for name, leg in (("precision", precision_leg), ("batching", multi_leg)):
+            t0 = time.time()
+            ln = leg(fa)
+            forms[name] = {"metric": ln["metric"], "value": ln["value"], "unit": ln["unit"],
+                           "dtype": ln["dtype"], "steps": fa.steps, "config": ln["config"]}
+            log(f"forms: {name} {ln['value']:.2f} {ln['unit']} in {time.time() - t0:.0f}s")
     pl = None if args.no_plain else plain_leg(args, 5, 1, cpu=not args.no_cpu)
     s = sv["summary"]
     emit({
@@ -529,6 +544,7 @@ def mode_serve(args) -> int:
             "p99_ratio_plain_over_served": sv["plain"]["p99_ms"] / s["p99_ms"],
             "plain_grid3_128": pl["line"] if pl else None,
             "admission_cache": cl,
+            "forms": forms,
         },
         "roofline": pl["roofline"] if pl else None,
         "cpu_baseline": pl["cpu_baseline"] if pl else None,
@@ -537,7 +553,9 @@ def mode_serve(args) -> int:
 
 
 # ----------------------------------------------------------------- multi ---
-def mode_multi(args) -> int:
+def multi_leg(args) -> dict:
+    """The batching form on its workload (64-RHS block of grid2(1024) and
+    256 small grids): the bench line of --mode multi."""
     import lapgpu
     from lapgpu import _native, workloads
     from lapgpu._native import DeviceArray, synchronize
@@ -654,7 +672,7 @@ def mode_multi(args) -> int:
     plain_wall = k * t_big + len(smalls) * float(np.mean(t_small))
     plain_sps = (k + len(smalls)) / plain_wall
     best = out_modes["fp32_inner"]  # the workload names FP64 outer / FP32 inner
-    emit({
+    line = {
         "metric": "solves/s (64-RHS block + 256 small grids, relres<=1e-8) vs plain FP64 GPU PCG",
         "value": best["solves_per_s"],
         "unit": "solves/s",
@@ -683,14 +701,23 @@ def mode_multi(args) -> int:
         },
         "roofline": None,
         "cpu_baseline": None,
-    })
+    }
     ctx.close()
     batch.close()
+    return line
+
+
+def mode_multi(args) -> int:
+    emit(multi_leg(args))
     return 0
 
 
 # ------------------------------------------------------------- precision ---
-def mode_precision(args) -> int:
+def precision_leg(args) -> dict:
+    """The precision form on its workload (path + 4 matchings, n = 1e6):
+    FP32-inner against FP64 on the same factor, with each solve's FP64 true
+    residual as the device reports it (lapgpu_last_true_relres): the bench
+    line of --mode precision."""
     import lapgpu
     from lapgpu import _native, workloads
     from lapgpu._native import DeviceArray, synchronize
@@ -711,23 +738,25 @@ def mode_precision(args) -> int:
         ctx.set_precision(prec, 20)
         ctx.solve_device(db.address, dx.address, n, tol=1e-8, maxits=1000)
         synchronize()
-        ts = []
+        ts, trs = [], []
         for _ in range(args.steps):
             t0 = time.perf_counter()
             r = ctx.solve_device(db.address, dx.address, n, tol=1e-8, maxits=1000)
             synchronize()
             ts.append(time.perf_counter() - t0)
+            trs.append(ctx.last_true_relres(1))
         x = dx.download()
         ctx.solve_device(db.address, dx.address, n, tol=1e-8, maxits=1000, flags=_native.PROFILE)
         log(f"{name} schedule {ctx.factor_info()} tail {ctx.tail_info()} dense {ctx.dense_info()} per-iteration ms "
             f"{ctx.last_timings()} {ctx.last_sweep_timings()} {ctx.last_dense_timings()}")
         res[name] = {"its": r["its"], "stop": r["stop"], "solve_ms": float(np.median(ts)) * 1e3,
-                     "true_relres": float(np.linalg.norm(la @ x - b) / np.linalg.norm(b)), "x": x}
+                     "true_relres": float(np.linalg.norm(la @ x - b) / np.linalg.norm(b)),
+                     "device_true_relres_per_solve": trs, "x": x}
         log(f"{name}: {res[name]['its']} its {res[name]['solve_ms']:.1f} ms relres {res[name]['true_relres']:.2e}")
     err = float(np.linalg.norm(res["fp32_inner"]["x"] - res["fp64"]["x"]) / np.linalg.norm(res["fp64"]["x"]))
     for v in res.values():
         v.pop("x")
-    emit({
+    line = {
         "metric": "FP32-inner vs FP64 solve (relres<=1e-8): error vs FP64 and iteration ratio",
         "value": 1e3 / res["fp32_inner"]["solve_ms"],
         "unit": "solves/s",
@@ -747,11 +776,17 @@ def mode_precision(args) -> int:
             "fp32_inner": res["fp32_inner"],
             "rel_err_vs_fp64": err,
             "iteration_ratio": res["fp32_inner"]["its"] / max(res["fp64"]["its"], 1),
+            "speedup_vs_fp64": res["fp64"]["solve_ms"] / res["fp32_inner"]["solve_ms"],
         },
         "roofline": None,
         "cpu_baseline": None,
-    })
+    }
     ctx.close()
+    return line
+
+
+def mode_precision(args) -> int:
+    emit(precision_leg(args))
     return 0
 
 
@@ -765,6 +800,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="serve: skip the plain grid3(128) leg")
     ap.add_argument("--no-cache-leg", action="store_true", help="serve: skip the admission-cache measurement")
+    ap.add_argument("--no-forms", action="store_true", help="serve: skip the precision and batching lines")
+    ap.add_argument("--form-steps", type=int, default=5, help="serve: timed solves of each form")
     ap.add_argument("--step-requests", type=int, default=STEP_REQUESTS)
     ap.add_argument("--check", type=int, default=200, help="serve: requests whose true residual is checked")
     ap.add_argument("--rhs", type=int, default=64)

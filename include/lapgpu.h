@@ -128,6 +128,17 @@ int lapgpu_solve_multi_device(lapgpu_ctx* ctx, const double* d_B, double* d_X,
                               int64_t* its_out, double* relres_out,
                               int* stop_reason_out);
 
+/* The FP64 true relative residual ||(b - mean b) - la x|| / ||b - mean b||
+ * (no mean removed under LAPGPU_NO_CENTER) of each column that the last
+ * solve call on ctx returned: k = 1 after lapgpu_solve(_device), the call's
+ * k after lapgpu_solve_multi(_device); -1 for another k.  It is computed on
+ * the device after the solve (one more SpMV, in FP64 whatever the
+ * precision mode), so in FP32-inner mode it is the per-solve error
+ * against FP64 that the precision form reports (convergence there is
+ * accepted only on this residual).  The reference's pcg reports only the
+ * recursive residual (src/pcg.jl:434-437). */
+int lapgpu_last_true_relres(lapgpu_ctx* ctx, double* out, int64_t k);
+
 /* The plan file: everything lapgpu_upload_csc + lapgpu_upload_ldli build
  * on the host (level order, the permuted Laplacian, both sweep layouts, the
  * tail and dense-block slot programs, the level groups), so that a graph

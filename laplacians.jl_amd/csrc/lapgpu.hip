@@ -126,6 +126,8 @@ struct lapgpu_ctx {
   bool running = false;
   // global column index of column 0 of the multi-RHS chunk being solved
   int64_t col_base = 0, chunk0 = 0;
+  // FP64 true relative residual of each column the last solve call returned
+  std::vector<double> last_true;
 
   // device graph
   int *perm = nullptr, *Lptr = nullptr, *Lidx = nullptr;
@@ -359,6 +361,14 @@ void launch_spmv(lapgpu_ctx* c, const double* p, double* q) {
 // q = la * p (p.q partials); FP32 mode reads the FP32 copies of la and p
 void spmv(lapgpu_ctx* c, const double* p, double* q) {
   LAPGPU_DISPATCH_G(c->G_spmv, launch_spmv<G_>, c, p, q);
+}
+
+// the FP64 true residual of the returned x (bestx after the final copy)
+template <int G>
+void launch_true_resid(lapgpu_ctx* c) {
+  hipLaunchKernelGGL(k_true_resid<G>, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, (int)c->n,
+                     c->Lptr, c->Lidx, c->Lval, c->bestx, c->b, c->part, c->st);
+  hipLaunchKernelGGL(k_scalar_true, dim3(1), dim3(kScalarThreads), 0, c->stream, c->part, c->st);
 }
 
 template <int G>
@@ -1001,6 +1011,7 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
   }
   hipLaunchKernelGGL(k_final_copy, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
                      c->x, c->bestx, c->st);
+  LAPGPU_DISPATCH_G(c->G_spmv, launch_true_resid<G_>, c);
   hipLaunchKernelGGL(k_scatter_perm, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
                      c->perm, c->bestx, d_x);
   HIPC(c, hipMemcpyAsync(c->hst, c->st, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
@@ -1012,6 +1023,7 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
   if (its_out) *its_out = h.itcnt;
   if (relres_out) *relres_out = h.nr;
   if (stop_out) *stop_out = h.stop_reason;
+  c->last_true.assign(1, h.true_nr);
   if (prof && h.itcnt > 0) {
     c->t_spmv = acc_spmv / (double)enq;
     c->t_ldl = acc_ldl / (double)enq;
@@ -1424,6 +1436,11 @@ int solve_multi_chunk(lapgpu_ctx* c, const double* d_B, double* d_X, int k, doub
         hipLaunchKernelGGL(k_cancel, dim3(1), dim3(kWave), 0, c->stream, m.st, k, mk);
     }
   }
+  // the true residual reads B, which the final copy may overwrite (X = B)
+  hipLaunchKernelGGL(k_mtrue_resid<K>, dim3(kMRed), dim3(kBlock), 0, c->stream, n, k, c->perm,
+                     c->Lptr, c->Lidx, c->Lval, m.x, m.bestx, d_B, m.part, m.st);
+  hipLaunchKernelGGL(k_mscalar_true<K>, dim3(1), dim3(kScalarThreads), 0, c->stream, m.part,
+                     m.st);
   hipLaunchKernelGGL(k_mfinal<K>, dim3(kMRed), dim3(kBlock), 0, c->stream, n, k, c->perm, m.x,
                      m.bestx, d_X, m.st);
   HIPC(c, hipMemcpyAsync(m.hst, m.st, sizeof(DevState) * K, hipMemcpyDeviceToHost, c->stream));
@@ -1434,6 +1451,7 @@ int solve_multi_chunk(lapgpu_ctx* c, const double* d_B, double* d_X, int k, doub
     if (its_out) its_out[j] = h.itcnt;
     if (relres_out) relres_out[j] = h.nr;
     if (stop_out) stop_out[j] = h.stop_reason;
+    if ((size_t)(c->chunk0 + j) < c->last_true.size()) c->last_true[c->chunk0 + j] = h.true_nr;
     if (h.stop_reason == 3) printf("PCG Stopped by stagnation test %d\n", stag_test);
     if ((flags & LAPGPU_VERBOSE) && h.stop_reason != 9)
       printf("rhs %d: PCG stopped after %lld iterations with relative error %.17g (%s).\n", j,
@@ -1880,6 +1898,16 @@ int lapgpu_solve_device(lapgpu_ctx* c, const double* d_b, double* d_x,
                       relres_out, stop_out);
 }
 
+int lapgpu_last_true_relres(lapgpu_ctx* c, double* out, int64_t k) {
+  if (!c || !out) return -1;
+  if (k != (int64_t)c->last_true.size())
+    return set_err(c, -1, "last_true_relres: k differs from the last solve's column count (" +
+                              std::to_string(c->last_true.size()) + ")");
+  std::copy(c->last_true.begin(), c->last_true.end(), out);
+  c->err.clear();
+  return 0;
+}
+
 int lapgpu_cancel(lapgpu_ctx* c, int64_t column) {
   if (!c || column < -1) return -1;
   // c->err belongs to the solving thread: errors are return codes only
@@ -1924,6 +1952,7 @@ int lapgpu_solve_multi_device(lapgpu_ctx* c, const double* d_B, double* d_X, int
   if (k == 0) return 0;
   HIPC(c, hipSetDevice(c->device));
   RunGuard run(c, k);
+  c->last_true.assign((size_t)k, NAN);
   c->col_base = 0;
   return solve_multi_device(c, d_B, d_X, k, tol, maxits, maxtime, stag_test, flags, its_out,
                             relres_out, stop_out);
@@ -1939,6 +1968,7 @@ int lapgpu_solve_multi(lapgpu_ctx* c, const double* B, double* X_out, int64_t n,
   if (k == 0) return 0;
   HIPC(c, hipSetDevice(c->device));
   RunGuard run(c, k);
+  c->last_true.assign((size_t)k, NAN);
   // staging: one chunk of up to 64 columns at a time through m.io
   for (int64_t c0 = 0; c0 < k; c0 += kMultiMax) {
     const int64_t kc = std::min<int64_t>(kMultiMax, k - c0);

@@ -42,6 +42,8 @@ struct DevState {
   int32_t verify;  // set when the recursive residual passed tol (FP32 mode)
   int32_t spmv64;  // FP32 mode: the SpMV has switched to FP64
   int32_t pad2;
+  double bmu;      // mean of b removed by the closure (0 without centering)
+  double true_nr;  // after the solve: ||b - bmu - la x|| / nb in FP64
 };
 
 // ---------------------------------------------------------------------------

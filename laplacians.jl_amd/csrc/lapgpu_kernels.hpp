@@ -170,7 +170,7 @@ __global__ void k_scatter_perm(int n, const int* __restrict__ perm,
 __global__ void k_scalar_mean(int n, const double* __restrict__ part,
                               DevState* st) {
   const double s = wave_total(part, S_SUM);
-  if (threadIdx.x == 0) st->mu = st->center ? s / (double)n : 0.0;
+  if (threadIdx.x == 0) st->mu = st->bmu = st->center ? s / (double)n : 0.0;
 }
 
 // r = b - mean(b); x = bestx = 0; partial r.r
@@ -619,6 +619,34 @@ k_resid(int n, int always, const int* __restrict__ Lptr,
                        });
   block_sum<1>(v);
   if (threadIdx.x == 0) part[S_RR * kRedBlocks + blockIdx.x] = v[0];
+}
+
+// After the solve: (b - bmu) - la x in FP64 for the returned x (level
+// order), partial sums of squares
+template <int G>
+__global__ void __launch_bounds__(kBlock)
+k_true_resid(int n, const int* __restrict__ Lptr, const int* __restrict__ Lidx,
+             const double* __restrict__ Lval, const double* __restrict__ x,
+             const double* __restrict__ b, double* __restrict__ part,
+             const DevState* __restrict__ st) {
+  const int lane = threadIdx.x % G;
+  const int rows_per_grid = gridDim.x * (kBlock / G);
+  const double mu = st->bmu;
+  double v[1] = {0.0};
+  rows_in_flight<G, 4>(blockIdx.x * (kBlock / G) + threadIdx.x / G, rows_per_grid, n, Lptr, Lidx,
+                       Lval, [&](int i) { return x[i]; }, [&](int row, double acc) {
+                         if (lane == 0) {
+                           const double t = (b[row] - mu) - acc;
+                           v[0] += t * t;
+                         }
+                       });
+  block_sum<1>(v);
+  if (threadIdx.x == 0) part[S_RR * kRedBlocks + blockIdx.x] = v[0];
+}
+
+__global__ void k_scalar_true(const double* __restrict__ part, DevState* st) {
+  const double rr = wave_total(part, S_RR);
+  if (threadIdx.x == 0) st->true_nr = st->nb > 0.0 ? sqrt(rr) / st->nb : 0.0;
 }
 
 // After k_resid: the true relative residual replaces nr; convergence is

@@ -245,7 +245,7 @@ __global__ void k_mscalar_mean(int n, const double* __restrict__ part, DevState*
   const double s = mtotal<K>(part, S_SUM);
   if (threadIdx.x < K) {
     DevState* st = mst + threadIdx.x;
-    st->mu = st->center ? s / (double)n : 0.0;
+    st->mu = st->bmu = st->center ? s / (double)n : 0.0;
   }
 }
 
@@ -906,6 +906,45 @@ k_mresid(int n, int always, const int* __restrict__ Lptr, const int* __restrict_
                         });
   const int slots[1] = {S_RR};
   mblock_sum<K, 1>(a, part, slots);
+}
+
+// After the solve: (b - bmu) - la x in FP64 for each returned column
+// (x = the iterate k_mfinal returned: x when its copy is pending, else
+// bestx; b from the caller's column-major B through perm), partial sums of
+// squares per column
+template <int K>
+__global__ void __launch_bounds__(kBlock)
+k_mtrue_resid(int n, int k, const int* __restrict__ perm, const int* __restrict__ Lptr,
+              const int* __restrict__ Lidx, const double* __restrict__ Lval,
+              const double* __restrict__ x, const double* __restrict__ bestx,
+              const double* __restrict__ B, double* __restrict__ part,
+              const DevState* __restrict__ mst) {
+  constexpr int R = kWave / K;
+  const int lane = threadIdx.x % kWave, j = lane % K, sub = lane / K;
+  const bool run = j < k;
+  const double* __restrict__ src = mst[j].pending_copy ? x : bestx;
+  const double mu = mst[j].bmu;
+  const int wave = (blockIdx.x * kBlock + threadIdx.x) / kWave;
+  const int nwaves = gridDim.x * (kBlock / kWave);
+  double a[1] = {0.0};
+  mrows_in_flight<K, 2>(wave * R + sub, nwaves * R, n, j, sub, Lptr, Lidx, Lval,
+                        [&](int i) { return src[(int64_t)i * K + j]; }, [&](int row, double acc) {
+                          if (run) {
+                            const double t = (B[(int64_t)j * n + perm[row]] - mu) - acc;
+                            a[0] += t * t;
+                          }
+                        });
+  const int slots[1] = {S_RR};
+  mblock_sum<K, 1>(a, part, slots);
+}
+
+template <int K>
+__global__ void k_mscalar_true(const double* __restrict__ part, DevState* mst) {
+  const double rr = mtotal<K>(part, S_RR);
+  if (threadIdx.x < K) {
+    DevState* st = mst + threadIdx.x;
+    st->true_nr = st->nb > 0.0 ? sqrt(rr) / st->nb : 0.0;
+  }
 }
 
 // after k_mresid: the true relative residual replaces nr (as

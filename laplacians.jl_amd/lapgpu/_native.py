@@ -66,6 +66,7 @@ EXPORTS = (
     "lapgpu_dense_info",
     "lapgpu_cancel",
     "lapgpu_save_plan",
+    "lapgpu_last_true_relres",
     "lapgpu_load_plan",
     "lapgpu_batch_create",
     "lapgpu_batch_add_graph",
@@ -140,6 +141,7 @@ def load() -> ctypes.CDLL:
     L.lapgpu_batch_cancel.argtypes = [P, ctypes.c_int32, i64]
     L.lapgpu_cancel.argtypes = [P, i64]
     L.lapgpu_save_plan.argtypes = [P, ctypes.c_char_p]
+    L.lapgpu_last_true_relres.argtypes = [P, P, i64]
     L.lapgpu_load_plan.argtypes = [P, ctypes.c_char_p, pi64]
     L.lapgpu_batch_submit_host.argtypes = [P, i64, P, P, dbl, i64, dbl, ci, ci, pi32]
     L.lapgpu_batch_collect.argtypes = [P, ctypes.c_int32, P, P, P, P]
@@ -289,6 +291,13 @@ class Context:
         nz = np.ascontiguousarray(la.data, dtype=np.float64)
         self._check(self._lib.lapgpu_upload_csc(self._h, n, _p(cp), _p(rv), _p(nz), 0), "lapgpu_upload_csc")
         self.n = n
+
+    def last_true_relres(self, k: int = 1):
+        """FP64 true relative residuals of the last solve's k columns
+        (lapgpu_last_true_relres): a float for k = 1, else a list."""
+        out = np.zeros(k)
+        self._check(self._lib.lapgpu_last_true_relres(self._h, _p(out), int(k)), "lapgpu_last_true_relres")
+        return float(out[0]) if k == 1 else out.tolist()
 
     def save_plan(self, path) -> None:
         """Write the context's plan (lapgpu_save_plan) to `path`."""

@@ -64,7 +64,11 @@ def _set_its(pcgIts, its):
 class GpuLapSolver:
     """The solve closure of approxchol_lapGreedy (src/approxChol.jl:1624):
     f(b; tol, maxits, maxtime, verbose, pcgIts) = pcg(la, b .- mean(b), F; ...)
-    with keyword defaults captured at construction."""
+    with keyword defaults captured at construction.  One more optional
+    output beside pcgIts: trueRelres[0] receives the FP64 true relative
+    residual ||(b .- mean(b)) - la x|| / ||b .- mean(b)|| of the returned x
+    (lapgpu_last_true_relres; in FP32-inner mode the solve's error against
+    FP64), also kept in `last["true_relres"]`."""
 
     def __init__(self, ctx: _native.Context, ldli: LDLinv, tol, maxits, maxtime, verbose, pcgIts, stag_test):
         self.ctx = ctx
@@ -73,7 +77,7 @@ class GpuLapSolver:
         self.stag_test = stag_test
         self.last = None
 
-    def __call__(self, b, tol=None, maxits=None, maxtime=None, verbose=None, pcgIts=None):
+    def __call__(self, b, tol=None, maxits=None, maxtime=None, verbose=None, pcgIts=None, trueRelres=None):
         d = self.defaults
         tol = d["tol"] if tol is None else tol
         maxits = d["maxits"] if maxits is None else maxits
@@ -86,7 +90,10 @@ class GpuLapSolver:
         mi = -1 if (maxits is None or (isinstance(maxits, float) and math.isinf(maxits))) else int(maxits)
         flags = _native.VERBOSE if verbose else 0
         x, info = self.ctx.solve(b, tol=tol, maxits=mi, maxtime=float(maxtime), stag_test=self.stag_test, flags=flags)
+        info["true_relres"] = self.ctx.last_true_relres(1)
         self.last = info
+        if trueRelres is not None and len(trueRelres) > 0:
+            trueRelres[0] = info["true_relres"]
         # a zero right-hand side returns before pcg writes pcgIts
         # (src/pcg.jl:269-271)
         if info["stop"] != "zero_rhs":
@@ -99,12 +106,16 @@ _ORDER_NAMES = {"deg": "greedy degree", "wdeg": "wted degree", "random": "random
 
 def approxchol_lap1_gpu(a, tol=1e-6, maxits=1000, maxtime=math.inf, verbose=False, pcgIts=None,
                         params: ApproxCholParams | None = None, seed: int = 1, device: int = 0,
-                        ldli: LDLinv | None = None):
+                        ldli: LDLinv | None = None, precision: str = "fp64", replace_every: int = 20):
     """approxchol_lap1 (src/approxChol.jl:1791-1851) for a connected graph:
     la = lap(a) and the LDLinv of params.order on the host (approxchol_lapGreedy
     :deg with its split/merge forms, approxchol_lapWdeg, approxchol_lapRandom,
     and approxchol_lapGiven for any other order, src/approxChol.jl:1581-1790),
-    both uploaded once; returns the GPU solve closure."""
+    both uploaded once; returns the GPU solve closure.  precision="fp32"
+    selects the FP32-inner form (lapgpu_set_precision: SpMV and LDLinv apply
+    in FP32, FP64 scalars, an FP64 residual replacement every replace_every
+    iterations); the closure's trueRelres then reports each solve's FP64 true
+    residual."""
     params = params or ApproxCholParams()
     t1 = time.time()
     a = sp.csc_matrix(a)
@@ -113,9 +124,13 @@ def approxchol_lap1_gpu(a, tol=1e-6, maxits=1000, maxtime=math.inf, verbose=Fals
     if ldli is None:
         ldli = build_ldli(a, seed=seed, order=order, split=params.split if order == "deg" else 0,
                           merge=params.merge if order == "deg" else 0)
+    if precision not in ("fp64", "fp32"):
+        raise ValueError(f"precision must be 'fp64' or 'fp32', not {precision!r}")
     ctx = _native.Context(device)
     ctx.upload_lap(la)
     ctx.upload_ldli(ldli)
+    if precision == "fp32":
+        ctx.set_precision(_native.FP32_INNER, replace_every)
     if verbose:
         print(f"Using {_ORDER_NAMES.get(order, 'given')} ordering. Factorization time: {time.time() - t1}")
         print(f"Factorization size: {2 * ldli.nnz}")
@@ -127,10 +142,13 @@ def approxchol_lap1_gpu(a, tol=1e-6, maxits=1000, maxtime=math.inf, verbose=Fals
 
 def block_solver(comps, solvers, tol=1e-6, maxits=1000, maxtime=math.inf, verbose=False, pcgIts=None):
     """blockSolver (src/solverInterface.jl:152-188): apply the i-th solver on
-    the i-th component; pcgIts[1] receives the max over components."""
+    the i-th component; pcgIts[1] receives the max over components, and
+    trueRelres[1] (a GPU closure's extra output) the true relative residual
+    of the whole solution, sqrt(sum_c (t_c |b_c|)^2 / sum_c |b_c|^2) over the
+    components' centred right-hand sides b_c."""
     defaults = dict(tol=tol, maxits=maxits, maxtime=maxtime, verbose=verbose, pcgIts=pcgIts)
 
-    def f(b, tol=None, maxits=None, maxtime=None, verbose=None, pcgIts=None):
+    def f(b, tol=None, maxits=None, maxtime=None, verbose=None, pcgIts=None, trueRelres=None):
         tol = defaults["tol"] if tol is None else tol
         maxits = defaults["maxits"] if maxits is None else maxits
         maxtime = defaults["maxtime"] if maxtime is None else maxtime
@@ -143,12 +161,24 @@ def block_solver(comps, solvers, tol=1e-6, maxits=1000, maxtime=math.inf, verbos
             tmp = []
         b = np.asarray(b, dtype=np.float64)
         x = np.zeros(len(b))
+        num = den = 0.0
         for i, ind in enumerate(comps):
             if verbose:
                 print(f"BlockSolver component index {i + 1}")
-            x[ind] = solvers[i](b[ind], tol=tol, maxits=maxits, maxtime=maxtime, verbose=verbose, pcgIts=tmp)
+            if trueRelres is not None and isinstance(solvers[i], GpuLapSolver):
+                tr = [0.0]
+                x[ind] = solvers[i](b[ind], tol=tol, maxits=maxits, maxtime=maxtime, verbose=verbose, pcgIts=tmp,
+                                    trueRelres=tr)
+                bc = b[ind] - b[ind].mean()
+                nb2 = float(bc @ bc)
+                num += tr[0] ** 2 * nb2
+                den += nb2
+            else:
+                x[ind] = solvers[i](b[ind], tol=tol, maxits=maxits, maxtime=maxtime, verbose=verbose, pcgIts=tmp)
             if pcgIts is not None and len(pcgIts) > 0:
                 pcgIts[0] = max(pcgIts[0], tmp[0])
+        if trueRelres is not None and len(trueRelres) > 0:
+            trueRelres[0] = math.sqrt(num / den) if den > 0 else 0.0
         return x
 
     return f
