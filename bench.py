@@ -790,13 +790,69 @@ def mode_precision(args) -> int:
     return 0
 
 
+# --------------------------------------------------------------- batched ---
+def mode_batched(args) -> int:
+    """The batched engine alone: the 256 small unit 2-D grids of the
+    batching workload (n log-uniform in [1e3, 1e5], seed 4), one request
+    each, in one launch per step (k_wg_pcg, one workgroup per request)."""
+    import lapgpu
+    from lapgpu import _native, workloads
+    from lapgpu._native import DeviceArray, synchronize
+    from concurrent.futures import ThreadPoolExecutor
+
+    _, _, smalls, sb = workloads.config4(rhs=1, small=args.small)
+    with ThreadPoolExecutor(8) as ex:
+        sf = list(ex.map(lambda g: lapgpu.build_ldli(g, seed=1), smalls))
+    batch = _native.Batch(0)
+    gids = [batch.add_graph(lapgpu.lap(g), f) for g, f in zip(smalls, sf)]
+    dsb = [DeviceArray.from_host(v) for v in sb]
+    dsx = [DeviceArray(len(v)) for v in sb]
+
+    def run():
+        return batch.solve_device(gids, [d.address for d in dsb], [d.address for d in dsx], tol=1e-8,
+                                  maxits=1000)
+
+    for _ in range(args.warmup):
+        run()
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        infos = run()
+    synchronize()
+    wall = time.perf_counter() - t0
+    rel = max(float(np.linalg.norm(lapgpu.lap(g) @ d.download() - v) / np.linalg.norm(v))
+              for g, d, v in zip(smalls, dsx, sb))
+    n_tot = sum(g.shape[0] for g in smalls)
+    emit({
+        "metric": "solves/s (batched engine, 256 small grids per launch, relres<=1e-8)",
+        "value": len(smalls) * args.steps / wall,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{len(smalls)} unit 2-D grids, n log-uniform [1e3,1e5] (total {n_tot}), aspect "
+                               "[1,8], 1 RHS each N(0,1) sum-zero, seed 4, tol 1e-8, one launch per step",
+                   "its_max": int(max(i["its"] for i in infos)), "true_relres_max": rel},
+        "roofline": None,
+        "cpu_baseline": None,
+    })
+    batch.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20,
                     help="serve: timed steps of --step-requests requests; plain/multi/precision: timed solves")
     ap.add_argument("--warmup", type=int, default=2, help="serve: untimed steps of another stream first")
-    ap.add_argument("--mode", default="serve", choices=["serve", "plain", "multi", "precision"])
+    ap.add_argument("--mode", default="serve", choices=["serve", "plain", "multi", "precision", "batched"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="serve: skip the plain grid3(128) leg")
     ap.add_argument("--no-cache-leg", action="store_true", help="serve: skip the admission-cache measurement")
@@ -835,7 +891,8 @@ def main() -> int:
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1"}))
         return 2
-    return {"plain": mode_plain, "multi": mode_multi, "precision": mode_precision, "serve": mode_serve}[args.mode](args)
+    return {"plain": mode_plain, "multi": mode_multi, "precision": mode_precision, "serve": mode_serve,
+            "batched": mode_batched}[args.mode](args)
 
 
 if __name__ == "__main__":

@@ -31,6 +31,21 @@ enum Slot {
   kNumSlots = 10
 };
 
+// MI355X deals workgroups to its 8 XCDs round-robin (workgroup b to XCD
+// b mod 8), and each XCD has its own L2.  A kernel that gathers takes its
+// rows through xcd_block(): the logical block of this workgroup, chosen so
+// that each XCD runs one contiguous range of logical blocks, so that the
+// rows an XCD holds at once are neighbours and share their gathered lines
+// in its L2 instead of every XCD fetching them.  A bijection of [0,
+// gridDim.x): logical block l computes what workgroup l computed before
+// (same rows, same partial-sum slot), so results do not change.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block() {
+  const int b = blockIdx.x, nb = gridDim.x;
+  const int x = b % kXcds, per = nb / kXcds, rem = nb % kXcds;
+  return x * per + min(x, rem) + b / kXcds;
+}
+
 struct DevState {
   double nb, rho, best_rho, al, beta, nr, bestnr, mu;
   double tol, stag_fac, rho0, bestnr_prev, switch_tol;

@@ -473,11 +473,12 @@ k_spmv_dot_mixed(int n, const int* __restrict__ Lptr, const int* __restrict__ Li
                  const double* __restrict__ Lval, const double* __restrict__ p,
                  double* __restrict__ q, double* __restrict__ part,
                  const DevState* __restrict__ st) {
+  const int lb = xcd_block();
   if (st->stopped) return;
   const bool f64 = st->spmv64 != 0;
   const int lane = threadIdx.x % G;
   const int rows_per_grid = gridDim.x * (kBlock / G);
-  const int first = blockIdx.x * (kBlock / G) + threadIdx.x / G;
+  const int first = lb * (kBlock / G) + threadIdx.x / G;
   double v[1] = {0.0};
   auto emit = [&](int row, double acc) {
     if (lane == 0) {
@@ -493,7 +494,7 @@ k_spmv_dot_mixed(int n, const int* __restrict__ Lptr, const int* __restrict__ Li
                          [&](int i) { return p32[i]; },
                          [&](int row, float acc) { emit(row, (double)acc); });
   block_sum<1>(v);
-  if (threadIdx.x == 0) part[S_PQ * kRedBlocks + blockIdx.x] = v[0];
+  if (threadIdx.x == 0) part[S_PQ * kRedBlocks + lb] = v[0];
 }
 
 // rho of the initial residual; p = z follows as a copy
@@ -523,11 +524,12 @@ k_spmv_dot(int n, const int* __restrict__ Lptr, const int* __restrict__ Lidx,
            const T* __restrict__ Lval, const T* __restrict__ p,
            const double* __restrict__ pd, double* __restrict__ q,
            double* __restrict__ part, const DevState* __restrict__ st) {
+  const int lb = xcd_block();
   if (st->stopped) return;
   const int lane = threadIdx.x % G;
   const int rows_per_grid = gridDim.x * (kBlock / G);
   double v[1] = {0.0};
-  rows_in_flight<G, 4>(blockIdx.x * (kBlock / G) + threadIdx.x / G, rows_per_grid, n, Lptr, Lidx,
+  rows_in_flight<G, 4>(lb * (kBlock / G) + threadIdx.x / G, rows_per_grid, n, Lptr, Lidx,
                        Lval, [&](int i) { return p[i]; }, [&](int row, T acc) {
                          if (lane == 0) {
                            q[row] = (double)acc;
@@ -535,7 +537,7 @@ k_spmv_dot(int n, const int* __restrict__ Lptr, const int* __restrict__ Lidx,
                          }
                        });
   block_sum<1>(v);
-  if (threadIdx.x == 0) part[S_PQ * kRedBlocks + blockIdx.x] = v[0];
+  if (threadIdx.x == 0) part[S_PQ * kRedBlocks + lb] = v[0];
 }
 
 // itcnt += 1; al = rho / (p.q)   (src/pcg.jl:289-304)
@@ -605,11 +607,12 @@ k_resid(int n, int always, const int* __restrict__ Lptr,
         const double* __restrict__ x, const double* __restrict__ r0,
         double* __restrict__ r, double* __restrict__ part,
         const DevState* __restrict__ st) {
+  const int lb = xcd_block();
   if (st->stopped || !(always || st->verify)) return;
   const int lane = threadIdx.x % G;
   const int rows_per_grid = gridDim.x * (kBlock / G);
   double v[1] = {0.0};
-  rows_in_flight<G, 4>(blockIdx.x * (kBlock / G) + threadIdx.x / G, rows_per_grid, n, Lptr, Lidx,
+  rows_in_flight<G, 4>(lb * (kBlock / G) + threadIdx.x / G, rows_per_grid, n, Lptr, Lidx,
                        Lval, [&](int i) { return x[i]; }, [&](int row, double acc) {
                          if (lane == 0) {
                            const double t = r0[row] - acc;
@@ -618,7 +621,7 @@ k_resid(int n, int always, const int* __restrict__ Lptr,
                          }
                        });
   block_sum<1>(v);
-  if (threadIdx.x == 0) part[S_RR * kRedBlocks + blockIdx.x] = v[0];
+  if (threadIdx.x == 0) part[S_RR * kRedBlocks + lb] = v[0];
 }
 
 // After the solve: (b - bmu) - la x in FP64 for the returned x (level
@@ -629,11 +632,12 @@ k_true_resid(int n, const int* __restrict__ Lptr, const int* __restrict__ Lidx,
              const double* __restrict__ Lval, const double* __restrict__ x,
              const double* __restrict__ b, double* __restrict__ part,
              const DevState* __restrict__ st) {
+  const int lb = xcd_block();
   const int lane = threadIdx.x % G;
   const int rows_per_grid = gridDim.x * (kBlock / G);
   const double mu = st->bmu;
   double v[1] = {0.0};
-  rows_in_flight<G, 4>(blockIdx.x * (kBlock / G) + threadIdx.x / G, rows_per_grid, n, Lptr, Lidx,
+  rows_in_flight<G, 4>(lb * (kBlock / G) + threadIdx.x / G, rows_per_grid, n, Lptr, Lidx,
                        Lval, [&](int i) { return x[i]; }, [&](int row, double acc) {
                          if (lane == 0) {
                            const double t = (b[row] - mu) - acc;
@@ -641,7 +645,7 @@ k_true_resid(int n, const int* __restrict__ Lptr, const int* __restrict__ Lidx,
                          }
                        });
   block_sum<1>(v);
-  if (threadIdx.x == 0) part[S_RR * kRedBlocks + blockIdx.x] = v[0];
+  if (threadIdx.x == 0) part[S_RR * kRedBlocks + lb] = v[0];
 }
 
 __global__ void k_scalar_true(const double* __restrict__ part, DevState* st) {

@@ -30,7 +30,8 @@ constexpr int kMRed = 1024;  // fixed grid of the multi-RHS reduction kernels
 // part[(slot * kMRed + block) * K + j].
 template <int K, int NV>
 __device__ __forceinline__ void mblock_sum(double (&a)[NV], double* __restrict__ part,
-                                           const int (&slots)[NV]) {
+                                           const int (&slots)[NV], int blk = -1) {
+  if (blk < 0) blk = blockIdx.x;
   __shared__ double red[NV][kBlock / kWave][K];
   const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
 #pragma unroll
@@ -49,7 +50,7 @@ __device__ __forceinline__ void mblock_sum(double (&a)[NV], double* __restrict__
       double s = 0.0;
 #pragma unroll
       for (int w = 0; w < kBlock / kWave; ++w) s += red[i][w][threadIdx.x];
-      part[((int64_t)slots[i] * kMRed + blockIdx.x) * K + threadIdx.x] = s;
+      part[((int64_t)slots[i] * kMRed + blk) * K + threadIdx.x] = s;
     }
   }
 }
