@@ -3,7 +3,7 @@
 ONE JSON line.
 
 Default (--mode serve): the serving workload, a Poisson stream of solve
-requests at 200 QPS (seed 5) over 32 device-resident graphs with n
+requests at 300 QPS (seed 5) over 32 device-resident graphs with n
 log-uniform in [1e4, 4e6] (2-D / 3-D grids with exp(U(-2,2)) weights, path +
 4 matchings graphs with log-uniform weights), 20 % of the requests bringing
 a new graph of the same families (n up to 2e5) that is factored on the host
@@ -15,10 +15,12 @@ stream) are timed.  `value` is served solves/s over the timed stream, with
 p50/p99 latency beside it, against plain one-at-a-time FP64 GPU PCG on the
 same GPU (a FIFO replay of the same arrivals with the measured plain solve
 time of every resident graph and a fit for the new graphs: modelled).  The
-same line carries the plain FP64 solve of a 3-D 128^3 grid (n = 2,097,152,
-exp(U(-2,2)) weights, seed 2): its per-kernel HBM roofline from HIP events
-on the library's stream and its CPU baseline (one full solve of the C
-restatement on one host core).
+same line carries: the admission time of a new graph with and without the
+factor/plan cache; the precision and batching forms on their workloads (a
+few solves each); and the plain FP64 solve of a 3-D 128^3 grid (n =
+2,097,152, exp(U(-2,2)) weights, seed 2) with its per-kernel HBM roofline
+from HIP events on the library's stream and its CPU baseline (one full solve
+of the C restatement on one host core).
 
 Other modes (same keys):
   plain      the 3-D 128^3 grid solve alone (--steps solves with b resident
@@ -27,7 +29,10 @@ Other modes (same keys):
              (FP32-inner and FP64), next to 256 small unit-weight 2-D grids
              solved by the batched engine at the same time; seed 4.
   precision  union of a path and 4 random matchings, n = 1e6, weights
-             log-uniform in [1e-3, 1e3], seed 3: FP32-inner vs FP64.
+             log-uniform in [1e-3, 1e3], seed 3: FP32-inner vs FP64, with each
+             solve's FP64 true residual as the device reports it.
+  batched    the 256 small grids of the multi workload through the batched
+             engine alone (one launch per step).
 
 --gpus N > 1 exits: this build measures one GPU.
 """
@@ -862,7 +867,7 @@ def main() -> int:
     ap.add_argument("--check", type=int, default=200, help="serve: requests whose true residual is checked")
     ap.add_argument("--rhs", type=int, default=64)
     ap.add_argument("--small", type=int, default=256)
-    ap.add_argument("--qps", type=float, default=200.0)
+    ap.add_argument("--qps", type=float, default=300.0)
     ap.add_argument("--serve-graphs", type=int, default=32)
     ap.add_argument("--serve-min-n", type=float, default=1e4)
     ap.add_argument("--serve-max-n", type=float, default=4e6)
@@ -872,7 +877,7 @@ def main() -> int:
     ap.add_argument("--factor-workers", type=int, default=12, help="serving: host threads factoring new graphs")
     ap.add_argument("--new-wg-max-n", type=int, default=200000,
                     help="serving: largest graph that comes with its request served by the batched engine")
-    ap.add_argument("--wg-max-n", type=int, default=50000,
+    ap.add_argument("--wg-max-n", type=int, default=200000,
                     help="serving: graphs up to this size are solved by the batched engine")
     ap.add_argument("--spin-waits", action="store_true",
                     help="serving: solve workers spin while waiting for the device (default: they sleep)")
