@@ -311,7 +311,7 @@ def serve_leg(args) -> dict:
                  factor_workers=args.factor_workers, batch_window=args.batch_window,
                  max_wait=args.max_wait, quick_workers=args.quick_workers,
                  blocking_waits=not args.spin_waits, wg_max_n=args.wg_max_n, keep_ids=sample,
-                 new_wg_max_n=args.new_wg_max_n)
+                 new_wg_max_n=args.new_wg_max_n, context_precision=args.serve_precision)
     t0 = time.time()
     for i, (g, f) in enumerate(zip(G, F)):
         assert srv.register(i, g, ldli=f)
@@ -875,6 +875,8 @@ def main() -> int:
     ap.add_argument("--new-frac", type=float, default=0.2)
     ap.add_argument("--serve-workers", type=int, default=10, help="serving: graphs solved concurrently")
     ap.add_argument("--factor-workers", type=int, default=12, help="serving: host threads factoring new graphs")
+    ap.add_argument("--serve-precision", default="fp64", choices=["fp64", "fp32"],
+                    help="serving: precision of the context (multi-RHS) solves; both converge on FP64 true residuals")
     ap.add_argument("--new-wg-max-n", type=int, default=200000,
                     help="serving: largest graph that comes with its request served by the batched engine")
     ap.add_argument("--wg-max-n", type=int, default=200000,

@@ -111,12 +111,17 @@ def dense_bytes_estimate(n: int) -> int:
     return 8 * k * ((k + 255) // 256 * 256)
 
 
-def graph_bytes_estimate(n: int, nnz_l: int, nnzf: int) -> int:
+def graph_bytes_estimate(n: int, nnz_l: int, nnzf: int, fp32: bool = False) -> int:
     """Device bytes of a graph for admission: la (12 B/entry), the level-
     ordered factor in both sweep directions (about 16 B/entry each) and 8
     workspace vectors (a context adds its dense tail block,
-    dense_bytes_estimate)."""
-    return 12 * nnz_l + 32 * nnzf + 8 * 8 * n + 16 * n
+    dense_bytes_estimate); a context in the FP32-inner mode adds FP32 copies
+    of la's values and the sweep weights and 20 B per vertex of FP32/FP64
+    work vectors."""
+    b = 12 * nnz_l + 32 * nnzf + 8 * 8 * n + 16 * n
+    if fp32:
+        b += 4 * nnz_l + 8 * nnzf + 20 * n
+    return b
 
 
 def dense_correction(ctx, n: int) -> int:
@@ -124,13 +129,14 @@ def dense_correction(ctx, n: int) -> int:
     return ctx.dense_info()["bytes"] - dense_bytes_estimate(n)
 
 
-def multi_bytes_estimate(n: int, k: int) -> int:
+def multi_bytes_estimate(n: int, k: int, fp32: bool = False) -> int:
     """Workspace of a k-column multi-RHS solve: 9 interleaved n-by-K vectors
-    (K = k rounded up to a power of two, at least 8)."""
+    (K = k rounded up to a power of two, at least 8), plus 20 B per vertex and
+    column in the FP32-inner mode (y, z and p in FP32, r0 in FP64)."""
     K = 8
     while K < k:
         K *= 2
-    return 9 * 8 * n * K
+    return (9 * 8 + (20 if fp32 else 0)) * n * K
 
 
 class Server:
@@ -140,11 +146,18 @@ class Server:
                  max_wait: float = 1.0, new_graph_budget: tuple | None = (8, 2.0, 16384),
                  eager_new: bool = True, quick_workers: int = 4, quick_max_n: int = 300_000,
                  blocking_waits: bool = True, keep_ids: set | None = None,
-                 new_wg_max_n: int | None = 200_000, cache_dir: str | None = None):
+                 new_wg_max_n: int | None = 200_000, cache_dir: str | None = None,
+                 context_precision: str = "fp64"):
         self.device = device
         # factors and context plans of graphs seen before (lapgpu.cache):
         # a graph that comes back is admitted without factoring or planning
         self.cache = GraphCache(cache_dir) if cache_dir else None
+        # contexts (graphs above the batched engine's bound) solve in FP64 or
+        # in the FP32-inner mode; either way a request converges only on its
+        # FP64 true residual, so every answer meets its tol
+        if context_precision not in ("fp64", "fp32"):
+            raise ValueError(f"context_precision must be 'fp64' or 'fp32', not {context_precision!r}")
+        self.context_fp32 = context_precision == "fp32"
         # the solutions kept in the results: all (keep_x) or those of keep_ids
         self.keep_ids = keep_ids
         self.batch = _native.Batch(device)
@@ -246,7 +259,7 @@ class Server:
         (wg) or its own context, evicting idle non-resident graphs when
         needed; None when it does not fit."""
         n = la.shape[0]
-        need = graph_bytes_estimate(n, la.nnz, ldli.nnz)
+        need = graph_bytes_estimate(n, la.nnz, ldli.nnz, fp32=self.context_fp32 and not wg)
         if not wg:
             need += dense_bytes_estimate(n)  # contexts only (not the batched engine)
         if self.hbm_used + need > self.hbm_budget:
@@ -257,10 +270,11 @@ class Server:
         if not wg:
             # the widest multi-RHS workspace that still fits the budget
             k = self.max_rhs
-            while k > 1 and self.hbm_used + need + multi_bytes_estimate(n, k) > self.hbm_budget:
+            fp32 = self.context_fp32
+            while k > 1 and self.hbm_used + need + multi_bytes_estimate(n, k, fp32) > self.hbm_budget:
                 k //= 2
             if k > 1:
-                need += multi_bytes_estimate(n, k)
+                need += multi_bytes_estimate(n, k, fp32)
         return need, k
 
     def _upload_context(self, la, ldli, budget=None, dense_max=None, gkey=None):
@@ -283,6 +297,8 @@ class Server:
             ctx.upload_lap(la)
             t2 = time.perf_counter()
             ctx.upload_ldli(ldli)
+        if self.context_fp32:
+            ctx.set_precision(_native.FP32_INNER, 20)
         t3 = time.perf_counter()
         self.upload_log.append((t3 - t0, t1 - t0, t2 - t1, t3 - t2))
         self.upload_cpu_log.append(time.thread_time() - c0)
