@@ -30,7 +30,8 @@ over time; each is answered by the same solve closure as approxchol_lap's
   budget (`new_graph_budget`): fewer composite rows to build on the host, a
   few more launches on the device for its one solve; and without a dense
   tail block (lapgpu_set_dense_tail), whose matrix costs more to build than
-  one solve saves.
+  one solve saves.  Registered graphs' contexts get a small budget too
+  (`resident_budget`), for their multi-RHS solves.
 
 Every result carries its latency (arrival to completion); `summary` gives
 solves/s and p50/p99 latency.  `cancel(rid)` withdraws a request from any
@@ -144,6 +145,7 @@ class Server:
                  max_launch: int = 512, plain_workers: int = 4, factor_workers: int = 4,
                  keep_x: bool = False, max_rhs: int = 64, batch_window: float = 0.5,
                  max_wait: float = 1.0, new_graph_budget: tuple | None = (8, 2.0, 16384),
+                 resident_budget: tuple | None = (8, 2.0, 16384),
                  eager_new: bool = True, quick_workers: int = 4, quick_max_n: int = 300_000,
                  blocking_waits: bool = True, keep_ids: set | None = None,
                  new_wg_max_n: int | None = 200_000, cache_dir: str | None = None,
@@ -180,6 +182,13 @@ class Server:
         # costs host time in proportion to their entries, and a graph solved
         # once recovers little of it (None: the library's default budget)
         self.new_graph_budget = new_graph_budget
+        # level-group budget of registered graphs' contexts, which serve
+        # multi-RHS solves: every composite-row entry is read once per
+        # column there, so fewer of them beat fewer launches (on a 3.4M-
+        # vertex path + matchings graph 8 / 2 / 16,384 halves the entries,
+        # 208M -> 105M, and its 64-column solve 2.85 -> 1.73 s; the
+        # library's default favours single-RHS launches)
+        self.resident_budget = resident_budget
         self.eager_new = eager_new
         # solve workers sleep while the device works (lapgpu_set_wait_mode)
         # instead of spinning, leaving the cores to factoring and planning
@@ -329,7 +338,7 @@ class Server:
         if wg:
             self._wg_graph(key, n, self._upload_wg(la, ldli), resident)
         else:
-            ctx = self._upload_context(la, ldli, gkey=gkey)
+            ctx = self._upload_context(la, ldli, budget=self.resident_budget, gkey=gkey)
             g = _Graph(key, n, "plain", need + dense_correction(ctx, n), resident, ctx, max_rhs=k)
             self.graphs[key] = g
             self.hbm_used += g.bytes
