@@ -101,13 +101,16 @@ __device__ __forceinline__ void mbcast(int myi, T myw, int src, int& i, T& w) {
 // lanes (one per rhs) of row group `sub` of a wave, in entry order, in T.
 // Indices and weights are loaded K at a time, one per lane, and broadcast
 // within the group, so the gathers of one row are independent loads rather
-// than a chain of index -> value loads.  For K = 64 the row is wave-uniform
-// (k0, k1 must be uniform) and the broadcast is a readlane into scalar
-// registers.
+// than a chain of index -> value loads; the gathers go out U at a time,
+// the last (partial) batch predicated, so a row of up to U entries costs
+// one gather latency (a row of 13-17 entries, the level groups' mean on an
+// expander, used to wait for a batch of 8 and then for each remaining
+// entry in turn).  For K = 64 the row is wave-uniform (k0, k1 must be
+// uniform) and the broadcast is a readlane into scalar registers.
 template <int K, typename T, typename F>
 __device__ __forceinline__ T mrow_dot(int k0, int k1, int j, int sub, const int* __restrict__ idx,
                                       const T* __restrict__ w, F&& src, T acc = 0) {
-  constexpr int U = K == kWave ? 8 : 4;  // gathers in flight per lane
+  constexpr int U = K == kWave ? 16 : 8;  // gathers in flight per lane (U divides K)
   for (int base = k0; base < k1; base += K) {
     const int kk = base + j;
     int myi = 0;
@@ -118,22 +121,17 @@ __device__ __forceinline__ T mrow_dot(int k0, int k1, int j, int sub, const int*
     }
     const int cnt = min(K, k1 - base);
     const int g0 = sub * K;
-    int e = 0;
-    for (; e + U <= cnt; e += U) {
+    for (int e = 0; e < cnt; e += U) {
       int i[U];
       T we[U], v[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) mbcast<K>(myi, myw, g0 + e + u, i[u], we[u]);
 #pragma unroll
-      for (int u = 0; u < U; ++u) v[u] = src(i[u]);
+      for (int u = 0; u < U; ++u)
+        if (e + u < cnt) v[u] = src(i[u]);
 #pragma unroll
-      for (int u = 0; u < U; ++u) acc += we[u] * v[u];
-    }
-    for (; e < cnt; ++e) {
-      int i;
-      T we;
-      mbcast<K>(myi, myw, g0 + e, i, we);
-      acc += we * src(i);
+      for (int u = 0; u < U; ++u)
+        if (e + u < cnt) acc += we[u] * v[u];
     }
   }
   return acc;
@@ -296,6 +294,28 @@ __global__ void k_mscalar_nb(const double* __restrict__ part, DevState* mst, int
 // the S slices are added with shuffles (S = 1: one lane per rhs walks the
 // whole row, indices broadcast, see mrow_dot).  Longer rows are left to
 // k_mgroup_heavy.
+// The last (at most three) entries k, k + S, k + 2S < k1 of a slice, added
+// to acc in that order; their loads issued together.
+template <int S, typename T, typename F>
+__device__ __forceinline__ T slice_tail(int k, int k1, const int* __restrict__ idx,
+                                        const T* __restrict__ w, F&& src, T acc) {
+  int i[3];
+  T ww[3], v[3];
+#pragma unroll
+  for (int u = 0; u < 3; ++u)
+    if (k + u * S < k1) {
+      i[u] = idx[k + u * S];
+      ww[u] = w[k + u * S];
+    }
+#pragma unroll
+  for (int u = 0; u < 3; ++u)
+    if (k + u * S < k1) v[u] = src(i[u]);
+#pragma unroll
+  for (int u = 0; u < 3; ++u)
+    if (k + u * S < k1) acc += ww[u] * v[u];
+  return acc;
+}
+
 constexpr int kMHeavyLen = 64;
 constexpr int kMHeavyThreads = 1024;
 
@@ -334,7 +354,7 @@ k_mgroup(int n, int lo, int hi, const int* __restrict__ ptr, const int* __restri
           acc += w2 * v2;
           acc += w3 * v3;
         }
-        for (; k < k1; k += S) acc += w[k] * src(idx[k]);
+        acc = slice_tail<S>(k, k1, idx, w, src, acc);
       }
     }
 #pragma unroll
@@ -371,7 +391,7 @@ k_mgroup_heavy(int n, const int* __restrict__ heavy, const int* __restrict__ ptr
     acc += w2 * v2;
     acc += w3 * v3;
   }
-  for (; k < k1; k += SL) acc += w[k] * src(idx[k]);
+  acc = slice_tail<SL>(k, k1, idx, w, src, acc);
   red[s][j] = acc;
   __syncthreads();
 #pragma unroll
