@@ -190,6 +190,26 @@ int lapgpu_set_precision(lapgpu_ctx* ctx, int mode, int replace_every);
 #define LAPGPU_WAIT_BLOCKING 1
 int lapgpu_set_wait_mode(lapgpu_ctx* ctx, int mode);
 
+/* How narrow level groups of the single right-hand-side sweeps are launched
+ * (no reference counterpart).  LAPGPU_SWEEP_CHAIN (default): each run of two
+ * or more consecutive groups of at most 1024 rows (env LAPGPU_CHAIN_NARROW)
+ * is one launch, whose workgroups take the groups' rows in sweep order and
+ * wait for each gathered value to be written; LAPGPU_SWEEP_GROUPS: one
+ * launch per group and sweep.  Both give the same bits.  Multi-RHS solves
+ * always launch per group.  A chained run whose wait runs out
+ * (it never should) switches the context to LAPGPU_SWEEP_GROUPS and the call
+ * is run again; LAPGPU_CHAIN_RETRY is returned instead when its input is
+ * gone (a device solve with d_x == d_b).  Env LAPGPU_CHAIN=0 sets
+ * LAPGPU_SWEEP_GROUPS at context creation.  Call it between solves. */
+#define LAPGPU_SWEEP_GROUPS 0
+#define LAPGPU_SWEEP_CHAIN 1
+#define LAPGPU_CHAIN_RETRY (-12)
+int lapgpu_set_sweep_mode(lapgpu_ctx* ctx, int mode);
+
+/* The chained runs of the uploaded schedule: how many runs, and how many
+ * level groups they hold (each sweep; no reference counterpart). */
+int lapgpu_chain_info(lapgpu_ctx* ctx, int64_t* runs, int64_t* chained_groups);
+
 /* Level-group budget of the LDLinv schedule (no reference counterpart; the
  * reference has no device schedule).  The levels before the tail are run as
  * groups, one launch per group and sweep; a group spans at most max_depth

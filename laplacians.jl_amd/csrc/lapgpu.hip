@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "lapgpu.h"
+#include "lapgpu_chain.hpp"
 #include "lapgpu_lock.hpp"
 #include "lapgpu_mem.hpp"
 #include "lapgpu_multi.hpp"
@@ -150,6 +151,25 @@ struct lapgpu_ctx {
   double *gFw = nullptr, *gBw = nullptr;
   float *gFw32 = nullptr, *gBw32 = nullptr, *z32 = nullptr;
   double* zF = nullptr;  // backward output yF (double path)
+  // single-RHS sweeps: runs of narrow level groups in one launch each
+  // (lapgpu_chain.hpp); off (env LAPGPU_CHAIN=0, or after a wait ran out):
+  // one launch per group.  A group is narrow when it has at most
+  // chain_narrow rows; a run is two or more consecutive narrow groups.
+  // group_run: each group's run (-1: a launch of its own); runs[r]: its
+  // first and last group; tkF/tkB: its tickets in either sweep; ctl: the
+  // timeout flag and the runs' ticket counters (forward run r at 1 + r,
+  // backward at 1 + nruns + r); chain_lo/hi: the positions the runs write.
+  // (Multi-RHS sweeps keep one launch per group: chained runs measured
+  // slower there at every threshold.)
+  bool use_chain = true;
+  int chain_narrow = 1024;
+  std::vector<int> group_run;
+  std::vector<std::pair<int, int>> runs;
+  std::vector<std::pair<int, int>> tkF, tkB;  // ticket ranges per run
+  ChainTicket *chainF = nullptr, *chainB = nullptr;
+  int64_t chain_lo = 0, chain_hi = 0;
+  int* chain_ctl = nullptr;
+  int32_t* hchain = nullptr;
   // level-group limits: a level joins the group before it while the group
   // spans at most block_depth levels, its composite rows stay within
   // block_growth x its entries and add at most block_extra entries (each
@@ -398,11 +418,41 @@ DenseArgs<double> dense_args<double>(lapgpu_ctx* c) { return {c->D, c->dvec}; }
 template <>
 DenseArgs<float> dense_args<float>(lapgpu_ctx* c) { return {c->D32, c->dvec32}; }
 
+bool chain_on(const lapgpu_ctx* c) { return c->use_chain && !c->runs.empty() && c->chain_ctl; }
+
+// the sentinel into the runs' output positions (elements [lo K, hi K) of
+// y and z) and the runs' counters to zero, before a pair of sweeps
+template <typename T>
+void launch_chain_prep(lapgpu_ctx* c, int K, T* y, T* z) {
+  const int64_t lo = c->chain_lo * K, hi = c->chain_hi * K;
+  hipLaunchKernelGGL(k_chain_prep<T>, dim3((int)std::min<int64_t>(1024, (hi - lo + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, c->stream, lo, hi, y, z, c->chain_ctl,
+                     (int)(2 * c->runs.size()));
+}
+
+template <typename T, typename TB>
+void launch_chain(lapgpu_ctx* c, const ChainTicket* tk, std::pair<int, int> range, int ctr,
+                  const int* ptr, const int* idx, const T* w, const T* A, const TB* B, T* out) {
+  hipLaunchKernelGGL((k_chain<T, TB>), dim3(std::min(range.second - range.first, kChainGrid)),
+                     dim3(kBlock), 0, c->stream, (int)c->n, tk, range.first, range.second,
+                     c->chain_ctl, ctr, c->heavy, ptr, idx, w, A, B, out);
+}
+
 template <typename T, typename S>
 void launch_sweeps(lapgpu_ctx* c, const double* rin, T* y, T* z, const T* gFw, const T* gBw,
                    const T* sv, const T* Ew, T* ext, const S* fslots, const S* bslots) {
   const Plan& P = c->plan;
-  for (const auto& g : c->groups) {
+  const bool chain = chain_on(c);
+  const int nruns = (int)c->runs.size();
+  if (chain) launch_chain_prep<T>(c, 1, y, z);
+  for (int k = 0; k < (int)c->groups.size(); ++k) {
+    const auto& g = c->groups[k];
+    const int r = chain ? c->group_run[k] : -1;
+    if (r >= 0) {
+      if (k == c->runs[r].first)
+        launch_chain<T, double>(c, c->chainF, c->tkF[r], r, c->gFptr, c->gFidx, gFw, y, rin, y);
+      continue;
+    }
     const int* hv = c->heavy + g.hoff_f;
     LAPGPU_DISPATCH_G(g.Gf, (launch_group<G_, T, double>), c, g.lo, g.hi, hv, g.nhf, c->gFptr,
                       c->gFidx, gFw, y, rin, y);
@@ -448,12 +498,46 @@ void launch_sweeps(lapgpu_ctx* c, const double* rin, T* y, T* z, const T* gFw, c
     (void)hipEventRecord(c->ev[6], c->stream);
   }
   if (c->prof_sweeps) (void)hipEventRecord(c->ev[7], c->stream);
-  for (auto it = c->groups.rbegin(); it != c->groups.rend(); ++it) {
-    const auto& g = *it;
+  for (int k = (int)c->groups.size() - 1; k >= 0; --k) {
+    const auto& g = c->groups[k];
+    const int r = chain ? c->group_run[k] : -1;
+    if (r >= 0) {
+      if (k == c->runs[r].second - 1)
+        launch_chain<T, T>(c, c->chainB, c->tkB[r], nruns + r, c->gBptr, c->gBidx, gBw, z, y, z);
+      continue;
+    }
     const int* hv = c->heavy + g.hoff_b;
     LAPGPU_DISPATCH_G(g.Gb, (launch_group<G_, T, T>), c, g.lo, g.hi, hv, g.nhb, c->gBptr,
                       c->gBidx, gBw, z, y, z);
   }
+}
+
+// Enqueued before the final synchronisation of a solve or apply: the
+// timeout flag to the host.
+int chain_fetch(lapgpu_ctx* c) {
+  c->hchain[0] = 0;
+  if (!c->chain_ctl) return 0;
+  HIPC(c, hipMemcpyAsync(c->hchain, c->chain_ctl, sizeof(int32_t), hipMemcpyDeviceToHost,
+                         c->stream));
+  return 0;
+}
+
+// After that synchronisation: a wait of a chained run ran out (the call's
+// result is not valid).  The context goes back to one launch per level
+// group for good and the call reports LAPGPU_CHAIN_RETRY, which the entry
+// points answer by running it again where they still hold its input.
+constexpr int kChainRetry = LAPGPU_CHAIN_RETRY;
+void drop_graphs(lapgpu_ctx* c);
+int chain_check(lapgpu_ctx* c) {
+  if (c->hchain[0] == 0) return 0;
+  c->hchain[0] = 0;
+  c->use_chain = false;
+  drop_graphs(c);
+  HIPC(c, hipMemsetAsync(c->chain_ctl, 0, sizeof(int), c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return set_err(c, kChainRetry,
+                 "a chained run of level groups timed out waiting for a value; the context now "
+                 "runs one launch per level group: solve again");
 }
 
 template <int G>
@@ -548,6 +632,12 @@ int free_graph(lapgpu_ctx* c) {
   int* ip[] = {c->perm, c->Lptr, c->Lidx, c->lvl, c->gFptr, c->gFidx, c->gBptr, c->gBidx,
                c->heavy, c->mheavy};
   for (int* a : ip) dev_free(c, a);
+  void* cp[] = {c->chainF, c->chainB, c->chain_ctl};
+  for (void* a : cp) dev_free(c, a);
+  c->chainF = c->chainB = nullptr;
+  c->chain_ctl = nullptr;
+  c->runs.clear();
+  c->group_run.clear();
   double* dp[] = {c->Lval, c->s, c->b, c->r, c->x, c->bestx, c->p, c->q, c->y, c->z, c->io,
                   c->gFw, c->gBw, c->zF};
   for (double* a : dp) dev_free(c, a);
@@ -640,6 +730,62 @@ int build_dense(lapgpu_ctx* c) {
   return 0;
 }
 
+// Runs of narrow groups and their single-RHS tickets: per group of a run,
+// in sweep order, its heavy rows four at a time (a wave each), then its
+// light rows in ranges of whole passes of the workgroup (as many passes as
+// keep about kChainTickets tickets per group, at most kChainPasses).
+
+int build_chain(lapgpu_ctx* c) {
+  const int ng = (int)c->groups.size();
+  c->group_run.assign(ng, -1);
+  c->runs.clear();
+  c->tkF.clear();
+  c->tkB.clear();
+  for (int g = 0; g < ng;) {
+    int e = g;
+    while (e < ng && c->groups[e].hi - c->groups[e].lo <= c->chain_narrow) ++e;
+    if (e - g >= 2) {
+      for (int k = g; k < e; ++k) c->group_run[k] = (int)c->runs.size();
+      c->runs.push_back({g, e});
+    }
+    g = std::max(e, g + 1);
+  }
+  c->chain_lo = c->chain_hi = 0;
+  if (c->runs.empty()) return 0;
+  c->chain_lo = c->groups[c->runs.front().first].lo;
+  c->chain_hi = c->groups[c->runs.back().second - 1].hi;
+  std::vector<ChainTicket> tf, tb;
+  auto add = [&](std::vector<ChainTicket>& out, const lapgpu_ctx::GroupLaunch& g, int G, int nh,
+                 int hoff) {
+    constexpr int hpt = kBlock / kWave;
+    for (int h = 0; h < nh; h += hpt) out.push_back({hoff + h, hoff + std::min(nh, h + hpt), 0, 0});
+    const int rpp = kBlock / G;
+    const int64_t passes = ((int64_t)(g.hi - g.lo) + rpp - 1) / rpp;
+    const int per = (int)std::min<int64_t>(kChainPasses, std::max<int64_t>(1, passes / kChainTickets));
+    const int span = per * rpp;
+    for (int lo = g.lo; lo < g.hi; lo += span) out.push_back({lo, std::min(g.hi, lo + span), G, 0});
+  };
+  for (const auto& r : c->runs) {
+    const int f0 = (int)tf.size();
+    for (int k = r.first; k < r.second; ++k) {
+      const auto& g = c->groups[k];
+      add(tf, g, g.Gf, g.nhf, g.hoff_f);
+    }
+    c->tkF.push_back({f0, (int)tf.size()});
+    const int b0 = (int)tb.size();
+    for (int k = r.second - 1; k >= r.first; --k) {
+      const auto& g = c->groups[k];
+      add(tb, g, g.Gb, g.nhb, g.hoff_b);
+    }
+    c->tkB.push_back({b0, (int)tb.size()});
+  }
+  int rc;
+  if ((rc = dev_upload(c, &c->chainF, tf))) return rc;
+  if ((rc = dev_upload(c, &c->chainB, tb))) return rc;
+  std::vector<int> ctl(1 + 2 * c->runs.size(), 0);
+  return dev_upload(c, &c->chain_ctl, ctl);
+}
+
 // The level groups before the tail: composite rows, the per-group lane
 // count and the heavy-row lists of both sweeps.
 int upload_groups(lapgpu_ctx* c, bool blocks_ready) {
@@ -707,7 +853,7 @@ int upload_groups(lapgpu_ctx* c, bool blocks_ready) {
   if ((rc = dev_upload(c, &c->gBw, B.bw))) return rc;
   if ((rc = dev_upload(c, &c->heavy, heavy))) return rc;
   if ((rc = dev_upload(c, &c->mheavy, mheavy))) return rc;
-  return 0;
+  return build_chain(c);
 }
 
 // blocks_ready: P.blocks already holds the level groups (a loaded plan)
@@ -1015,8 +1161,10 @@ int solve_device(lapgpu_ctx* c, const double* d_b, double* d_x, double tol,
   hipLaunchKernelGGL(k_scatter_perm, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, n,
                      c->perm, c->bestx, d_x);
   HIPC(c, hipMemcpyAsync(c->hst, c->st, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
+  if (int rc = chain_fetch(c)) return rc;
   HIPC(c, sync_stream(c));
   HIPC(c, hipGetLastError());
+  if (int rc = chain_check(c)) return rc;
   const DevState& h = *c->hst;
   const double secs =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
@@ -1596,6 +1744,7 @@ int tune_lanes_t(lapgpu_ctx* c, const double* rin, T* y, T* z, const T* gFw, con
     gl.nhb = (int)heavy.size() - gl.hoff_b;
   }
   if ((rc = dev_upload(c, &c->heavy, heavy))) return rc;
+  if ((rc = build_chain(c))) return rc;
   *before_ms += t_before;
   *after_ms += t_after;
   return 0;
@@ -1720,6 +1869,11 @@ int lapgpu_create(lapgpu_ctx** out, int device) {
       return fail("hipEventCreate", e);
   for (auto& ev : c->ev)
     if ((e = hipEventCreate(&ev)) != hipSuccess) return fail("hipEventCreate", e);
+  if ((e = lapgpu::pinned_get((void**)&c->hchain, sizeof(int32_t))) != hipSuccess)
+    return fail("hipHostMalloc", e);
+  c->hchain[0] = 0;
+  if (const char* ch = getenv("LAPGPU_CHAIN")) c->use_chain = atoi(ch) != 0;
+  if (const char* cn = getenv("LAPGPU_CHAIN_NARROW")) c->chain_narrow = atoi(cn);
   if (const char* tw = getenv("LAPGPU_TAIL_STEPS")) c->max_tail_steps = atoll(tw);
   if (const char* ug = getenv("LAPGPU_NO_GRAPH")) c->use_graph = atoi(ug) == 0;
   if (const char* sw = getenv("LAPGPU_SWITCH_TOL")) c->switch_tol = atof(sw);
@@ -1744,6 +1898,7 @@ void lapgpu_destroy(lapgpu_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
   }
   lapgpu::pinned_put(c->hst, sizeof(DevState));
+  lapgpu::pinned_put(c->hchain, sizeof(int32_t));
   lapgpu::pinned_put(c->hflags, sizeof(int32_t) * lapgpu_ctx::kRing);
   for (auto& ev : c->ring_ev)
     if (ev) (void)hipEventDestroy(ev);
@@ -1894,8 +2049,13 @@ int lapgpu_solve_device(lapgpu_ctx* c, const double* d_b, double* d_x,
   if (!d_b || !d_x) return set_err(c, -1, "solve: null vector");
   HIPC(c, hipSetDevice(c->device));
   RunGuard run(c, 1);
-  return solve_device(c, d_b, d_x, tol, maxits, maxtime, stag_test, flags, its_out,
-                      relres_out, stop_out);
+  int rc = solve_device(c, d_b, d_x, tol, maxits, maxtime, stag_test, flags, its_out,
+                        relres_out, stop_out);
+  // b is still there unless x overwrote it
+  if (rc == kChainRetry && d_b != d_x)
+    rc = solve_device(c, d_b, d_x, tol, maxits, maxtime, stag_test, flags, its_out, relres_out,
+                      stop_out);
+  return rc;
 }
 
 int lapgpu_last_true_relres(lapgpu_ctx* c, double* out, int64_t k) {
@@ -1933,9 +2093,12 @@ int lapgpu_solve(lapgpu_ctx* c, const double* b, double* x_out, int64_t n,
   if (!c->has_F) return set_err(c, -2, "solve: upload the Laplacian and LDLinv first");
   HIPC(c, hipSetDevice(c->device));
   RunGuard run(c, 1);
-  HIPC(c, hipMemcpyAsync(c->io, b, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-  int rc = solve_device(c, c->io, c->io, tol, maxits, maxtime, stag_test, flags, its_out,
-                        relres_out, stop_out);
+  int rc = kChainRetry;
+  for (int attempt = 0; attempt < 2 && rc == kChainRetry; ++attempt) {
+    HIPC(c, hipMemcpyAsync(c->io, b, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    rc = solve_device(c, c->io, c->io, tol, maxits, maxtime, stag_test, flags, its_out,
+                      relres_out, stop_out);
+  }
   if (rc) return rc;
   HIPC(c, hipMemcpyAsync(x_out, c->io, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   HIPC(c, sync_stream(c));
@@ -1997,17 +2160,23 @@ int lapgpu_apply_ldli(lapgpu_ctx* c, const double* rin, double* zout, int64_t n)
   if (!c->has_F) return set_err(c, -2, "apply_ldli: upload the Laplacian and LDLinv first");
   if (n != c->n || !rin || !zout) return set_err(c, -1, "apply_ldli: bad arguments");
   HIPC(c, hipSetDevice(c->device));
-  init_state(c, 0.0, 1, 0.0, 0, LAPGPU_NO_CENTER);
-  HIPC(c, hipMemcpyAsync(c->st, c->hst, sizeof(DevState), hipMemcpyHostToDevice, c->stream));
-  HIPC(c, hipMemcpyAsync(c->io, rin, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_gather_perm, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, (int)n,
-                     c->perm, c->io, c->r, c->part);
-  precond(c, c->r);
-  hipLaunchKernelGGL(k_scatter_perm, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, (int)n,
-                     c->perm, c->z, c->io);
-  HIPC(c, hipGetLastError());
-  HIPC(c, hipMemcpyAsync(zout, c->io, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));
+  for (int attempt = 0;; ++attempt) {
+    init_state(c, 0.0, 1, 0.0, 0, LAPGPU_NO_CENTER);
+    HIPC(c, hipMemcpyAsync(c->st, c->hst, sizeof(DevState), hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(c->io, rin, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_gather_perm, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, (int)n,
+                       c->perm, c->io, c->r, c->part);
+    precond(c, c->r);
+    hipLaunchKernelGGL(k_scatter_perm, dim3(kRedBlocks), dim3(kBlock), 0, c->stream, (int)n,
+                       c->perm, c->z, c->io);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipMemcpyAsync(zout, c->io, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    if (int rc = chain_fetch(c)) return rc;
+    HIPC(c, hipStreamSynchronize(c->stream));
+    const int rc = chain_check(c);
+    if (rc == 0) break;
+    if (rc != kChainRetry || attempt > 0) return rc;
+  }
   c->err.clear();
   return 0;
 }
@@ -2048,6 +2217,31 @@ int lapgpu_set_precision(lapgpu_ctx* c, int mode, int replace_every) {
     int rc = prepare_f32(c);
     if (rc) return rc;
   }
+  c->err.clear();
+  return 0;
+}
+
+int lapgpu_set_sweep_mode(lapgpu_ctx* c, int mode) {
+  if (!c) return -1;
+  if (mode != LAPGPU_SWEEP_GROUPS && mode != LAPGPU_SWEEP_CHAIN)
+    return set_err(c, -1, "set_sweep_mode: mode must be LAPGPU_SWEEP_GROUPS or LAPGPU_SWEEP_CHAIN");
+  const bool want = mode == LAPGPU_SWEEP_CHAIN;
+  if (want != c->use_chain) {
+    HIPC(c, hipSetDevice(c->device));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);  // captured iterations hold the other launches
+    c->use_chain = want;
+  }
+  c->err.clear();
+  return 0;
+}
+
+int lapgpu_chain_info(lapgpu_ctx* c, int64_t* runs, int64_t* chained_groups) {
+  if (!c || !runs || !chained_groups) return -1;
+  int64_t g = 0;
+  for (const auto& r : c->runs) g += r.second - r.first;
+  *runs = (int64_t)c->runs.size();
+  *chained_groups = g;
   c->err.clear();
   return 0;
 }

@@ -34,6 +34,8 @@ PROFILE = 4
 EAGER = 8
 WAIT_SPIN = 0
 WAIT_BLOCKING = 1
+SWEEP_GROUPS = 0  # lapgpu_set_sweep_mode: one launch per level group
+SWEEP_CHAIN = 1   # runs of narrow level groups chained in one launch (default)
 TUNE_GROUPS = 1  # lapgpu_autotune: also tune the level-group budget
 FP64 = 0
 FP32_INNER = 1
@@ -52,6 +54,8 @@ EXPORTS = (
     "lapgpu_set_precision",
     "lapgpu_set_group_budget",
     "lapgpu_set_wait_mode",
+    "lapgpu_set_sweep_mode",
+    "lapgpu_chain_info",
     "lapgpu_factor_info",
     "lapgpu_last_timings",
     "lapgpu_last_sweep_timings",
@@ -114,6 +118,8 @@ def load() -> ctypes.CDLL:
     L.lapgpu_set_precision.argtypes = [P, ci, ci]
     L.lapgpu_set_group_budget.argtypes = [P, ci, dbl, i64]
     L.lapgpu_set_wait_mode.argtypes = [P, ci]
+    L.lapgpu_set_sweep_mode.argtypes = [P, ci]
+    L.lapgpu_chain_info.argtypes = [P, pi64, pi64]
     L.lapgpu_factor_info.argtypes = [P, pi64, pi64, pi64, pi64]
     L.lapgpu_last_timings.argtypes = [P, pdbl, pdbl, pdbl, pdbl, pi64]
     L.lapgpu_last_sweep_timings.argtypes = [P, pdbl, pdbl, pdbl, pdbl]
@@ -437,6 +443,18 @@ class Context:
         """Wait for the device by sleeping (blocking=True) or spinning."""
         self._check(self._lib.lapgpu_set_wait_mode(self._h, WAIT_BLOCKING if blocking else WAIT_SPIN),
                     "lapgpu_set_wait_mode")
+
+    def set_sweep_mode(self, chain: bool) -> None:
+        """Runs of narrow level groups chained in one launch each (chain=True,
+        the default) or one launch per group; the results are the same bits."""
+        self._check(self._lib.lapgpu_set_sweep_mode(self._h, SWEEP_CHAIN if chain else SWEEP_GROUPS),
+                    "lapgpu_set_sweep_mode")
+
+    def chain_info(self) -> dict:
+        """Chained runs of narrow level groups in the uploaded schedule."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self._lib.lapgpu_chain_info(self._h, ctypes.byref(a), ctypes.byref(b)), "lapgpu_chain_info")
+        return {"runs": a.value, "chained_groups": b.value}
 
     def factor_info(self) -> dict:
         a, b, c, d = (ctypes.c_int64() for _ in range(4))

@@ -1,0 +1,146 @@
+"""Runs of narrow level groups in one launch (lapgpu_set_sweep_mode): in the
+single right-hand-side sweeps the workgroups of a run take its groups'
+rows in sweep order and wait for each gathered value to be written.  Each
+row is summed as the one-launch-per-group kernels sum it, so both modes
+must give the same bits — one LDLinv apply, whole solves (FP64 and
+FP32-inner; multi-RHS solves, which always launch per group, alongside),
+after the autotuner changed lanes per row and after a regroup, and over
+many iterations (a stale value read by a wait would change them).  Graphs: a path + 4 matchings expander (long
+runs of narrow groups near the tail), a random graph of mean degree 30
+(long composite rows, run one wave or one workgroup each), a 3-D grid, and
+the same graphs with every group counted narrow (LAPGPU_CHAIN_NARROW), so
+that whole sweeps run chained."""
+
+import os
+
+import numpy as np
+import pytest
+
+import lapgpu
+from lapgpu import _native, graphs
+from _bars import assert_relres
+
+pytestmark = pytest.mark.gpu
+
+
+def _rhs(n, rng, k=None):
+    b = rng.standard_normal((n, k) if k else n)
+    return b - b.mean(axis=0)
+
+
+@pytest.fixture(scope="module")
+def cases():
+    rng = np.random.default_rng(71)
+    out = []
+    # the last item: right-hand sides of the multi-RHS solve
+    for name, a, budget, k in (
+        ("pathmatch_250k", graphs.path_matchings(250000, 4, rng, "logU3"), (8, 2.0, 16384), 40),
+        ("dense_40k", graphs.random_dense(40000, 30, rng), None, 20),
+        ("grid3_32", graphs.weighted_grid3(32, 32, 32, rng, "expU2"), (8, 2.0, 16384), 5),
+    ):
+        out.append((name, a, lapgpu.lap(a), lapgpu.build_ldli(a, seed=1), budget, k))
+    return out
+
+
+def _ctx(la, f, budget, chain, prec="fp64", narrow=None):
+    old = os.environ.get("LAPGPU_CHAIN_NARROW")
+    if narrow is not None:
+        os.environ["LAPGPU_CHAIN_NARROW"] = str(narrow)
+    try:
+        ctx = lapgpu.Context(0)
+    finally:
+        if narrow is not None:
+            if old is None:
+                del os.environ["LAPGPU_CHAIN_NARROW"]
+            else:
+                os.environ["LAPGPU_CHAIN_NARROW"] = old
+    if budget:
+        ctx.set_group_budget(*budget)
+    ctx.upload_lap(la)
+    ctx.upload_ldli(f)
+    ctx.set_sweep_mode(chain)
+    if prec == "fp32":
+        ctx.set_precision(_native.FP32_INNER, 20)
+    return ctx
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("narrow", [None, 10**9])
+def test_chain_matches_group_launches_bit_for_bit(cases, prec, narrow):
+    rng = np.random.default_rng(72)
+    for name, a, la, f, budget, k in cases:
+        n = a.shape[0]
+        b = _rhs(n, rng)
+        B = np.asfortranarray(_rhs(n, rng, k))
+        res = {}
+        for chain in (True, False):
+            ctx = _ctx(la, f, budget, chain, prec, narrow)
+            try:
+                info = ctx.chain_info()
+                groups = ctx.group_info()["ngroups"]
+                z = ctx.apply_ldli(b)
+                x, i = ctx.solve(b, tol=1e-10, maxits=1000)
+                x2, _ = ctx.solve(b, tol=1e-10, maxits=1000)  # the captured iteration again
+                X, j = ctx.solve_multi(B, tol=1e-10, maxits=1000)
+                res[chain] = (z, x, i["its"], x2, X, j["its"])
+            finally:
+                ctx.close()
+        if narrow is not None or name == "pathmatch_250k":
+            assert info["runs"] >= 1 and info["chained_groups"] >= 2, (name, info, groups)
+        if narrow is not None:
+            assert info["chained_groups"] == groups, (name, info, groups)
+        z1, x1, its1, x12, X1, jt1 = res[True]
+        z0, x0, its0, x02, X0, jt0 = res[False]
+        assert np.array_equal(z1, z0), f"{name}: apply differs"
+        assert its1 == its0 and np.array_equal(x1, x0), f"{name}: solve differs"
+        assert np.array_equal(x12, x1), f"{name}: repeated solve differs"
+        assert jt1 == jt0 and np.array_equal(X1, X0), f"{name}: multi-RHS solve differs"
+        assert_relres(la, x1, b, 1e-10, f, what=f"{name} {prec}")
+
+
+def test_chain_after_autotune_and_regroup(cases):
+    rng = np.random.default_rng(73)
+    name, a, la, f, budget, _ = cases[0]
+    b = _rhs(a.shape[0], rng)
+    B = np.asfortranarray(_rhs(a.shape[0], rng, 16))
+    ctx = _ctx(la, f, budget, True)
+    try:
+        ctx.autotune(16, groups=True)  # lanes, slices and the group budget
+        assert ctx.chain_info()["runs"] >= 1
+        x1, i1 = ctx.solve(b, tol=1e-10, maxits=1000)
+        X1, j1 = ctx.solve_multi(B, tol=1e-10, maxits=1000)
+        ctx.set_sweep_mode(False)
+        x0, i0 = ctx.solve(b, tol=1e-10, maxits=1000)
+        X0, j0 = ctx.solve_multi(B, tol=1e-10, maxits=1000)
+        ctx.set_sweep_mode(True)
+        ctx.set_group_budget(4, 1.5, 4096)  # regroups the uploaded factor
+        x3, i3 = ctx.solve(b, tol=1e-10, maxits=1000)
+        ctx.set_sweep_mode(False)
+        x4, i4 = ctx.solve(b, tol=1e-10, maxits=1000)
+    finally:
+        ctx.close()
+    assert i1["its"] == i0["its"] and np.array_equal(x1, x0)
+    assert j1["its"] == j0["its"] and np.array_equal(X1, X0)
+    assert i3["its"] == i4["its"] and np.array_equal(x3, x4)
+    assert_relres(la, x1, b, 1e-10, f, what="tuned")
+    assert_relres(la, x3, b, 1e-10, f, what="regrouped")
+
+
+def test_chain_over_many_iterations(cases):
+    """A solve at tol 0 runs until one of pcg's stagnation rules stops it
+    (tens of iterations, at most 400): each iteration runs the chained runs
+    of both sweeps on the same device arrays, so a wait that returned a
+    value of an earlier iteration would show."""
+    rng = np.random.default_rng(74)
+    name, a, la, f, budget, _ = cases[0]
+    b = _rhs(a.shape[0], rng)
+    out = {}
+    for chain in (True, False):
+        ctx = _ctx(la, f, budget, chain)
+        try:
+            out[chain] = ctx.solve(b, tol=0.0, maxits=400, stag_test=0)
+        finally:
+            ctx.close()
+    (x1, i1), (x0, i0) = out[True], out[False]
+    assert i1["its"] == i0["its"] and i1["stop"] == i0["stop"]
+    assert np.array_equal(x1, x0)
