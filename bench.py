@@ -539,7 +539,7 @@ def mode_serve(args) -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.serve_precision == "fp64" else "f64 (multi-RHS context solves: f64 outer / f32 inner)",
         "data": "synthetic",
         "config": {
             "workload": f"serving: {sv['nreq']} requests ({args.steps} steps of {args.step_requests}), Poisson "
@@ -547,6 +547,7 @@ def mode_serve(args) -> int:
                         f"[{args.serve_min_n:g},{args.serve_max_n:g}] (2-D/3-D grids exp(U(-2,2)), path + 4 "
                         f"matchings log-uniform weights), {args.new_frac:.0%} with a new graph of the same "
                         f"families n <= {args.serve_new_max_n:g}, seed 5, tol 1e-8",
+            "context_precision": args.serve_precision,
             "served": s,
             "paths": sv["paths"],
             "true_relres_sample": sv["check"],
@@ -741,6 +742,10 @@ def precision_leg(args) -> dict:
     f = lapgpu.build_ldli(a, seed=1)
     log(f"path+4 matchings n={n} nnzL={la.nnz} nnzF={f.nnz} factor {time.time() - t0:.1f}s")
     ctx = _native.Context(0)
+    # a smaller level-group budget than the default: its many narrow groups
+    # near the tail are run as chained launches, the fastest single-RHS
+    # schedule of this expander (FP64 63 ms against 75 ms per solve)
+    ctx.set_group_budget(16, 4.0, 65536)
     ctx.upload_lap(la)
     ctx.upload_ldli(f)
     db = DeviceArray.from_host(b)
@@ -784,6 +789,8 @@ def precision_leg(args) -> dict:
         "config": {
             "workload": "path + 4 random perfect matchings, n=1e6, weights log-uniform [1e-3,1e3], seed 3, "
                         "1 RHS N(0,1) sum-zero, tol 1e-8, FP64 residual replacement every 20 iterations",
+            "level_group_budget": [16, 4.0, 65536],
+            "chain": ctx.chain_info(),
             "fp64": res["fp64"],
             "fp32_inner": res["fp32_inner"],
             "rel_err_vs_fp64": err,
@@ -882,8 +889,9 @@ def main() -> int:
     ap.add_argument("--new-frac", type=float, default=0.2)
     ap.add_argument("--serve-workers", type=int, default=10, help="serving: graphs solved concurrently")
     ap.add_argument("--factor-workers", type=int, default=12, help="serving: host threads factoring new graphs")
-    ap.add_argument("--serve-precision", default="fp64", choices=["fp64", "fp32"],
-                    help="serving: precision of the context (multi-RHS) solves; both converge on FP64 true residuals")
+    ap.add_argument("--serve-precision", default="fp32", choices=["fp64", "fp32"],
+                    help="serving: precision of the context (multi-RHS) solves (fp32: the FP32-inner mode); "
+                         "both converge on FP64 true residuals")
     ap.add_argument("--new-wg-max-n", type=int, default=200000,
                     help="serving: largest graph that comes with its request served by the batched engine")
     ap.add_argument("--wg-max-n", type=int, default=200000,

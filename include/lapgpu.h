@@ -192,8 +192,9 @@ int lapgpu_set_wait_mode(lapgpu_ctx* ctx, int mode);
 
 /* How narrow level groups of the single right-hand-side sweeps are launched
  * (no reference counterpart).  LAPGPU_SWEEP_CHAIN (default): each run of two
- * or more consecutive groups of at most 1024 rows (env LAPGPU_CHAIN_NARROW)
- * is one launch, whose workgroups take the groups' rows in sweep order and
+ * or more consecutive groups of at most 1024 rows (env LAPGPU_CHAIN_NARROW;
+ * runs of fewer than 40 groups, env LAPGPU_CHAIN_MIN_RUN, stay one launch
+ * per group, which measured faster for them) is one launch, whose workgroups take the groups' rows in sweep order and
  * wait for each gathered value to be written; LAPGPU_SWEEP_GROUPS: one
  * launch per group and sweep.  Both give the same bits.  Multi-RHS solves
  * always launch per group.  A chained run whose wait runs out
@@ -281,8 +282,10 @@ int lapgpu_tail_info(lapgpu_ctx* ctx, int64_t* nt, int64_t* nsteps, int64_t* nsl
  * get a wave each) and, when k > 1, each candidate slices-per-row S of the
  * k-column (rounded up to 8..64) multi-RHS kernels, and keeps the fastest
  * per group and sweep direction; with LAPGPU_TUNE_GROUPS it first tries
- * three level-group budgets (a regrouping each, seconds on large graphs)
- * and keeps the one with the fastest LDLinv apply.  Later solves use the
+ * five level-group budgets (the context's depth and growth with 65536,
+ * 196608 and 524288 extra entries, then 16 / 4 / 65536 and 8 / 2 / 16384,
+ * whose many narrow groups the chained runs take; a regrouping each,
+ * seconds on large graphs) and keeps the one with the fastest LDLinv apply.  Later solves use the
  * tuned schedule.  The lane split changes the order in which a row's terms
  * are summed, so tuned results may differ from untuned ones in the last
  * bits (and, through timing noise, between tuning runs).  times_ms (4

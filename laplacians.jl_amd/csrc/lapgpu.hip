@@ -163,6 +163,10 @@ struct lapgpu_ctx {
   // slower there at every threshold.)
   bool use_chain = true;
   int chain_narrow = 1024;
+  // shorter runs stay one launch per group: measured on 1e6..2e6-vertex
+  // expanders and 3-D grids, runs of 44 or more groups were 6-27 % faster,
+  // runs of 14-34 groups 2-4 % slower
+  int chain_min_run = 40;
   std::vector<int> group_run;
   std::vector<std::pair<int, int>> runs;
   std::vector<std::pair<int, int>> tkF, tkB;  // ticket ranges per run
@@ -744,7 +748,7 @@ int build_chain(lapgpu_ctx* c) {
   for (int g = 0; g < ng;) {
     int e = g;
     while (e < ng && c->groups[e].hi - c->groups[e].lo <= c->chain_narrow) ++e;
-    if (e - g >= 2) {
+    if (e - g >= std::max(2, c->chain_min_run)) {
       for (int k = g; k < e; ++k) c->group_run[k] = (int)c->runs.size();
       c->runs.push_back({g, e});
     }
@@ -1874,6 +1878,7 @@ int lapgpu_create(lapgpu_ctx** out, int device) {
   c->hchain[0] = 0;
   if (const char* ch = getenv("LAPGPU_CHAIN")) c->use_chain = atoi(ch) != 0;
   if (const char* cn = getenv("LAPGPU_CHAIN_NARROW")) c->chain_narrow = atoi(cn);
+  if (const char* cm = getenv("LAPGPU_CHAIN_MIN_RUN")) c->chain_min_run = atoi(cm);
   if (const char* tw = getenv("LAPGPU_TAIL_STEPS")) c->max_tail_steps = atoll(tw);
   if (const char* ug = getenv("LAPGPU_NO_GRAPH")) c->use_graph = atoi(ug) == 0;
   if (const char* sw = getenv("LAPGPU_SWITCH_TOL")) c->switch_tol = atof(sw);
@@ -1930,21 +1935,37 @@ int lapgpu_autotune(lapgpu_ctx* c, int64_t k, int flags, double* times_ms) {
   if (K) LAPGPU_DISPATCH_K(K, t[2] = mapply_time<K_>, c);
   if (flags & LAPGPU_TUNE_GROUPS) {
     // the level-group budget: each candidate regroups and times an apply
-    // (of K right-hand sides when K > 0)
-    const int64_t cand[3] = {65536, 196608, 524288};
+    // (of K right-hand sides when K > 0): the context's depth and growth
+    // with three extra-entry limits, and two small budgets whose many
+    // narrow groups the chained runs take (the fastest single-RHS
+    // schedules of expanders)
+    struct Budget {
+      int depth;
+      double growth;
+      int64_t extra;
+    };
+    const Budget cand[5] = {{c->block_depth, c->block_growth, 65536},
+                            {c->block_depth, c->block_growth, 196608},
+                            {c->block_depth, c->block_growth, 524288},
+                            {16, 4.0, 65536},
+                            {8, 2.0, 16384}};
     float best = 0.0f;
-    int64_t bestx = c->block_extra;
-    for (int i = 0; i < 3; ++i) {
-      c->block_extra = cand[i];
+    Budget pick = {c->block_depth, c->block_growth, c->block_extra};
+    for (int i = 0; i < 5; ++i) {
+      c->block_depth = cand[i].depth;
+      c->block_growth = cand[i].growth;
+      c->block_extra = cand[i].extra;
       if ((rc = regroup(c))) return rc;
       float tt = apply_time(c);
       if (K) LAPGPU_DISPATCH_K(K, tt = mapply_time<K_>, c);
       if (i == 0 || tt < best) {
         best = tt;
-        bestx = cand[i];
+        pick = cand[i];
       }
     }
-    c->block_extra = bestx;
+    c->block_depth = pick.depth;
+    c->block_growth = pick.growth;
+    c->block_extra = pick.extra;
     if ((rc = regroup(c))) return rc;
   }
   double b = 0.0, a = 0.0;

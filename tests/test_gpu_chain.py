@@ -43,17 +43,20 @@ def cases():
 
 
 def _ctx(la, f, budget, chain, prec="fp64", narrow=None):
-    old = os.environ.get("LAPGPU_CHAIN_NARROW")
+    # runs of any length (the library chains only long ones by default)
+    env = {"LAPGPU_CHAIN_MIN_RUN": "2"}
     if narrow is not None:
-        os.environ["LAPGPU_CHAIN_NARROW"] = str(narrow)
+        env["LAPGPU_CHAIN_NARROW"] = str(narrow)
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
     try:
         ctx = lapgpu.Context(0)
     finally:
-        if narrow is not None:
-            if old is None:
-                del os.environ["LAPGPU_CHAIN_NARROW"]
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
             else:
-                os.environ["LAPGPU_CHAIN_NARROW"] = old
+                os.environ[k] = v
     if budget:
         ctx.set_group_budget(*budget)
     ctx.upload_lap(la)
@@ -98,6 +101,22 @@ def test_chain_matches_group_launches_bit_for_bit(cases, prec, narrow):
         assert_relres(la, x1, b, 1e-10, f, what=f"{name} {prec}")
 
 
+def _both_modes(ctx, b, B=None):
+    """Solve in chained mode, then one launch per group, on one context."""
+    out = []
+    for chain in (True, False):
+        ctx.set_sweep_mode(chain)
+        x, i = ctx.solve(b, tol=1e-10, maxits=1000)
+        X, j = ctx.solve_multi(B, tol=1e-10, maxits=1000) if B is not None else (None, {"its": None})
+        out.append((x, i["its"], X, j["its"]))
+    ctx.set_sweep_mode(True)
+    (x1, i1, X1, j1), (x0, i0, X0, j0) = out
+    assert i1 == i0 and np.array_equal(x1, x0)
+    if B is not None:
+        assert j1 == j0 and np.array_equal(X1, X0)
+    return x1
+
+
 def test_chain_after_autotune_and_regroup(cases):
     rng = np.random.default_rng(73)
     name, a, la, f, budget, _ = cases[0]
@@ -105,25 +124,16 @@ def test_chain_after_autotune_and_regroup(cases):
     B = np.asfortranarray(_rhs(a.shape[0], rng, 16))
     ctx = _ctx(la, f, budget, True)
     try:
-        ctx.autotune(16, groups=True)  # lanes, slices and the group budget
+        ctx.autotune(16)  # lanes and slices per row: the tickets follow
         assert ctx.chain_info()["runs"] >= 1
-        x1, i1 = ctx.solve(b, tol=1e-10, maxits=1000)
-        X1, j1 = ctx.solve_multi(B, tol=1e-10, maxits=1000)
-        ctx.set_sweep_mode(False)
-        x0, i0 = ctx.solve(b, tol=1e-10, maxits=1000)
-        X0, j0 = ctx.solve_multi(B, tol=1e-10, maxits=1000)
-        ctx.set_sweep_mode(True)
+        assert_relres(la, _both_modes(ctx, b, B), b, 1e-10, f, what="tuned")
         ctx.set_group_budget(4, 1.5, 4096)  # regroups the uploaded factor
-        x3, i3 = ctx.solve(b, tol=1e-10, maxits=1000)
-        ctx.set_sweep_mode(False)
-        x4, i4 = ctx.solve(b, tol=1e-10, maxits=1000)
+        assert ctx.chain_info()["runs"] >= 1
+        assert_relres(la, _both_modes(ctx, b), b, 1e-10, f, what="regrouped")
+        ctx.autotune(0, groups=True)  # whichever budget it keeps
+        assert_relres(la, _both_modes(ctx, b), b, 1e-10, f, what="budget tuned")
     finally:
         ctx.close()
-    assert i1["its"] == i0["its"] and np.array_equal(x1, x0)
-    assert j1["its"] == j0["its"] and np.array_equal(X1, X0)
-    assert i3["its"] == i4["its"] and np.array_equal(x3, x4)
-    assert_relres(la, x1, b, 1e-10, f, what="tuned")
-    assert_relres(la, x3, b, 1e-10, f, what="regrouped")
 
 
 def test_chain_over_many_iterations(cases):
