@@ -137,6 +137,9 @@ def plain_leg(args, nsolves: int, warmup: int, cpu: bool) -> dict:
     f = lapgpu.build_ldli(a, seed=1)
     t_factor = time.time() - t0
     ctx = _native.Context(0)
+    budget = getattr(args, "budget", None)
+    if budget:
+        ctx.set_group_budget(*budget)
     t0 = time.time()
     ctx.upload_lap(la)
     ctx.upload_ldli(f)
@@ -149,7 +152,7 @@ def plain_leg(args, nsolves: int, warmup: int, cpu: bool) -> dict:
     info, ti, gi = ctx.factor_info(), ctx.tail_info(), ctx.group_info()
     se, di = ctx.sweep_entries(), ctx.dense_info()
     log(f"plain: factor nnzF={f.nnz} in {t_factor:.1f}s, upload+plan {t_upload:.1f}s; schedule {info} tail {ti} "
-        f"groups {gi} dense {di}")
+        f"groups {gi} chained {ctx.chain_info()} dense {di}")
     db = DeviceArray.from_host(b)
     dx = DeviceArray(n)
     synchronize()
@@ -245,6 +248,7 @@ def plain_leg(args, nsolves: int, warmup: int, cpu: bool) -> dict:
         "true_relres": relres,
         "n": n, "nnz_L": int(la.nnz), "nnzF": int(f.nnz),
         "levels": info["nlevels"], "level_groups": info["nwide"], "tail_vertices": NT,
+        "level_group_budget": list(budget) if budget else "default", "chained": ctx.chain_info(),
         "dense_block_vertices": K, "tail_sparse_steps": ti["nsteps"],
         "per_iteration_ms": {"spmv": tm["spmv_ms"], "ldl_apply": tm["ldl_ms"], "vectors": tm["vec_ms"], **st, **dt},
         "factor_host_s": t_factor, "upload_plan_s": t_upload, "autotune": tune,
@@ -903,6 +907,8 @@ def main() -> int:
     ap.add_argument("--batch-window", type=float, default=2.0, help="serving: wait for more requests, x last solve time")
     ap.add_argument("--max-wait", type=float, default=3.0, help="serving: longest wait for more requests (s)")
     ap.add_argument("--hw-queues", type=int, default=16, help="serving: HIP hardware queues (0: HIP's default)")
+    ap.add_argument("--budget", type=lambda v: (int(v.split(",")[0]), float(v.split(",")[1]), int(v.split(",")[2])),
+                    default=None, help="plain: level-group budget depth,growth,extra (default: the library's)")
     ap.add_argument("--autotune", action="store_true",
                     help="plain: tune the schedule on the graph first (lapgpu_autotune)")
     ap.add_argument("--no-autotune", action="store_true",
