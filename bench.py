@@ -311,15 +311,21 @@ def serve_leg(args) -> dict:
     log(f"serve: resident graphs factored in {time.time() - t0:.1f}s")
     # solutions kept for the true-residual check: an even sample of the stream
     sample = set(r.rid for r in reqs[:: max(1, nreq // args.check)])
-    srv = Server(0, hbm_budget=200 * 2**30, plain_workers=args.serve_workers,
+    srv = Server(0, hbm_budget=args.hbm_gib * 2**30, plain_workers=args.serve_workers,
                  factor_workers=args.factor_workers, batch_window=args.batch_window,
                  max_wait=args.max_wait, quick_workers=args.quick_workers,
                  blocking_waits=not args.spin_waits, wg_max_n=args.wg_max_n, keep_ids=sample,
-                 new_wg_max_n=args.new_wg_max_n, context_precision=args.serve_precision)
+                 new_wg_max_n=args.new_wg_max_n, context_precision=args.serve_precision,
+                 exact_batches=args.exact_batches, resident_budget=args.resident_budget)
     t0 = time.time()
     for i, (g, f) in enumerate(zip(G, F)):
         assert srv.register(i, g, ldli=f)
     log(f"serve: resident graphs uploaded in {time.time() - t0:.1f}s, HBM reserved {srv.hbm_used / 2**30:.1f} GiB")
+    if args.replica_nnzf > 0:
+        t0 = time.time()
+        got = srv.add_replicas(args.replica_nnzf)
+        log(f"serve: second contexts for graphs {got} (LDLinv >= {args.replica_nnzf} entries) in "
+            f"{time.time() - t0:.1f}s, HBM reserved {srv.hbm_used / 2**30:.1f} GiB")
     # warm-up: one solve per resident graph (captures, workspaces), then the
     # warm-up stream
     wrng = np.random.default_rng(55)
@@ -350,6 +356,17 @@ def serve_leg(args) -> dict:
         lat = np.array([o.latency for _, o in sel])
         log(f"  path {pth}: {len(sel)} ({sum(r.new_graph is not None for r, _ in sel)} new graphs), latency p50 "
             f"{np.percentile(lat, 50):.2f} s p99 {np.percentile(lat, 99):.2f} s")
+    # the registered graphs whose requests waited longest: graph, n,
+    # requests, latency p50 / max, and when its last answer came
+    per = {}
+    for r, o in zip(reqs, res):
+        if r.graph is not None:
+            per.setdefault(r.graph, []).append((o.latency, r.arrival + o.latency))
+    worst = sorted(per.items(), key=lambda kv: -max(lt for lt, _ in kv[1]))[:4]
+    for gi, v in worst:
+        lat = np.array([lt for lt, _ in v])
+        log(f"  graph {gi} n={G[gi].shape[0]}: {len(v)} requests, latency p50 {np.median(lat):.2f} s max "
+            f"{lat.max():.2f} s, last answer at {max(e for _, e in v):.1f} s")
     nf = np.array(srv.factor_log)
     up = np.array(srv.upload_log).reshape(-1, 4)
     wgu = np.array(srv.wg_upload_log).reshape(-1, 2)
@@ -369,8 +386,10 @@ def serve_leg(args) -> dict:
     # columns, and the worker-seconds they held
     sl = np.array([(n, k, dt) for n, k, dt, _, _ in srv.solve_log]).reshape(-1, 3)
     if len(sl):
+        padded = sum(max(8, 1 << (int(k) - 1).bit_length()) for k in sl[:, 1] if k > 1)
         log(f"serve: context solves {len(sl)}, columns mean {sl[:, 1].mean():.1f} max {int(sl[:, 1].max())}, "
-            f"worker-seconds {sl[:, 2].sum():.0f} over {wall:.1f} s")
+            f"worker-seconds {sl[:, 2].sum():.0f} over {wall:.1f} s; multi-RHS columns "
+            f"{int(sl[sl[:, 1] > 1, 1].sum())} padded to {padded}")
         for lo, hi in ((5e4, 3e5), (3e5, 1e6), (1e6, 5e6)):
             m = (sl[:, 0] > lo) & (sl[:, 0] <= hi)
             if m.any():
@@ -893,6 +912,15 @@ def main() -> int:
     ap.add_argument("--new-frac", type=float, default=0.2)
     ap.add_argument("--serve-workers", type=int, default=10, help="serving: graphs solved concurrently")
     ap.add_argument("--factor-workers", type=int, default=12, help="serving: host threads factoring new graphs")
+    ap.add_argument("--resident-budget", type=lambda v: tuple(t(x) for t, x in zip((int, float, int), v.split(","))),
+                    default=(16, 4.0, 65536),
+                    help="serving: level-group budget depth,growth,extra of the registered graphs' contexts")
+    ap.add_argument("--hbm-gib", type=float, default=225, help="serving: HBM budget of graphs and workspaces (GiB)")
+    ap.add_argument("--replica-nnzf", type=int, default=60_000_000,
+                    help="serving: registered graphs whose LDLinv has at least this many entries get a second "
+                         "context (0: none)")
+    ap.add_argument("--exact-batches", action="store_true",
+                    help="serving: multi-RHS batches of exactly 8/16/32/64 requests (no padding columns)")
     ap.add_argument("--serve-precision", default="fp32", choices=["fp64", "fp32"],
                     help="serving: precision of the context (multi-RHS) solves (fp32: the FP32-inner mode); "
                          "both converge on FP64 true residuals")

@@ -94,6 +94,12 @@ class _Graph:
     resident: bool
     handle: object = None  # batch graph id or Context
     busy: int = 0
+    # replicas: contexts holding the same graph (handle is the first), so
+    # that as many multi-RHS solves of it run at once; idle: those free;
+    # source: (la, LDLinv, cache key) of a registered graph, for add_replicas
+    replicas: list = field(default_factory=list)
+    idle: list = field(default_factory=list)
+    source: tuple | None = None
     pending: deque = field(default_factory=deque)
     max_rhs: int = 1  # columns per multi-RHS solve on this graph
     last_solve_s: float = 0.0  # duration of the graph's last solve
@@ -145,11 +151,12 @@ class Server:
                  max_launch: int = 512, plain_workers: int = 4, factor_workers: int = 4,
                  keep_x: bool = False, max_rhs: int = 64, batch_window: float = 0.5,
                  max_wait: float = 1.0, new_graph_budget: tuple | None = (8, 2.0, 16384),
-                 resident_budget: tuple | None = (8, 2.0, 16384),
+                 resident_budget: tuple | None = (16, 4.0, 65536),
                  eager_new: bool = True, quick_workers: int = 4, quick_max_n: int = 300_000,
                  blocking_waits: bool = True, keep_ids: set | None = None,
                  new_wg_max_n: int | None = 200_000, cache_dir: str | None = None,
-                 context_precision: str = "fp64"):
+                 context_precision: str = "fp64", exact_batches: bool = False,
+                 replica_batch: int = 16):
         self.device = device
         # factors and context plans of graphs seen before (lapgpu.cache):
         # a graph that comes back is admitted without factoring or planning
@@ -184,12 +191,24 @@ class Server:
         self.new_graph_budget = new_graph_budget
         # level-group budget of registered graphs' contexts, which serve
         # multi-RHS solves: every composite-row entry is read once per
-        # column there, so fewer of them beat fewer launches (on a 3.4M-
-        # vertex path + matchings graph 8 / 2 / 16,384 halves the entries,
-        # 208M -> 105M, and its 64-column solve 2.85 -> 1.73 s; the
-        # library's default favours single-RHS launches)
+        # column there, so fewer of them pay (a 3.4M-vertex path + matchings
+        # graph's 64-column solve alone: 2.85 s with the library's default
+        # budget, 1.73 s with 8 / 2 / 16,384), but beside many other solves
+        # each dependent launch also waits its turn on the device, so fewer
+        # launches pay too: 16 / 4 / 65,536 served a mixed stream fastest
+        # (236 solves/s against 221 with 8 / 2 / 16,384 and 159 with the
+        # library's default)
         self.resident_budget = resident_budget
         self.eager_new = eager_new
+        # multi-RHS batches of exactly 8, 16, 32 or 64 requests (K, the
+        # columns every launch of the solve processes, padding included),
+        # the rest waiting for the graph's next solve: on a saturated device
+        # a padding column costs as much as a request
+        self.exact_batches = exact_batches
+        # a graph with replicas (add_replicas) starts a second solve while
+        # one runs once replica_batch requests are queued
+        self.replica_batch = replica_batch
+        self._take_lock = threading.Lock()
         # solve workers sleep while the device works (lapgpu_set_wait_mode)
         # instead of spinning, leaving the cores to factoring and planning
         self.blocking_waits = blocking_waits
@@ -340,9 +359,36 @@ class Server:
         else:
             ctx = self._upload_context(la, ldli, budget=self.resident_budget, gkey=gkey)
             g = _Graph(key, n, "plain", need + dense_correction(ctx, n), resident, ctx, max_rhs=k)
+            g.replicas = [ctx]
+            g.idle = [ctx]
+            if resident:
+                g.source = (la, ldli, gkey)
             self.graphs[key] = g
             self.hbm_used += g.bytes
         return True
+
+    def add_replicas(self, min_nnzf: int, copies: int = 2) -> list:
+        """Give registered graphs on contexts whose LDLinv has at least
+        min_nnzf entries up to `copies` contexts each, largest factors first,
+        while the HBM budget allows (call between runs, after registering).
+        Their multi-RHS solves are the longest: one context (one solve at a
+        time) falls behind once requests arrive faster than a solve of the
+        batch they form completes, and a second context lets two such solves
+        run at once.  Returns the keys that got a replica."""
+        got = []
+        cands = [g for g in self.graphs.values()
+                 if g.kind == "plain" and g.resident and g.source is not None and g.source[1].nnz >= min_nnzf]
+        for g in sorted(cands, key=lambda g: -g.source[1].nnz):
+            per = g.bytes // len(g.replicas)
+            while len(g.replicas) < copies and self.hbm_used + per <= self.hbm_budget:
+                la, ldli, gkey = g.source
+                ctx = self._upload_context(la, ldli, budget=self.resident_budget, gkey=gkey)
+                g.replicas.append(ctx)
+                g.idle.append(ctx)
+                g.bytes += per
+                self.hbm_used += per
+                got.append(g.key)
+        return got
 
     def _evict(self, need: int) -> None:
         for key in list(self.graphs):
@@ -357,8 +403,9 @@ class Server:
         if g.kind == "wg":
             self.batch.remove_graph(g.handle)
         else:
-            # closing waits for the device's frees; the context is idle
-            self._close_pool.submit(g.handle.close)
+            # closing waits for the device's frees; the contexts are idle
+            for ctx in g.replicas or [g.handle]:
+                self._close_pool.submit(ctx.close)
         self.hbm_used -= g.bytes
         del self.graphs[g.key]
 
@@ -411,22 +458,32 @@ class Server:
                 g.pending.append(r)
                 pump_plain(g)
 
-        def solve_queued(g: _Graph):
-            # runs in a worker: takes everything queued for the graph NOW
-            # (not when the task was submitted, so requests that arrive while
-            # the task waits for a worker join it), with the same tol and
-            # maxits as the oldest, up to max_rhs columns
-            take = [g.pending.popleft()] if g.pending else []
-            while (g.pending and len(take) < g.max_rhs and g.pending[0].tol == take[0].tol
-                   and g.pending[0].maxits == take[0].maxits):
-                take.append(g.pending.popleft())
+        def solve_queued(g: _Graph, ctx):
+            # runs in a worker on one of the graph's contexts: takes
+            # everything queued for the graph NOW (not when the task was
+            # submitted, so requests that arrive while the task waits for a
+            # worker join it), with the same tol and maxits as the oldest, up
+            # to max_rhs columns
+            with self._take_lock:  # replicas' workers take from one queue
+                take = [g.pending.popleft()] if g.pending else []
+                while (g.pending and len(take) < g.max_rhs and g.pending[0].tol == take[0].tol
+                       and g.pending[0].maxits == take[0].maxits):
+                    take.append(g.pending.popleft())
+                if self.exact_batches and len(take) > 8:
+                    # a multi-RHS solve runs K = a power of two columns: take
+                    # that many and leave the rest, first in line, for the
+                    # next solve
+                    m = 1 << (len(take).bit_length() - 1)
+                    for r in reversed(take[m:]):
+                        g.pending.appendleft(r)
+                    take = take[:m]
             # requests cancelled while queued are answered without a solve
             gone = [r for r in take if self._is_cancelled(r.rid)]
             take = [r for r in take if not self._is_cancelled(r.rid)]
             if not take:
                 return take, gone, None, None
             for j, r in enumerate(take):
-                self._running[r.rid] = (g, j)
+                self._running[r.rid] = (ctx, j)
             r0 = take[0]
             # a graph that came with its request is solved without capturing
             # its iteration as a HIP graph (the capture would be used once)
@@ -434,10 +491,10 @@ class Server:
             c_start = time.thread_time()
             t_start = time.perf_counter()
             if len(take) == 1:
-                x, info = g.handle.solve(r0.b, r0.tol, r0.maxits, float("inf"), 5, flags)
+                x, info = ctx.solve(r0.b, r0.tol, r0.maxits, float("inf"), 5, flags)
             else:
                 B = np.stack([r.b for r in take], axis=1)
-                x, info = g.handle.solve_multi(B, r0.tol, r0.maxits, float("inf"), 5, flags)
+                x, info = ctx.solve_multi(B, r0.tol, r0.maxits, float("inf"), 5, flags)
             dt = time.perf_counter() - t_start
             for r in take:
                 self._running.pop(r.rid, None)
@@ -447,19 +504,27 @@ class Server:
             return take, gone, x, info
 
         def pump_plain(g: _Graph):
-            # one solve at a time per context (a context owns one stream)
+            # one solve at a time per context (a context owns one stream); a
+            # graph with replicas starts another solve while one runs once
+            # replica_batch requests are queued
             if not g.pending:
                 waiting.discard(g.key)
-            if g.busy or not g.pending:
+            idle = g.idle if g.replicas else ([g.handle] if not g.busy else [])
+            if not idle or not g.pending:
                 return
-            if len(g.pending) < g.max_rhs and time.perf_counter() < g.ready_at:
+            if g.busy:
+                if len(g.pending) < min(g.max_rhs, self.replica_batch):
+                    return
+            elif len(g.pending) < g.max_rhs and time.perf_counter() < g.ready_at:
                 waiting.add(g.key)
                 return
             waiting.discard(g.key)
-            g.busy = 1
+            ctx = idle.pop(0) if g.replicas else g.handle
+            g.busy += 1
+            self.stats["max_busy_contexts"] = max(self.stats.get("max_busy_contexts", 0), g.busy)
             quick = self._quick_pool is not None and g.n <= self.quick_max_n
             pool = self._quick_pool if quick else self._plain_pool
-            plain_futs[pool.submit(solve_queued, g)] = g
+            plain_futs[pool.submit(solve_queued, g, ctx)] = (g, ctx)
 
         while len(results) < total:
             now = time.perf_counter() - t0
@@ -542,9 +607,11 @@ class Server:
                 progressed = True
             # finished plain solves
             for fut in [f for f in plain_futs if f.done()]:
-                g = plain_futs.pop(fut)
+                g, ctx = plain_futs.pop(fut)
                 take, gone, x, info = fut.result()
-                g.busy = 0
+                g.busy -= 1
+                if g.replicas:
+                    g.idle.append(ctx)
                 g.ready_at = 0.0  # requests queued meanwhile go at once
                 for r in gone:
                     finish_cancelled(r, "queued")
@@ -568,9 +635,9 @@ class Server:
                     self.batch.cancel(h, i)
                     done_now = True
                 elif not done_now and rid in self._running:
-                    g, j = self._running.get(rid, (None, 0))
+                    ctx, j = self._running.get(rid, (None, 0))
                     # False: the solve has not started or has just ended
-                    done_now = g is not None and g.handle.cancel(j)
+                    done_now = ctx is not None and ctx.cancel(j)
                 elif not done_now and rid in placed:
                     g = placed[rid]
                     if g.kind == "wg":
@@ -651,7 +718,8 @@ class Server:
         self._close_pool.shutdown(wait=True)
         for g in list(self.graphs.values()):
             if g.kind == "plain":
-                g.handle.close()
+                for ctx in g.replicas or [g.handle]:
+                    ctx.close()
         self.graphs.clear()
         self.batch.close()
 

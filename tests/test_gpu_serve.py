@@ -170,3 +170,31 @@ def test_serve_new_graphs_take_both_engines():
         srv.batch.remove_graph(g)
     finally:
         srv.close()
+
+
+def test_serve_replica_contexts():
+    """A registered graph with a large factor gets a second context
+    (add_replicas): a burst of requests for it is served by two multi-RHS
+    solves at once, each answer at the bar and, like any multi-RHS answer,
+    the same as the request alone to the stated tolerance."""
+    rng = np.random.default_rng(21)
+    a = graphs.path_matchings(30000, 4, rng)
+    srv = Server(0, wg_max_n=1000, keep_x=True, replica_batch=4, max_rhs=8, batch_window=0.0, max_wait=0.0)
+    try:
+        assert srv.register("pm", a)
+        assert srv.add_replicas(1) == ["pm"]
+        g = srv.graphs["pm"]
+        assert len(g.replicas) == 2 and g.kind == "plain"
+        reqs = [Request(rid, 0.0, _rhs(a.shape[0], rng), graph="pm", tol=1e-8) for rid in range(40)]
+        res = srv.run(reqs, realtime=False)
+        la = lapgpu.lap(a)
+        for r, out in zip(reqs, res):
+            assert out.stop == "converged", out
+            assert_relres(la, out.x, r.b, 1e-8, lambda: lapgpu.build_ldli(a, seed=1))
+        # two solves of the graph ran at once, one on each context
+        assert srv.stats["max_busy_contexts"] == 2
+        alone = srv.run([Request(100, 0.0, reqs[7].b, graph="pm", tol=1e-8)])[0]
+        assert alone.stop == res[7].stop and abs(alone.its - res[7].its) <= 2
+        assert np.linalg.norm(alone.x - res[7].x) <= 1e-6 * np.linalg.norm(res[7].x)
+    finally:
+        srv.close()
