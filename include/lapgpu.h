@@ -207,9 +207,12 @@ int lapgpu_set_wait_mode(lapgpu_ctx* ctx, int mode);
 #define LAPGPU_CHAIN_RETRY (-12)
 int lapgpu_set_sweep_mode(lapgpu_ctx* ctx, int mode);
 
-/* The chained runs of the uploaded schedule: how many runs, and how many
- * level groups they hold (each sweep; no reference counterpart). */
-int lapgpu_chain_info(lapgpu_ctx* ctx, int64_t* runs, int64_t* chained_groups);
+/* The chained runs of the uploaded schedule: how many runs, how many level
+ * groups they hold (each sweep), and whether solves use them (active: 1;
+ * 0 in LAPGPU_SWEEP_GROUPS mode or after a wait ran out).  No reference
+ * counterpart.  Env LAPGPU_CHAIN_POLL_MAX (default 262144) bounds the polls
+ * of one wait, for tests of the fallback. */
+int lapgpu_chain_info(lapgpu_ctx* ctx, int64_t* runs, int64_t* chained_groups, int* active);
 
 /* Level-group budget of the LDLinv schedule (no reference counterpart; the
  * reference has no device schedule).  The levels before the tail are run as

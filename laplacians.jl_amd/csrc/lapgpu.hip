@@ -157,8 +157,9 @@ struct lapgpu_ctx {
   // chain_narrow rows; a run is two or more consecutive narrow groups.
   // group_run: each group's run (-1: a launch of its own); runs[r]: its
   // first and last group; tkF/tkB: its tickets in either sweep; ctl: the
-  // timeout flag and the runs' ticket counters (forward run r at 1 + r,
-  // backward at 1 + nruns + r); chain_lo/hi: the positions the runs write.
+  // timeout flag, the poll limit of a wait (chain_poll_max) and the runs'
+  // ticket counters (forward run r at 2 + r, backward at 2 + nruns + r);
+  // chain_lo/hi: the positions the runs write.
   // (Multi-RHS sweeps keep one launch per group: chained runs measured
   // slower there at every threshold.)
   bool use_chain = true;
@@ -167,6 +168,7 @@ struct lapgpu_ctx {
   // expanders and 3-D grids, runs of 44 or more groups were 6-27 % faster,
   // runs of 14-34 groups 2-4 % slower
   int chain_min_run = 40;
+  int chain_poll_max = kChainPollMax;
   std::vector<int> group_run;
   std::vector<std::pair<int, int>> runs;
   std::vector<std::pair<int, int>> tkF, tkB;  // ticket ranges per run
@@ -786,7 +788,8 @@ int build_chain(lapgpu_ctx* c) {
   int rc;
   if ((rc = dev_upload(c, &c->chainF, tf))) return rc;
   if ((rc = dev_upload(c, &c->chainB, tb))) return rc;
-  std::vector<int> ctl(1 + 2 * c->runs.size(), 0);
+  std::vector<int> ctl(2 + 2 * c->runs.size(), 0);
+  ctl[1] = c->chain_poll_max;
   return dev_upload(c, &c->chain_ctl, ctl);
 }
 
@@ -1879,6 +1882,7 @@ int lapgpu_create(lapgpu_ctx** out, int device) {
   if (const char* ch = getenv("LAPGPU_CHAIN")) c->use_chain = atoi(ch) != 0;
   if (const char* cn = getenv("LAPGPU_CHAIN_NARROW")) c->chain_narrow = atoi(cn);
   if (const char* cm = getenv("LAPGPU_CHAIN_MIN_RUN")) c->chain_min_run = atoi(cm);
+  if (const char* cp = getenv("LAPGPU_CHAIN_POLL_MAX")) c->chain_poll_max = std::max(1, atoi(cp));
   if (const char* tw = getenv("LAPGPU_TAIL_STEPS")) c->max_tail_steps = atoll(tw);
   if (const char* ug = getenv("LAPGPU_NO_GRAPH")) c->use_graph = atoi(ug) == 0;
   if (const char* sw = getenv("LAPGPU_SWITCH_TOL")) c->switch_tol = atof(sw);
@@ -2257,12 +2261,13 @@ int lapgpu_set_sweep_mode(lapgpu_ctx* c, int mode) {
   return 0;
 }
 
-int lapgpu_chain_info(lapgpu_ctx* c, int64_t* runs, int64_t* chained_groups) {
-  if (!c || !runs || !chained_groups) return -1;
+int lapgpu_chain_info(lapgpu_ctx* c, int64_t* runs, int64_t* chained_groups, int* active) {
+  if (!c || !runs || !chained_groups || !active) return -1;
   int64_t g = 0;
   for (const auto& r : c->runs) g += r.second - r.first;
   *runs = (int64_t)c->runs.size();
   *chained_groups = g;
+  *active = chain_on(c) ? 1 : 0;
   c->err.clear();
   return 0;
 }

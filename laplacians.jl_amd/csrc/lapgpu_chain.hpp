@@ -52,9 +52,11 @@ constexpr int kChainTickets = 64;
 constexpr int kChainPasses = 8;
 constexpr int kChainGrid = 512;
 
-// ctl[0]: a wait ran out (sticky until the host clears it); ctl[1 + r]:
-// ticket counter of run r (set to zero before the sweeps)
-constexpr int kChainPollMax = 1 << 18;  // polls of one wait (~0.3 s)
+// ctl[0]: a wait ran out (sticky until the host clears it); ctl[1]: the
+// polls one wait may take (set by the host: kChainPollMax, or fewer to
+// exercise the fallback); ctl[2 + r]: ticket counter of run r (set to
+// zero before the sweeps)
+constexpr int kChainPollMax = 1 << 18;  // ~0.3 s
 
 constexpr unsigned long long kSentD = 0x7ff4dead5e47f00dULL;
 constexpr unsigned int kSentF = 0x7fa5e47fu;
@@ -102,6 +104,7 @@ struct Sent<float> {
 template <int U, typename T>
 __device__ __forceinline__ void chain_poll(T (&xv)[U], const T* const (&addr)[U], unsigned pend,
                                            int* ctl) {
+  const int pmax = ctl[1];
   for (int polls = 1; pend; ++polls) {
     __builtin_amdgcn_s_sleep(LAPGPU_CHAIN_SLEEP);
 #pragma unroll
@@ -110,7 +113,7 @@ __device__ __forceinline__ void chain_poll(T (&xv)[U], const T* const (&addr)[U]
 #pragma unroll
     for (int u = 0; u < U; ++u)
       if ((pend & (1u << u)) && !Sent<T>::is(xv[u])) pend &= ~(1u << u);
-    if (pend && (polls >= kChainPollMax ||
+    if (pend && (polls >= pmax ||
                  ((polls & 63) == 0 &&
                   __hip_atomic_load(ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0))) {
       __hip_atomic_store(ctl, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -222,7 +225,7 @@ __device__ __forceinline__ void chain_heavy(const ChainTicket& tk, int n,
   if (lane == 0) Sent<T>::store(out + row, acc);
 }
 
-// Tickets [t0, t1) of one run (counter ctl[1 + run]):
+// Tickets [t0, t1) of one run (counter ctl[2 + run]):
 //   out[row] = sum_k w[k] * (idx[k] < n ? A[idx[k]] : B[idx[k] - n])  as k_group
 template <typename T, typename TB>
 __global__ void __launch_bounds__(kBlock)
@@ -230,7 +233,7 @@ k_chain(int n, const ChainTicket* __restrict__ tickets, int t0, int t1, int* ctl
         const int* __restrict__ heavy, const int* __restrict__ ptr,
         const int* __restrict__ idx, const T* __restrict__ w, const T* A, const TB* B,
         T* out) {
-  chain_tickets(tickets, t0, t1, ctl + 1 + run, [&](const ChainTicket& tk) {
+  chain_tickets(tickets, t0, t1, ctl + 2 + run, [&](const ChainTicket& tk) {
     switch (tk.G) {
       case 4: chain_light<4>(tk, n, ptr, idx, w, A, B, out, ctl); break;
       case 8: chain_light<8>(tk, n, ptr, idx, w, A, B, out, ctl); break;
@@ -254,7 +257,7 @@ k_chain_prep(int64_t lo, int64_t hi, T* y, T* z, int* ctl, int nctr) {
     z[i] = s;
   }
   if (blockIdx.x == 0)
-    for (int i = threadIdx.x; i < nctr; i += kBlock) ctl[1 + i] = 0;
+    for (int i = threadIdx.x; i < nctr; i += kBlock) ctl[2 + i] = 0;
 }
 
 }  // namespace lapgpu

@@ -119,7 +119,7 @@ def load() -> ctypes.CDLL:
     L.lapgpu_set_group_budget.argtypes = [P, ci, dbl, i64]
     L.lapgpu_set_wait_mode.argtypes = [P, ci]
     L.lapgpu_set_sweep_mode.argtypes = [P, ci]
-    L.lapgpu_chain_info.argtypes = [P, pi64, pi64]
+    L.lapgpu_chain_info.argtypes = [P, pi64, pi64, pint]
     L.lapgpu_factor_info.argtypes = [P, pi64, pi64, pi64, pi64]
     L.lapgpu_last_timings.argtypes = [P, pdbl, pdbl, pdbl, pdbl, pi64]
     L.lapgpu_last_sweep_timings.argtypes = [P, pdbl, pdbl, pdbl, pdbl]
@@ -452,9 +452,10 @@ class Context:
 
     def chain_info(self) -> dict:
         """Chained runs of narrow level groups in the uploaded schedule."""
-        a, b = ctypes.c_int64(), ctypes.c_int64()
-        self._check(self._lib.lapgpu_chain_info(self._h, ctypes.byref(a), ctypes.byref(b)), "lapgpu_chain_info")
-        return {"runs": a.value, "chained_groups": b.value}
+        a, b, on = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+        self._check(self._lib.lapgpu_chain_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(on)),
+                    "lapgpu_chain_info")
+        return {"runs": a.value, "chained_groups": b.value, "active": bool(on.value)}
 
     def factor_info(self) -> dict:
         a, b, c, d = (ctypes.c_int64() for _ in range(4))

@@ -42,9 +42,9 @@ def cases():
     return out
 
 
-def _ctx(la, f, budget, chain, prec="fp64", narrow=None):
+def _ctx(la, f, budget, chain, prec="fp64", narrow=None, env=None):
     # runs of any length (the library chains only long ones by default)
-    env = {"LAPGPU_CHAIN_MIN_RUN": "2"}
+    env = dict(env or {}, LAPGPU_CHAIN_MIN_RUN="2")
     if narrow is not None:
         env["LAPGPU_CHAIN_NARROW"] = str(narrow)
     old = {k: os.environ.get(k) for k in env}
@@ -154,3 +154,42 @@ def test_chain_over_many_iterations(cases):
     (x1, i1), (x0, i0) = out[True], out[False]
     assert i1["its"] == i0["its"] and i1["stop"] == i0["stop"]
     assert np.array_equal(x1, x0)
+
+
+def test_chain_wait_timeout_falls_back(cases):
+    """With one poll per wait (LAPGPU_CHAIN_POLL_MAX=1) a wait that finds a
+    value still pending gives up at once: the launch flags it, every wait
+    after it gives up too (every kernel exits normally), the call is run
+    again with one launch per group and the context stays so.  The answers
+    equal those of a context that launches per group from the start, bit
+    for bit, for solves (host and device vectors) and applies."""
+    rng = np.random.default_rng(76)
+    name, a, la, f, budget, _ = cases[0]
+    n = a.shape[0]
+    b = _rhs(n, rng)
+    ref = _ctx(la, f, budget, False)
+    ctx = _ctx(la, f, budget, True, env={"LAPGPU_CHAIN_POLL_MAX": "1"})
+    try:
+        assert ctx.chain_info()["active"] and ctx.chain_info()["runs"] >= 1
+        xr, ir = ref.solve(b, tol=1e-10, maxits=1000)
+        zr = ref.apply_ldli(b)
+        for _ in range(3):
+            x, i = ctx.solve(b, tol=1e-10, maxits=1000)
+            assert i["its"] == ir["its"] and np.array_equal(x, xr)
+        assert not ctx.chain_info()["active"]
+        assert np.array_equal(ctx.apply_ldli(b), zr)
+        db = _native.DeviceArray.from_host(b)
+        dx = _native.DeviceArray(n)
+        ctx.solve_device(db.address, dx.address, n, tol=1e-10, maxits=1000)
+        assert np.array_equal(dx.download(), xr)
+    finally:
+        ctx.close()
+        ref.close()
+    # a fresh context applies with the waits bounded the same way: the apply
+    # itself falls back
+    ctx = _ctx(la, f, budget, True, env={"LAPGPU_CHAIN_POLL_MAX": "1"})
+    try:
+        for _ in range(3):
+            assert np.array_equal(ctx.apply_ldli(b), zr)
+    finally:
+        ctx.close()
