@@ -169,6 +169,12 @@ struct lapgpu_ctx {
   // runs of 14-34 groups 2-4 % slower
   int chain_min_run = 40;
   int chain_poll_max = kChainPollMax;
+  // multi-RHS level groups: heavy and light rows in one launch for K up to
+  // this many columns (env LAPGPU_MGROUP_MERGE_K).  Measured (1024-thread
+  // workgroups against a 256-thread launch plus a heavy-row launch, 3.4M
+  // expander and 1.3M 3-D grid, FP32-inner LDLinv apply): K = 8 -14/-17 %,
+  // 16 -11/-15 %, 32 -6/-11 %, 64 +4/+11 %.
+  int mgroup_merge_k = 32;
   std::vector<int> group_run;
   std::vector<std::pair<int, int>> runs;
   std::vector<std::pair<int, int>> tkF, tkB;  // ticket ranges per run
@@ -1290,39 +1296,43 @@ int multi_prepare(lapgpu_ctx* c, int K) {
 
 // One level group for K right-hand sides: light rows with S entry slices
 // per row (S bounded by 64/K), rows longer than kMHeavyLen one workgroup
-// each.
+// each — in the same launch (its first nh workgroups) when K <= mgroup_merge_k,
+// else in a launch of their own after the light rows.
+template <int K, int S, int NT, typename T, typename TB>
+void launch_mgroup_s(lapgpu_ctx* c, int lo, int hi, const int* heavy, int nh, const int* ptr,
+                     const int* idx, const T* w, const T* A, const TB* B, T* out) {
+  const int64_t threads = (int64_t)(hi - lo) * K * S;
+  const int lead = NT == kMGroupThreads ? nh : 0;
+  hipLaunchKernelGGL((k_mgroup<K, S, NT, T, TB>), dim3(lead + grid_for(threads, NT)), dim3(NT),
+                     0, c->stream, (int)c->n, lo, hi, heavy, lead, ptr, idx, w, A, B, out,
+                     c->m.flags);
+  if (NT != kMGroupThreads && nh > 0)
+    hipLaunchKernelGGL((k_mgroup_heavy<K, T, TB>), dim3(nh), dim3(kMGroupThreads), 0, c->stream,
+                       (int)c->n, heavy, ptr, idx, w, A, B, out, c->m.flags);
+}
+
 template <int K, typename T, typename TB>
 void launch_mgroup(lapgpu_ctx* c, int lo, int hi, int S, const int* heavy, int nh,
                    const int* ptr, const int* idx, const T* w, const T* A, const TB* B,
                    T* out) {
-  auto& m = c->m;
-  const int n = (int)c->n;
   const int s = std::min(S, kWave / K);
-  const int64_t threads = (int64_t)(hi - lo) * K * s;
-  switch (s) {
-    case 1:
-      hipLaunchKernelGGL((k_mgroup<K, 1, T, TB>), dim3(grid_for(threads, kBlock)), dim3(kBlock),
-                         0, c->stream, n, lo, hi, ptr, idx, w, A, B, out, m.flags);
-      break;
-    case 2:
-      hipLaunchKernelGGL((k_mgroup<K, (K <= 32 ? 2 : 1), T, TB>),
-                         dim3(grid_for(threads, kBlock)), dim3(kBlock), 0, c->stream, n, lo, hi,
-                         ptr, idx, w, A, B, out, m.flags);
-      break;
-    case 4:
-      hipLaunchKernelGGL((k_mgroup<K, (K <= 16 ? 4 : 1), T, TB>),
-                         dim3(grid_for(threads, kBlock)), dim3(kBlock), 0, c->stream, n, lo, hi,
-                         ptr, idx, w, A, B, out, m.flags);
-      break;
-    default:
-      hipLaunchKernelGGL((k_mgroup<K, (K <= 8 ? 8 : 1), T, TB>),
-                         dim3(grid_for(threads, kBlock)), dim3(kBlock), 0, c->stream, n, lo, hi,
-                         ptr, idx, w, A, B, out, m.flags);
-      break;
+  if (K <= c->mgroup_merge_k) {
+    constexpr int NT = kMGroupThreads;
+    switch (s) {
+      case 1: launch_mgroup_s<K, 1, NT>(c, lo, hi, heavy, nh, ptr, idx, w, A, B, out); break;
+      case 2: launch_mgroup_s<K, (K <= 32 ? 2 : 1), NT>(c, lo, hi, heavy, nh, ptr, idx, w, A, B, out); break;
+      case 4: launch_mgroup_s<K, (K <= 16 ? 4 : 1), NT>(c, lo, hi, heavy, nh, ptr, idx, w, A, B, out); break;
+      default: launch_mgroup_s<K, (K <= 8 ? 8 : 1), NT>(c, lo, hi, heavy, nh, ptr, idx, w, A, B, out); break;
+    }
+  } else {
+    constexpr int NT = kBlock;
+    switch (s) {
+      case 1: launch_mgroup_s<K, 1, NT>(c, lo, hi, heavy, nh, ptr, idx, w, A, B, out); break;
+      case 2: launch_mgroup_s<K, (K <= 32 ? 2 : 1), NT>(c, lo, hi, heavy, nh, ptr, idx, w, A, B, out); break;
+      case 4: launch_mgroup_s<K, (K <= 16 ? 4 : 1), NT>(c, lo, hi, heavy, nh, ptr, idx, w, A, B, out); break;
+      default: launch_mgroup_s<K, (K <= 8 ? 8 : 1), NT>(c, lo, hi, heavy, nh, ptr, idx, w, A, B, out); break;
+    }
   }
-  if (nh > 0)
-    hipLaunchKernelGGL((k_mgroup_heavy<K, T, TB>), dim3(nh), dim3(kMHeavyThreads), 0, c->stream,
-                       n, heavy, ptr, idx, w, A, B, out, m.flags);
 }
 
 // z = LDLsolver(rin) without the mean removal for K interleaved right-hand
@@ -1883,6 +1893,7 @@ int lapgpu_create(lapgpu_ctx** out, int device) {
   if (const char* cn = getenv("LAPGPU_CHAIN_NARROW")) c->chain_narrow = atoi(cn);
   if (const char* cm = getenv("LAPGPU_CHAIN_MIN_RUN")) c->chain_min_run = atoi(cm);
   if (const char* cp = getenv("LAPGPU_CHAIN_POLL_MAX")) c->chain_poll_max = std::max(1, atoi(cp));
+  if (const char* mk = getenv("LAPGPU_MGROUP_MERGE_K")) c->mgroup_merge_k = atoi(mk);
   if (const char* tw = getenv("LAPGPU_TAIL_STEPS")) c->max_tail_steps = atoll(tw);
   if (const char* ug = getenv("LAPGPU_NO_GRAPH")) c->use_graph = atoi(ug) == 0;
   if (const char* sw = getenv("LAPGPU_SWITCH_TOL")) c->switch_tol = atof(sw);

@@ -289,11 +289,21 @@ __global__ void k_mscalar_nb(const double* __restrict__ part, DevState* mst, int
 // One level group (composite rows, as k_group) for K right-hand sides:
 //   out[v*K+j] = sum_e w[e] * (idx[e] < n ? A[idx[e]*K+j] : B[(idx[e]-n)*K+j])
 // forward: A = y0, B = r, out = y0; backward: A = yF, B = y0, out = yF.
-// Light rows (at most kMHeavyLen entries): a wave holds 64 / (K S) rows;
-// lane (row r, slice s, rhs j) sums the entries s, s + S, ... of row r, and
-// the S slices are added with shuffles (S = 1: one lane per rhs walks the
-// whole row, indices broadcast, see mrow_dot).  Longer rows are left to
-// k_mgroup_heavy.
+//  * rows longer than kMHeavyLen (heavy[0..nh)): one kMGroupThreads-thread
+//    workgroup each; thread (slice s, rhs j) sums the entries s, s + SL, ...
+//    (SL = kMGroupThreads / K) and the slices are added in a fixed tree
+//    order in LDS (mgroup_heavy_row);
+//  * the light rows lo..hi: a wave holds 64 / (K S) rows; lane (row r,
+//    slice s, rhs j) sums the entries s, s + S, ... of row r, and the S
+//    slices are added with shuffles (S = 1: one lane per rhs walks the
+//    whole row, indices broadcast, see mrow_dot).
+// For K <= 32 both kinds share one launch of kMGroupThreads-thread
+// workgroups, the heavy rows first (a launch costs a few microseconds, and
+// on the expanders most groups have rows of both kinds; 64-column rows ran
+// slower in workgroups of that size, so there the heavy rows keep a launch
+// of their own, k_mgroup_heavy, after one of kBlock-thread workgroups).
+// The sums do not depend on the split: the same bits either way.
+// The stop flag is loaded together with the first index loads.
 // The last (at most three) entries k, k + S, k + 2S < k1 of a slice, added
 // to acc in that order; their loads issued together.
 template <int S, typename T, typename F>
@@ -317,69 +327,16 @@ __device__ __forceinline__ T slice_tail(int k, int k1, const int* __restrict__ i
 }
 
 constexpr int kMHeavyLen = 64;
-constexpr int kMHeavyThreads = 1024;
+constexpr int kMGroupThreads = 1024;
 
-template <int K, int S, typename T, typename TB>
-__global__ void __launch_bounds__(kBlock)
-k_mgroup(int n, int lo, int hi, const int* __restrict__ ptr, const int* __restrict__ idx,
-         const T* __restrict__ w, const T* A, const TB* B, T* out,
-         const int* __restrict__ flags) {
-  if (flags[0]) return;
-  constexpr int R = kWave / (K * S);  // rows per wave
-  const int lane = threadIdx.x % kWave;
-  const int j = lane % K, s = (lane / K) % S, r = lane / (K * S);
-  const int row = lo + ((blockIdx.x * kBlock + threadIdx.x) / kWave) * R + r;
-  auto src = [&](int i) -> T {
-    return i < n ? A[(int64_t)i * K + j] : (T)B[(int64_t)(i - n) * K + j];
-  };
-  if constexpr (S == 1) {
-    if (row >= hi) return;  // whole row groups leave together
-    const int k0 = wave_uniform<K>(ptr[row]), k1 = wave_uniform<K>(ptr[row + 1]);
-    if (k1 - k0 > kMHeavyLen) return;
-    out[(int64_t)row * K + j] = mrow_dot<K, T>(k0, k1, j, r, idx, w, src);
-  } else {
-    T acc = 0;
-    bool mine = false;
-    if (row < hi) {
-      const int k0 = ptr[row], k1 = ptr[row + 1];
-      if (k1 - k0 <= kMHeavyLen) {
-        mine = true;
-        int k = k0 + s;
-        for (; k + 3 * S < k1; k += 4 * S) {
-          const int i0 = idx[k], i1 = idx[k + S], i2 = idx[k + 2 * S], i3 = idx[k + 3 * S];
-          const T w0 = w[k], w1 = w[k + S], w2 = w[k + 2 * S], w3 = w[k + 3 * S];
-          const T v0 = src(i0), v1 = src(i1), v2 = src(i2), v3 = src(i3);
-          acc += w0 * v0;
-          acc += w1 * v1;
-          acc += w2 * v2;
-          acc += w3 * v3;
-        }
-        acc = slice_tail<S>(k, k1, idx, w, src, acc);
-      }
-    }
-#pragma unroll
-    for (int o = K; o < K * S; o <<= 1) acc += __shfl_xor(acc, o, kWave);
-    if (mine && s == 0) out[(int64_t)row * K + j] = acc;
-  }
-}
-
-// The rows of a group longer than kMHeavyLen, one workgroup per row: thread
-// (slice s, rhs j) sums the entries s, s + SL, ... (SL = kMHeavyThreads / K)
-// and the slices are added in a fixed tree order in LDS.
-template <int K, typename T, typename TB>
-__global__ void __launch_bounds__(kMHeavyThreads)
-k_mgroup_heavy(int n, const int* __restrict__ heavy, const int* __restrict__ ptr,
-               const int* __restrict__ idx, const T* __restrict__ w, const T* A, const TB* B,
-               T* out, const int* __restrict__ flags) {
-  if (flags[0]) return;
-  constexpr int SL = kMHeavyThreads / K;
+template <int K, typename T, typename F>
+__device__ __forceinline__ void mgroup_heavy_row(int row, const int* __restrict__ ptr,
+                                                 const int* __restrict__ idx,
+                                                 const T* __restrict__ w, F&& src, T* out) {
+  constexpr int SL = kMGroupThreads / K;
   __shared__ T red[SL][K];
-  const int row = heavy[blockIdx.x];
   const int j = threadIdx.x % K, s = threadIdx.x / K;
   const int k0 = ptr[row], k1 = ptr[row + 1];
-  auto src = [&](int i) -> T {
-    return i < n ? A[(int64_t)i * K + j] : (T)B[(int64_t)(i - n) * K + j];
-  };
   T acc = 0;
   int k = k0 + s;
   for (; k + 3 * SL < k1; k += 4 * SL) {
@@ -400,6 +357,82 @@ k_mgroup_heavy(int n, const int* __restrict__ heavy, const int* __restrict__ ptr
     __syncthreads();
   }
   if (s == 0) out[(int64_t)row * K + j] = red[0][j];
+}
+
+// NT-thread workgroups; the first nh take the heavy rows (NT must then be
+// kMGroupThreads), the rest the light rows.
+template <int K, int S, int NT, typename T, typename TB>
+__global__ void __launch_bounds__(NT)
+k_mgroup(int n, int lo, int hi, const int* __restrict__ heavy, int nh,
+         const int* __restrict__ ptr, const int* __restrict__ idx, const T* __restrict__ w,
+         const T* A, const TB* B, T* out, const int* __restrict__ flags) {
+  const int j = threadIdx.x % K;
+  auto src = [&](int i) -> T {
+    return i < n ? A[(int64_t)i * K + j] : (T)B[(int64_t)(i - n) * K + j];
+  };
+  if constexpr (NT == kMGroupThreads) {
+    if ((int)blockIdx.x < nh) {
+      const int stop = flags[0];
+      const int row = heavy[blockIdx.x];
+      if (stop) return;
+      mgroup_heavy_row<K, T>(row, ptr, idx, w, src, out);
+      return;
+    }
+  }
+  constexpr int R = kWave / (K * S);  // rows per wave
+  const int lane = threadIdx.x % kWave;
+  const int s = (lane / K) % S, r = lane / (K * S);
+  const int row = lo + (((int)blockIdx.x - nh) * NT + (int)threadIdx.x) / kWave * R + r;
+  const int stop = flags[0];
+  int k0 = 0, k1 = 0;
+  if (row < hi) {
+    k0 = ptr[row];
+    k1 = ptr[row + 1];
+  }
+  if (stop) return;
+  if constexpr (S == 1) {
+    if (row >= hi) return;  // whole row groups leave together
+    k0 = wave_uniform<K>(k0);
+    k1 = wave_uniform<K>(k1);
+    if (k1 - k0 > kMHeavyLen) return;
+    out[(int64_t)row * K + j] = mrow_dot<K, T>(k0, k1, j, r, idx, w, src);
+  } else {
+    T acc = 0;
+    bool mine = false;
+    if (row < hi && k1 - k0 <= kMHeavyLen) {
+      mine = true;
+      int k = k0 + s;
+      for (; k + 3 * S < k1; k += 4 * S) {
+        const int i0 = idx[k], i1 = idx[k + S], i2 = idx[k + 2 * S], i3 = idx[k + 3 * S];
+        const T w0 = w[k], w1 = w[k + S], w2 = w[k + 2 * S], w3 = w[k + 3 * S];
+        const T v0 = src(i0), v1 = src(i1), v2 = src(i2), v3 = src(i3);
+        acc += w0 * v0;
+        acc += w1 * v1;
+        acc += w2 * v2;
+        acc += w3 * v3;
+      }
+      acc = slice_tail<S>(k, k1, idx, w, src, acc);
+    }
+#pragma unroll
+    for (int o = K; o < K * S; o <<= 1) acc += __shfl_xor(acc, o, kWave);
+    if (mine && s == 0) out[(int64_t)row * K + j] = acc;
+  }
+}
+
+// The heavy rows of a group in a launch of their own (heavy[blockIdx.x])
+template <int K, typename T, typename TB>
+__global__ void __launch_bounds__(kMGroupThreads)
+k_mgroup_heavy(int n, const int* __restrict__ heavy, const int* __restrict__ ptr,
+               const int* __restrict__ idx, const T* __restrict__ w, const T* A, const TB* B,
+               T* out, const int* __restrict__ flags) {
+  const int j = threadIdx.x % K;
+  auto src = [&](int i) -> T {
+    return i < n ? A[(int64_t)i * K + j] : (T)B[(int64_t)(i - n) * K + j];
+  };
+  const int stop = flags[0];
+  const int row = heavy[blockIdx.x];
+  if (stop) return;
+  mgroup_heavy_row<K, T>(row, ptr, idx, w, src, out);
 }
 
 // ext[i*K+j] = r[(T0+i)*K+j] + the forward in-edges of tail row i from

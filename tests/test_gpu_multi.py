@@ -144,3 +144,44 @@ def test_reupload_after_multi_sizes_workspace_again(gpu_ctx_factory, oracle_mod)
             assert info["stop"][j] == io["stop"] and abs(info["its"][j] - io["its"]) <= 2, (j, info, io)
             assert np.linalg.norm(X[:, j] - xo) <= 1e-4 * np.linalg.norm(xo)
         prev_nt = nt
+
+
+@pytest.mark.parametrize("budget", [None, (16, 4.0, 65536)])
+@pytest.mark.parametrize("k", [12, 64])
+def test_multi_long_composite_rows(budget, k, gpu_ctx_factory, oracle_mod):
+    """Level groups whose composite rows are longer than 64 entries (on this
+    path + matchings graph of 1e5 vertices: thousands of such rows in every
+    group, under the library's budget and under 16 / 4 / 65,536): those rows
+    take one workgroup each in the same launch as the group's other rows.
+    Columns against the oracle, every column on its true residual, FP64 and
+    FP32-inner, and a second solve gives the same bits."""
+    a = graphs.path_matchings(100000, 4, np.random.default_rng(3))
+    la = lapgpu.lap(a)
+    f = lapgpu.build_ldli(a, seed=1)
+    n = la.shape[0]
+    B = _rhs_block(n, k, 40 + k)
+    ctx = lapgpu.Context(0)
+    try:
+        if budget is not None:
+            ctx.set_group_budget(*budget)
+        ctx.upload_lap(la)
+        ctx.upload_ldli(f)
+        assert ctx.group_info()["ngroups"] >= 3
+        X, info = ctx.solve_multi(B, tol=1e-8, maxits=1000, stag_test=5)
+        for j in (0, k // 2, k - 1):
+            xo, io = oracle_mod.approxchol_solve(la, f, B[:, j], tol=1e-8, maxits=1000, stag_test=5)
+            assert info["stop"][j] == io["stop"], (j, info["stop"][j], io)
+            assert abs(info["its"][j] - io["its"]) <= 2, (j, info["its"][j], io)
+            assert np.linalg.norm(X[:, j] - xo) <= 1e-4 * np.linalg.norm(xo), j
+        assert_relres_cols(la, X, B, 1e-8, f, what="long rows fp64")
+        X2, _ = ctx.solve_multi(B, tol=1e-8, maxits=1000, stag_test=5)
+        assert np.array_equal(X, X2)
+        ctx.set_precision(lapgpu._native.FP32_INNER, 20)
+        X32, i32 = ctx.solve_multi(B, tol=1e-8, maxits=1000, stag_test=5)
+        assert all(s == "converged" for s in i32["stop"]), i32
+        rel = np.linalg.norm(la @ X32 - B, axis=0) / np.linalg.norm(B, axis=0)
+        assert np.all(rel <= 1e-8), rel
+        for j in range(k):
+            assert np.linalg.norm(X32[:, j] - X[:, j]) <= 1e-5 * np.linalg.norm(X[:, j]), j
+    finally:
+        ctx.close()
