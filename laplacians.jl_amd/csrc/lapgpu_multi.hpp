@@ -304,30 +304,17 @@ __global__ void k_mscalar_nb(const double* __restrict__ part, DevState* mst, int
 // of their own, k_mgroup_heavy, after one of kBlock-thread workgroups).
 // The sums do not depend on the split: the same bits either way.
 // The stop flag is loaded together with the first index loads.
-// The last (at most three) entries k, k + S, k + 2S < k1 of a slice, added
-// to acc in that order; their loads issued together.
-template <int S, typename T, typename F>
-__device__ __forceinline__ T slice_tail(int k, int k1, const int* __restrict__ idx,
-                                        const T* __restrict__ w, F&& src, T acc) {
-  int i[3];
-  T ww[3], v[3];
-#pragma unroll
-  for (int u = 0; u < 3; ++u)
-    if (k + u * S < k1) {
-      i[u] = idx[k + u * S];
-      ww[u] = w[k + u * S];
-    }
-#pragma unroll
-  for (int u = 0; u < 3; ++u)
-    if (k + u * S < k1) v[u] = src(i[u]);
-#pragma unroll
-  for (int u = 0; u < 3; ++u)
-    if (k + u * S < k1) acc += ww[u] * v[u];
-  return acc;
-}
-
 constexpr int kMHeavyLen = 64;
 constexpr int kMGroupThreads = 1024;
+// entries whose loads a lane of a sliced row issues together (strided_dot):
+// a slice of up to kMSliceU entries waits for one index and one gather
+// latency (the sums are added in entry order whatever the batch).  Measured
+// against batches of 4 (FP32-inner LDLinv apply, 3.4M-vertex expander):
+// K = 8 / 16 / 32 / 64 -9 / -12 / -15 / -16 %; batches of 16 were slower
+// for K <= 32 (registers), faster for the FP32 64-column long rows (-4 %)
+constexpr int kMSliceU = 8;
+template <int K, typename T>
+constexpr int heavy_slice_u() { return K == kWave && sizeof(T) == 4 ? 16 : kMSliceU; }
 
 template <int K, typename T, typename F>
 __device__ __forceinline__ void mgroup_heavy_row(int row, const int* __restrict__ ptr,
@@ -337,19 +324,7 @@ __device__ __forceinline__ void mgroup_heavy_row(int row, const int* __restrict_
   __shared__ T red[SL][K];
   const int j = threadIdx.x % K, s = threadIdx.x / K;
   const int k0 = ptr[row], k1 = ptr[row + 1];
-  T acc = 0;
-  int k = k0 + s;
-  for (; k + 3 * SL < k1; k += 4 * SL) {
-    const int i0 = idx[k], i1 = idx[k + SL], i2 = idx[k + 2 * SL], i3 = idx[k + 3 * SL];
-    const T w0 = w[k], w1 = w[k + SL], w2 = w[k + 2 * SL], w3 = w[k + 3 * SL];
-    const T v0 = src(i0), v1 = src(i1), v2 = src(i2), v3 = src(i3);
-    acc += w0 * v0;
-    acc += w1 * v1;
-    acc += w2 * v2;
-    acc += w3 * v3;
-  }
-  acc = slice_tail<SL>(k, k1, idx, w, src, acc);
-  red[s][j] = acc;
+  red[s][j] = strided_dot<SL, heavy_slice_u<K, T>()>(k0, k1, s, idx, w, src);
   __syncthreads();
 #pragma unroll
   for (int h = SL / 2; h > 0; h >>= 1) {
@@ -401,17 +376,7 @@ k_mgroup(int n, int lo, int hi, const int* __restrict__ heavy, int nh,
     bool mine = false;
     if (row < hi && k1 - k0 <= kMHeavyLen) {
       mine = true;
-      int k = k0 + s;
-      for (; k + 3 * S < k1; k += 4 * S) {
-        const int i0 = idx[k], i1 = idx[k + S], i2 = idx[k + 2 * S], i3 = idx[k + 3 * S];
-        const T w0 = w[k], w1 = w[k + S], w2 = w[k + 2 * S], w3 = w[k + 3 * S];
-        const T v0 = src(i0), v1 = src(i1), v2 = src(i2), v3 = src(i3);
-        acc += w0 * v0;
-        acc += w1 * v1;
-        acc += w2 * v2;
-        acc += w3 * v3;
-      }
-      acc = slice_tail<S>(k, k1, idx, w, src, acc);
+      acc = strided_dot<S, kMSliceU>(k0, k1, s, idx, w, src);
     }
 #pragma unroll
     for (int o = K; o < K * S; o <<= 1) acc += __shfl_xor(acc, o, kWave);

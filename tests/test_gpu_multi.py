@@ -185,3 +185,31 @@ def test_multi_long_composite_rows(budget, k, gpu_ctx_factory, oracle_mod):
             assert np.linalg.norm(X32[:, j] - X[:, j]) <= 1e-5 * np.linalg.norm(X[:, j]), j
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_multi_alternating_k(prec, gpu_ctx_factory):
+    """One context serving blocks of 5, 12, 40, 5, 20 and 12 columns (K = 8,
+    16, 64, 8, 32, 16: the captured iteration is made again whenever K
+    changes) gives every block the bits a fresh context gives it."""
+    a = graphs.path_matchings(30000, 4, np.random.default_rng(8))
+    la = lapgpu.lap(a)
+    f = lapgpu.build_ldli(a, seed=1)
+    n = la.shape[0]
+    B = _rhs_block(n, 40, 12)
+
+    def make():
+        c = gpu_ctx_factory(la, f)
+        if prec == "fp32":
+            c.set_precision(lapgpu._native.FP32_INNER, 20)
+        return c
+
+    ctx = make()
+    fresh = {}
+    for k in (5, 12, 40, 5, 20, 12):
+        X, info = ctx.solve_multi(B[:, :k], tol=1e-8, maxits=1000)
+        if k not in fresh:
+            fresh[k] = make().solve_multi(B[:, :k], tol=1e-8, maxits=1000)
+        Xf, infof = fresh[k]
+        assert np.array_equal(X, Xf), k
+        assert list(info["its"]) == list(infof["its"]), k
