@@ -34,7 +34,7 @@ Other modes (same keys):
   batched    the 256 small grids of the multi workload through the batched
              engine alone (one launch per step).
 
---gpus N > 1 exits: this build measures one GPU.
+--gpus N > 1 exits: the benchmark measures one GPU.
 """
 
 from __future__ import annotations
@@ -945,7 +945,7 @@ def main() -> int:
                     help="plain mode: end with a device-to-device copy of 512 MiB (PMC calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "the benchmark measures one GPU; --gpus must be 1"}))
         return 2
     return {"plain": mode_plain, "multi": mode_multi, "precision": mode_precision, "serve": mode_serve,
             "batched": mode_batched}[args.mode](args)

@@ -90,7 +90,7 @@ def test_factor_seed_determinism_and_variation():
 
 def test_factor_quality_condition_number(oracle_mod):
     """The property behind test/testSolvers.jl:103-107 (condNumber < 20), on
-    this build's own factor: the extreme eigenvalues of LDLsolver * L on the
+    the library's own factor: the extreme eigenvalues of LDLsolver * L on the
     mean-zero subspace have a ratio well below 20."""
     a = graphs.weighted_grid2(14, 13, np.random.default_rng(3), "expN")
     la = lapgpu.lap(a).toarray()
