@@ -316,7 +316,8 @@ def serve_leg(args) -> dict:
                  max_wait=args.max_wait, quick_workers=args.quick_workers,
                  blocking_waits=not args.spin_waits, wg_max_n=args.wg_max_n, keep_ids=sample,
                  new_wg_max_n=args.new_wg_max_n, context_precision=args.serve_precision,
-                 exact_batches=args.exact_batches, resident_budget=args.resident_budget)
+                 exact_batches=args.exact_batches, resident_budget=args.resident_budget,
+                 idle_sleep=args.idle_sleep)
     t0 = time.time()
     for i, (g, f) in enumerate(zip(G, F)):
         assert srv.register(i, g, ldli=f)
@@ -343,6 +344,7 @@ def serve_leg(args) -> dict:
                 srv.solve_log):
         lst.clear()
     ru0 = resource.getrusage(resource.RUSAGE_SELF)
+    loop0 = srv.stats["loop_cpu_s"]
     t0 = time.perf_counter()
     res = srv.run(reqs, progress=lambda d, t, e: log(f"  served {d} of {t} requests at {e:.0f} s"))
     wall = time.perf_counter() - t0
@@ -350,7 +352,9 @@ def serve_leg(args) -> dict:
     cpu_s = (ru1.ru_utime - ru0.ru_utime) + (ru1.ru_stime - ru0.ru_stime)
     s = summary(res, wall)
     paths = {p: sum(o.path == p for o in res) for p in sorted({o.path for o in res})}
-    log(f"serve: {s} paths {paths}; host CPU {cpu_s:.0f} s over {wall:.1f} s")
+    log(f"serve: {s} paths {paths}; host CPU {cpu_s:.0f} s over {wall:.1f} s (dispatch loop "
+        f"{srv.stats['loop_cpu_s'] - loop0:.0f} s, solve workers {sum(e[4] for e in srv.solve_log):.0f} s, "
+        f"factoring {sum(srv.factor_cpu_log):.0f} s, uploads {sum(srv.upload_cpu_log):.0f} s)")
     for pth in paths:
         sel = [(r, o) for r, o in zip(reqs, res) if o.path == pth]
         lat = np.array([o.latency for _, o in sel])
@@ -919,6 +923,8 @@ def main() -> int:
     ap.add_argument("--replica-nnzf", type=int, default=60_000_000,
                     help="serving: registered graphs whose LDLinv has at least this many entries get a second "
                          "context (0: none)")
+    ap.add_argument("--idle-sleep", type=float, default=5e-4,
+                    help="serving: seconds the dispatch loop pauses after a pass that changed nothing")
     ap.add_argument("--exact-batches", action="store_true",
                     help="serving: multi-RHS batches of exactly 8/16/32/64 requests (no padding columns)")
     ap.add_argument("--serve-precision", default="fp32", choices=["fp64", "fp32"],

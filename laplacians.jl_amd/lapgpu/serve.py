@@ -156,8 +156,14 @@ class Server:
                  blocking_waits: bool = True, keep_ids: set | None = None,
                  new_wg_max_n: int | None = 200_000, cache_dir: str | None = None,
                  context_precision: str = "fp64", exact_batches: bool = False,
-                 replica_batch: int = 16):
+                 replica_batch: int = 16, idle_sleep: float = 5e-4):
         self.device = device
+        # the dispatch loop's pause after a pass that changed nothing: it
+        # holds the interpreter lock while it scans, which the solve workers
+        # and the factoring threads need between their native calls
+        # (measured on the serving stream: 0.5 ms against 50 us, host CPU
+        # 389-405 against 444 core-seconds, throughput the same within 2 %)
+        self.idle_sleep = idle_sleep
         # factors and context plans of graphs seen before (lapgpu.cache):
         # a graph that comes back is admitted without factoring or planning
         self.cache = GraphCache(cache_dir) if cache_dir else None
@@ -702,7 +708,7 @@ class Server:
                 progress(len(results), total, time.perf_counter() - t0)
                 next_progress += progress_every
             if not progressed:
-                time.sleep(50e-6)
+                time.sleep(self.idle_sleep)
         self.stats["loop_cpu_s"] += time.thread_time() - c0
         return [results[r.rid] for r in requests]
 
